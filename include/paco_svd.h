@@ -1,0 +1,117 @@
+/*
+ * paco_svd.h -- C ABI of the MI355X (gfx950) SVD engine, libpacosvd.so.
+ *
+ * Drop-in for the two device entry points of DEAVEN-MITCH/paco-svd and their
+ * global-memory buffer layouts (row-major FP32, M >= N; sizes as main.cpp:33-43):
+ *   a[M*N] u[M*M] vt[N*N] d[N] e[N-1] tauq[N] taup[N-1] qt[2*N*N] wt[2*N*N] idx[16*N].
+ * All pointers passed to the entry points are device pointers; every entry point is
+ * stream-ordered and asynchronous (no host synchronisation inside), so a caller may
+ * capture it in a hipGraph.  Return value: 0 on success, otherwise a hipError_t code
+ * (launch failure) or one of the PACO_E* codes below (argument error).
+ *
+ * Deviation from the reference: the reference passes a 40*32-byte user workspace
+ * (main.cpp:46-48).  These entry points need the workspace sized by
+ * paco_workspace_size(M, N) (block reflector factors, secular vectors, merge states).
+ * Passing NULL makes the library use an internal buffer it owns (grown on demand,
+ * never freed before exit); that convenience allocates on first use.
+ */
+#ifndef PACO_SVD_H
+#define PACO_SVD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PACO_EINVAL (-1)   /* bad argument (M < N, N < 1, null pointer, size over the limits) */
+#define PACO_ENOMEM (-2)   /* an internal buffer could not be allocated */
+#define PACO_MAX_N 7680    /* D&C deflation stage keeps one merge in LDS (20 bytes per row) */
+
+/* Entry point (a): Golub-Kahan upper bidiagonalization, A = Q * B * P^T.
+ * Replaces the kernel upper_bidiagonalization (upper_bidiagonalization.cpp:967-999),
+ * launched at main.cpp:126-127 as ACLRT_LAUNCH_KERNEL(upper_bidiagonalization)
+ * (40, stream, M, N, A, U, Vt, D, E, Tauq, Taup, workspace).
+ * Out: a holds d/e on the band and the Householder vectors below / right of it
+ * (unit heads implicit), d[N], e[N-1], tauq[N], taup[N-1], u = Q (M x M),
+ * vt = P^T (N x N).  blockDim is accepted and ignored (the launch shape is chosen
+ * by the library). */
+int paco_upper_bidiagonalization(unsigned blockDim, void *stream, int M, int N, float *a, float *u, float *vt,
+                                 float *d, float *e, float *tauq, float *taup, void *workspace);
+
+/* Entry point (b): bidiagonal divide and conquer, B = U_B * S * V_B^T, followed by the
+ * back-transforms u[:, :N] <- u[:, :N] U_B and vt <- V_B^T vt.
+ * Replaces the kernel svd_DC (svd.cpp:2105-2118), launched at main.cpp:157-158 as
+ * ACLRT_LAUNCH_KERNEL(svd_DC)(1, stream, M, N, A, U, Vt, D, E, Qt, Wt, idx, workspace,
+ * Tiling).  In: the outputs of entry point (a).  Out: d = sigma (descending),
+ * u = left singular vectors (columns N..M-1 keep Q's), vt = right singular vectors
+ * (rows).  a, qt, wt and idx are scratch.  tiling_dev is the device copy of the blob
+ * written by GenerateTiling (its leaf list is read on the device); NULL derives the
+ * same partition from N. */
+int paco_svd_DC(unsigned blockDim, void *stream, int M, int N, float *a, float *u, float *vt, float *d, float *e,
+                float *qt, float *wt, uint32_t *idx, void *workspace, const void *tiling_dev);
+
+/* Host tiling (svd_tiling.cpp:67-80 getTilingSize, :83-130 GenerateTiling): size in
+ * bytes of the D&C tiling blob for N columns, and the blob itself (header + leaf list
+ * of the reference's 3-column partition). */
+int getTilingSize(int N, int svdBlockNum);
+void GenerateTiling(int N, int svdBlockNum, uint8_t *tiling_host);
+
+/* Bytes of device workspace the two entry points need for an M x N problem.
+ * No counterpart in the reference (which needs only 40*32 bytes for its barriers). */
+size_t paco_workspace_size(int M, int N);
+
+/* Arithmetic of the GEMMs (merge products, back-transforms):
+ * 0 = FP32 on the matrix cores (default), 1 = BF16x3 compensated split,
+ * 2 = FP32 vector FMA.  FP64 is used for secular roots, deflation and Loewner z in
+ * every mode.  Returns the previous mode.  No counterpart in the reference. */
+int paco_set_precision(int mode);
+
+/* ---- whole SVDs (no counterpart in the reference; they chain (a) and (b)) ---- */
+
+/* One SVD job: device pointers.  A (M x N, read-only), S (N), U (M x M), Vt (N x N). */
+typedef struct {
+    int M, N;
+    const float *A;
+    float *S;
+    float *U;
+    float *Vt;
+} paco_svd_job;
+
+/* Plain path: every job runs through the two entry points, one matrix per launch
+ * sequence (the baseline the batched path is measured against). */
+int paco_svd_plain(const paco_svd_job *jobs, int count, void *stream);
+
+/* Batched path: all jobs in one sequence of launches, each phase one launch over
+ * every matrix (tile shapes per size bucket from the autotune table).  jobs is a host
+ * array.  Scratch comes from a library pool sized by paco_batch_workspace_size. */
+int paco_svd_batched(const paco_svd_job *jobs, int count, void *stream);
+
+/* Same as paco_svd_batched, but the sequence is recorded into a hipGraph on first use
+ * of a batch signature (shapes and pointers) and replayed afterwards. */
+int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream);
+
+/* Device bytes the batched path needs for these shapes (scratch + plan). */
+size_t paco_batch_workspace_size(const int *M, const int *N, int count);
+
+/* Status word per matrix of the last batched call on this thread: LAPACK-style info
+ * (0 ok, > 0 secular iterations that hit the limit).  Copies min(count, last) values
+ * (synchronises the stream). Returns the count copied. */
+int paco_last_info(int *info_host, int count, void *stream);
+
+/* C = alpha * op(A) * op(B) + beta * C on one stream with the engine's GEMM tile
+ * (precision mode of paco_set_precision); row-major, op = transpose when trans != 0.
+ * No counterpart in the reference (its products go through the Ascend Matmul API,
+ * svd.cpp:78-80); exported so the tile can be checked on its own. */
+int paco_gemm_f32(void *stream, int transA, int transB, int m, int n, int k, float alpha, const float *A, int lda,
+                  const float *B, int ldb, float beta, float *C, int ldc);
+
+/* Release the library's pools (device memory, pinned staging, graphs). */
+void paco_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PACO_SVD_H */

@@ -1,0 +1,427 @@
+// Golub-Kahan upper bidiagonalization and formation of Q, P^T on gfx950.
+//
+// Reference: upper_bidiagonalization.cpp (Process :83-120, Householder :182-283,
+// rank-1 applications :426-563, GetUVt :774-893).  The reference streams one
+// reflector at a time through 40 vector cores with 4 grid barriers per column; here
+//  * bidiag_wg_kernel: one workgroup owns one matrix and fuses, per column, the left
+//    reflector's GEMV with the right reflector's GEMV + both rank-1 updates, so the
+//    trailing matrix is read twice per column (and written once) and no grid barrier
+//    is needed.  Used for the batched path and small matrices.
+//  * Q and P are formed by blocked WY (32 reflectors per block): T factors per block,
+//    then one launch in which each workgroup owns a column strip of the target and
+//    applies all blocks to it (strips are independent), replacing the reference's
+//    N rank-1 passes with barriers.
+#include "paco_common.hpp"
+#include "paco_kernels.hpp"
+
+namespace paco {
+
+#define BT 256
+
+__device__ inline float block_sum(float v, float *red)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    float r = 0.f;
+#pragma unroll
+    for (int w = 0; w < BT / 64; ++w) r += red[w];
+    return r;
+}
+
+// Reference sign convention (upper_bidiagonalization.cpp:182-218): beta = +mu unless
+// sigma == 0 (then tau = 0, beta = x1 signed).  inv = 1 / v1.
+__device__ inline void house_params(float x1, float sigma, float *beta, float *tau, float *inv)
+{
+    if (sigma == 0.f) {
+        *tau = 0.f;
+        *beta = x1;
+        *inv = 1.f;
+        return;
+    }
+    float mu = sqrtf(x1 * x1 + sigma);
+    float v1 = (x1 <= 0.f) ? x1 - mu : -sigma / (x1 + mu);
+    float v1sq = v1 * v1;
+    *tau = 2.f * v1sq / (sigma + v1sq);
+    *inv = 1.f / v1;
+    *beta = mu;
+}
+
+// ---------------------------------------------------------------------------
+// One workgroup per matrix.  LDS: vbuf[M] (current column reflector), wbuf[N]
+// (tau_q * v^T A), ubuf[N] (current row reflector), red[BT].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *order)
+{
+    const int pid = order ? order[blockIdx.x] : blockIdx.x;
+    const ProbDesc P = pl.get(pid);
+    const int M = P.M, N = P.N;
+    float *__restrict__ A = P.a;
+    extern __shared__ float sm[];
+    float *vbuf = sm, *wbuf = vbuf + M, *ubuf = wbuf + N, *red = ubuf + N;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+
+    for (int i = 0; i < N; ++i) {
+        // ---- column reflector on A[i:, i]
+        float part = 0.f;
+        for (int r = i + 1 + tid; r < M; r += BT) {
+            float x = A[(size_t)r * N + i];
+            vbuf[r] = x;
+            part += x * x;
+        }
+        const float sigma = block_sum(part, red);
+        const float x1 = A[(size_t)i * N + i];
+        float tau, beta, inv;
+        house_params(x1, sigma, &beta, &tau, &inv);
+        if (tau != 0.f) {
+            for (int r = i + 1 + tid; r < M; r += BT) {
+                float v = vbuf[r] * inv;
+                vbuf[r] = v;
+                A[(size_t)r * N + i] = v;
+            }
+        }
+        if (tid == 0) {
+            A[(size_t)i * N + i] = beta;
+            P.d[i] = beta;
+            P.tauq[i] = tau;
+            vbuf[i] = 1.f;
+        }
+        __syncthreads();
+        if (i == N - 1) break;
+
+        // ---- w_j = tau * sum_{r>=i} v_r A[r][j], j in (i, N)
+        const int nc = N - i - 1;
+        if (tau == 0.f) {
+            for (int j = i + 1 + tid; j < N; j += BT) wbuf[j] = 0.f;
+        } else if (nc >= BT) {
+            for (int j = i + 1 + tid; j < N; j += BT) {
+                float acc = 0.f;
+                for (int r = i; r < M; ++r) acc += vbuf[r] * A[(size_t)r * N + j];
+                wbuf[j] = tau * acc;
+            }
+        } else {
+            int tc = 1;
+            while (tc < nc) tc <<= 1;
+            const int rg = BT / tc, g = tid / tc, c = tid % tc;
+            float acc = 0.f;
+            if (c < nc) {
+                const int j = i + 1 + c;
+                for (int r = i + g; r < M; r += rg) acc += vbuf[r] * A[(size_t)r * N + j];
+            }
+            red[tid] = acc;
+            __syncthreads();
+            if (g == 0 && c < nc) {
+                float s = 0.f;
+                for (int q = 0; q < rg; ++q) s += red[q * tc + c];
+                wbuf[i + 1 + c] = tau * s;
+            }
+        }
+        __syncthreads();
+
+        // ---- row i: apply left reflector, then row reflector on A[i, i+1:]
+        part = 0.f;
+        for (int j = i + 1 + tid; j < N; j += BT) {
+            float x = A[(size_t)i * N + j] - wbuf[j];
+            ubuf[j] = x;
+            if (j > i + 1) part += x * x;
+        }
+        const float sigma2 = block_sum(part, red);
+        const float y1 = ubuf[i + 1];
+        float tau2, beta2, inv2;
+        house_params(y1, sigma2, &beta2, &tau2, &inv2);
+        __syncthreads();
+        for (int j = i + 1 + tid; j < N; j += BT) {
+            if (j == i + 1) {
+                A[(size_t)i * N + j] = beta2;
+                ubuf[j] = 1.f;
+            } else {
+                float uval = (tau2 != 0.f) ? ubuf[j] * inv2 : ubuf[j];
+                ubuf[j] = uval;
+                A[(size_t)i * N + j] = uval;
+            }
+        }
+        if (tid == 0) {
+            P.e[i] = beta2;
+            P.taup[i] = tau2;
+        }
+        __syncthreads();
+
+        // ---- rows r > i: left update (v_r, w), then right update (u, tau2); one wave per row
+        const bool in_regs = nc <= 8 * 64;
+        for (int r = i + 1 + wv; r < M; r += BT / 64) {
+            const float vr = (tau != 0.f) ? vbuf[r] : 0.f;
+            float *row = A + (size_t)r * N;
+            float acc = 0.f;
+            float regs[8];
+            if (in_regs) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int j = i + 1 + lane + 64 * c;
+                    regs[c] = 0.f;
+                    if (j < N) {
+                        float x = row[j] - vr * wbuf[j];
+                        regs[c] = x;
+                        acc += x * ubuf[j];
+                    }
+                }
+            } else {
+                for (int j = i + 1 + lane; j < N; j += 64) {
+                    float x = row[j] - vr * wbuf[j];
+                    row[j] = x;
+                    acc += x * ubuf[j];
+                }
+            }
+            acc = wave_sum(acc) * tau2;
+            if (in_regs) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int j = i + 1 + lane + 64 * c;
+                    if (j < N) row[j] = regs[c] - acc * ubuf[j];
+                }
+            } else {
+                for (int j = i + 1 + lane; j < N; j += 64) row[j] = row[j] - acc * ubuf[j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reflector accessor for blocked formation.  side 0 (Q): reflector i acts on rows
+// i..M-1, unit at row i, tail A[r][i].  side 1 (P): reflector i (0..N-2) acts on
+// coordinates i+1..N-1, unit at i+1, tail A[i][c].  Length L = M or N.
+// ---------------------------------------------------------------------------
+__device__ inline float refl_elem(const ProbDesc &P, int side, int i, int r)
+{
+    const int unit = i + side;
+    if (r < unit) return 0.f;
+    if (r == unit) return 1.f;
+    return side == 0 ? P.a[(size_t)r * P.N + i] : P.a[(size_t)i * P.N + r];
+}
+__device__ inline int refl_count(const ProbDesc &P, int side) { return side == 0 ? P.N : P.N - 1; }
+__device__ inline int refl_len(const ProbDesc &P, int side) { return side == 0 ? P.M : P.N; }
+__device__ inline float refl_tau(const ProbDesc &P, int side, int i) { return side == 0 ? P.tauq[i] : P.taup[i]; }
+
+// T factor (forward, columnwise; H_b ... H_{b+nb-1} = I - V T V^T) for every block of
+// 32 reflectors.  One workgroup per (problem, block); grid mapping through offsets.
+__global__ __launch_bounds__(BT) void tfactor_kernel(ProbList pl, const int *blk_off, int side)
+{
+    const int pid = find_task(blk_off, pl.count, blockIdx.x);
+    const int b = blockIdx.x - off_at(blk_off, pid);
+    const ProbDesc P = pl.get(pid);
+    const int nref = refl_count(P, side), L = refl_len(P, side);
+    const int i0 = b * 32, nb = min(32, nref - i0);
+    if (nb <= 0) return;
+    __shared__ float Vs[64][33];
+    __shared__ float G[32][33];
+    __shared__ float T[32][33];
+    const int tid = threadIdx.x;
+    float g[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int r0 = i0 + side; r0 < L; r0 += 64) {
+        for (int q = tid; q < 64 * 32; q += BT) {
+            const int rr = q / 32, t = q % 32, r = r0 + rr;
+            Vs[rr][t] = (t < nb && r < L) ? refl_elem(P, side, i0 + t, r) : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e4 = 0; e4 < 4; ++e4) {
+            const int q = tid + e4 * BT, s = q / 32, t = q % 32;
+            float acc = 0.f;
+            for (int rr = 0; rr < 64; ++rr) acc += Vs[rr][s] * Vs[rr][t];
+            g[e4] += acc;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int e4 = 0; e4 < 4; ++e4) {
+        const int q = tid + e4 * BT;
+        G[q / 32][q % 32] = g[e4];
+        T[q / 32][q % 32] = 0.f;
+    }
+    __syncthreads();
+    // T[0:t, t] = -tau_t * T[0:t, 0:t] * G[0:t, t]; every thread takes part in the barriers
+    for (int t = 0; t < nb; ++t) {
+        const float tt = refl_tau(P, side, i0 + t);
+        float acc = 0.f;
+        if (tid < t)
+            for (int q = tid; q < t; ++q) acc += T[tid][q] * G[q][t];
+        __syncthreads();
+        if (tid < t) T[tid][t] = -tt * acc;
+        if (tid == t) T[t][t] = tt;
+        __syncthreads();
+    }
+    float *out = (side == 0 ? P.tq : P.tp) + (size_t)b * 1024;
+    for (int q = tid; q < 1024; q += BT) out[q] = T[q / 32][q % 32];
+}
+
+// X <- (H_0 ... H_{n-1}) X, X initially the identity: X = Q (side 0, X = u, M x M) or
+// X = P (side 1, X = vt, N x N, transposed afterwards).  One workgroup per strip of
+// 64 columns; 4 row groups; blocks applied in descending order.
+__global__ __launch_bounds__(BT) void wy_apply_kernel(ProbList pl, const int *strip_off, int side)
+{
+    const int pid = find_task(strip_off, pl.count, blockIdx.x);
+    const int strip = blockIdx.x - off_at(strip_off, pid);
+    const ProbDesc P = pl.get(pid);
+    const int L = refl_len(P, side), nref = refl_count(P, side);
+    float *X = side == 0 ? P.u : P.vt;
+    const int ldx = L;
+    const int col = strip * 64 + (threadIdx.x & 63), grp = threadIdx.x >> 6;
+    const bool colok = col < L;
+    __shared__ float Vc[64][33];
+    __shared__ float Ts[32][33];
+    __shared__ float Wr[4][32][65];
+    const float *Tbase = side == 0 ? P.tq : P.tp;
+    const int nblk = (nref + 31) / 32;
+    for (int b = nblk - 1; b >= 0; --b) {
+        const int i0 = b * 32, nb = min(32, nref - i0);
+        const int rfirst = i0 + side;
+        if (strip * 64 + 63 < i0 + side) continue; // every column of the strip is untouched
+        for (int q = threadIdx.x; q < 1024; q += BT) Ts[q / 32][q % 32] = Tbase[(size_t)b * 1024 + q];
+        float w[32];
+#pragma unroll
+        for (int t = 0; t < 32; ++t) w[t] = 0.f;
+        for (int r0 = rfirst; r0 < L; r0 += 64) {
+            __syncthreads();
+            for (int q = threadIdx.x; q < 64 * 32; q += BT) {
+                const int rr = q / 32, t = q % 32, r = r0 + rr;
+                Vc[rr][t] = (t < nb && r < L) ? refl_elem(P, side, i0 + t, r) : 0.f;
+            }
+            __syncthreads();
+            if (colok) {
+                for (int rr = grp; rr < 64 && r0 + rr < L; rr += 4) {
+                    const float x = X[(size_t)(r0 + rr) * ldx + col];
+#pragma unroll
+                    for (int t = 0; t < 32; ++t) w[t] += Vc[rr][t] * x;
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 32; ++t) Wr[grp][t][threadIdx.x & 63] = w[t];
+        __syncthreads();
+        // W = sum over groups; W2 = T W (T upper triangular)
+        float wsum[32];
+#pragma unroll
+        for (int t = 0; t < 32; ++t)
+            wsum[t] = Wr[0][t][threadIdx.x & 63] + Wr[1][t][threadIdx.x & 63] + Wr[2][t][threadIdx.x & 63] +
+                      Wr[3][t][threadIdx.x & 63];
+        float w2[32];
+#pragma unroll
+        for (int s = 0; s < 32; ++s) {
+            float acc = 0.f;
+#pragma unroll
+            for (int t = 0; t < 32; ++t)
+                if (t >= s) acc += Ts[s][t] * wsum[t];
+            w2[s] = acc;
+        }
+        for (int r0 = rfirst; r0 < L; r0 += 64) {
+            __syncthreads();
+            for (int q = threadIdx.x; q < 64 * 32; q += BT) {
+                const int rr = q / 32, t = q % 32, r = r0 + rr;
+                Vc[rr][t] = (t < nb && r < L) ? refl_elem(P, side, i0 + t, r) : 0.f;
+            }
+            __syncthreads();
+            if (colok) {
+                for (int rr = grp; rr < 64 && r0 + rr < L; rr += 4) {
+                    float acc = 0.f;
+#pragma unroll
+                    for (int t = 0; t < 32; ++t) acc += Vc[rr][t] * w2[t];
+                    X[(size_t)(r0 + rr) * ldx + col] -= acc;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// X = identity (L x L), grid over (problem, row) through offsets.
+__global__ __launch_bounds__(BT) void identity_kernel(ProbList pl, const int *row_off, int side)
+{
+    const int pid = find_task(row_off, pl.count, blockIdx.x);
+    const int r = blockIdx.x - off_at(row_off, pid);
+    const ProbDesc P = pl.get(pid);
+    const int L = side == 0 ? P.M : P.N;
+    float *X = side == 0 ? P.u : P.vt;
+    if (r >= L) return;
+    for (int c = threadIdx.x; c < L; c += BT) X[(size_t)r * L + c] = (c == r) ? 1.f : 0.f;
+}
+
+// In-place transpose of vt (N x N) by 32x32 tile pairs; grid over (problem, tile pair).
+__global__ __launch_bounds__(BT) void transpose_vt_kernel(ProbList pl, const int *pair_off)
+{
+    const int pid = find_task(pair_off, pl.count, blockIdx.x);
+    int q = blockIdx.x - off_at(pair_off, pid);
+    const ProbDesc P = pl.get(pid);
+    const int N = P.N, nt = (N + 31) / 32;
+    int bi = 0;
+    while (q >= nt - bi) {
+        q -= nt - bi;
+        ++bi;
+    }
+    const int bj = bi + q;
+    __shared__ float t1[32][33], t2[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 8 rows per pass
+    float *X = P.vt;
+    for (int y = ty; y < 32; y += 8) {
+        int r = bi * 32 + y, c = bj * 32 + tx;
+        t1[y][tx] = (r < N && c < N) ? X[(size_t)r * N + c] : 0.f;
+        r = bj * 32 + y;
+        c = bi * 32 + tx;
+        t2[y][tx] = (r < N && c < N) ? X[(size_t)r * N + c] : 0.f;
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+        int r = bi * 32 + y, c = bj * 32 + tx;
+        if (r < N && c < N) X[(size_t)r * N + c] = t2[tx][y];
+        r = bj * 32 + y;
+        c = bi * 32 + tx;
+        if (r < N && c < N) X[(size_t)r * N + c] = t1[tx][y];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+size_t bidiag_wg_lds_bytes(int M, int N) { return sizeof(float) * ((size_t)M + 2 * (size_t)N + BT); }
+
+hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)bidiag_wg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL(bidiag_wg_kernel, dim3(count), dim3(BT), lds, s, pl, order);
+    return hipGetLastError();
+}
+
+hipError_t launch_tfactor(ProbList pl, const int *blk_off, int nblocks, int side, hipStream_t s)
+{
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tfactor_kernel, dim3(nblocks), dim3(BT), 0, s, pl, blk_off, side);
+    return hipGetLastError();
+}
+
+hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int side, hipStream_t s)
+{
+    if (nstrips <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wy_apply_kernel, dim3(nstrips), dim3(BT), 0, s, pl, strip_off, side);
+    return hipGetLastError();
+}
+
+hipError_t launch_identity(ProbList pl, const int *row_off, int nrows, int side, hipStream_t s)
+{
+    if (nrows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(identity_kernel, dim3(nrows), dim3(BT), 0, s, pl, row_off, side);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hipStream_t s)
+{
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(transpose_vt_kernel, dim3(npairs), dim3(BT), 0, s, pl, pair_off);
+    return hipGetLastError();
+}
+
+} // namespace paco

@@ -1,0 +1,785 @@
+// Bidiagonal divide and conquer on gfx950.
+//
+// Reference: svd.cpp (BDC::Process :134-195, MergeSubMatrix :225-323, Deflation
+// :325-975, SecularEquationSolver :977-1336, MergeSubMatrix_step2 :1337-1594, leaves
+// :1596-1864).  The reference runs every merge serially on one vector core.  Here the
+// merge tree is processed level by level; every phase of a level is one launch over
+// all merges of that level (and, in the batched path, of every matrix):
+//   leaves    one thread per leaf, FP64 one-sided Jacobi
+//   deflate   one workgroup per merge: parallel merge-path ranks, scan, compaction
+//   gather    one thread per basis column: gathered bases + deflation rotations
+//   secular   one wavefront per root, FP64 safeguarded rational iteration
+//   lowner    one thread per pole: FP64 Gu-Eisenstat z and output positions
+//   vectors   one wavefront per root: FP64 secular vectors, stored FP32
+//   gemm      grouped MFMA GEMM (gemm.hip) for left and right vectors
+//   finalize  deflated rows, null vector and sigma written in descending order
+// Node storage (as svd.cpp:87-133): d[c0..] sigma of the node (descending), qt rows
+// c0.. = left vectors, wt rows c0.. = right vectors, confined to the node's diagonal
+// block; qt+N*N and wt+N*N hold the gathered bases of the merge in flight.
+#include <float.h>
+
+#include "gemm_tile.hpp"
+#include "paco_kernels.hpp"
+
+namespace paco {
+
+#define DT 256
+#define SRC_POLE0 0x7fffffff
+
+// ---------------------------------------------------------------------------
+// leaves (replaces compute_2x3/2x2/1x2/1x1_svd, svd.cpp:1627-1864)
+// ---------------------------------------------------------------------------
+__device__ void leaf_svd(int n, int m, const double *dd, const double *ee, double *sigma, double *U, double *V)
+{
+    double W[2][3] = {{0, 0, 0}, {0, 0, 0}}, J[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int r = 0; r < n; r++) {
+        W[r][r] = dd[r];
+        if (r + 1 < m) W[r][r + 1] = ee[r];
+    }
+    for (int sweep = 0; sweep < 30; sweep++) {
+        int rotated = 0;
+        for (int p = 0; p < m; p++)
+            for (int q = p + 1; q < m; q++) {
+                double app = 0, aqq = 0, apq = 0;
+                for (int r = 0; r < n; r++) {
+                    app += W[r][p] * W[r][p];
+                    aqq += W[r][q] * W[r][q];
+                    apq += W[r][p] * W[r][q];
+                }
+                if (apq == 0.0 || fabs(apq) <= 1e-17 * sqrt(app * aqq)) continue;
+                rotated = 1;
+                double zeta = (aqq - app) / (2.0 * apq);
+                double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                for (int r = 0; r < n; r++) {
+                    double wp = W[r][p], wq = W[r][q];
+                    W[r][p] = c * wp - s * wq;
+                    W[r][q] = s * wp + c * wq;
+                }
+                for (int r = 0; r < m; r++) {
+                    double jp = J[r][p], jq = J[r][q];
+                    J[r][p] = c * jp - s * jq;
+                    J[r][q] = s * jp + c * jq;
+                }
+            }
+        if (!rotated) break;
+    }
+    double nrm[3] = {0, 0, 0};
+    int ord[3] = {0, 1, 2};
+    for (int c = 0; c < m; c++) {
+        double s = 0;
+        for (int r = 0; r < n; r++) s += W[r][c] * W[r][c];
+        nrm[c] = sqrt(s);
+    }
+    for (int a = 0; a < m; a++)
+        for (int b = a + 1; b < m; b++)
+            if (nrm[ord[b]] > nrm[ord[a]]) {
+                int t = ord[a];
+                ord[a] = ord[b];
+                ord[b] = t;
+            }
+    for (int i = 0; i < m; i++)
+        for (int r = 0; r < m; r++) V[r * m + i] = J[r][ord[i]];
+    for (int i = 0; i < n; i++) {
+        double s = nrm[ord[i]];
+        sigma[i] = s;
+        if (s > 0.0) {
+            for (int r = 0; r < n; r++) U[r * n + i] = W[r][ord[i]] / s;
+        } else if (i == 0) {
+            for (int r = 0; r < n; r++) U[r * n + i] = (r == 0) ? 1.0 : 0.0;
+        } else {
+            U[0 * n + 1] = -U[1 * n + 0];
+            U[1 * n + 1] = U[0 * n + 0];
+        }
+    }
+}
+
+__global__ __launch_bounds__(DT) void dc_leaf_kernel(ProbList pl, const int *leaf_off, const uint32_t *blob_leaves,
+                                                     int nleaf)
+{
+    const int l = blockIdx.x * DT + threadIdx.x;
+    if (l >= nleaf) return;
+    int prob, c0, c1;
+    if (blob_leaves) {
+        prob = 0;
+        c0 = (int)blob_leaves[2 * l];
+        c1 = (int)blob_leaves[2 * l + 1];
+    } else {
+        prob = find_task(leaf_off, pl.count, l);
+        const int ll = l - off_at(leaf_off, prob), NN = pl.get(prob).N;
+        c0 = leaf_start(NN, ll);
+        c1 = leaf_start(NN, ll + 1);
+    }
+    const ProbDesc P = pl.get(prob);
+    const int N = P.N, m = c1 - c0, n = (c1 == N) ? m : m - 1;
+    if (m < 1 || m > 3 || n < 1) return;
+    double dd[2] = {0, 0}, ee[2] = {0, 0}, sig[2], U[4], V[9];
+    for (int r = 0; r < n; r++) {
+        dd[r] = P.d[c0 + r];
+        if (r + 1 < m) ee[r] = P.e[c0 + r];
+    }
+    leaf_svd(n, m, dd, ee, sig, U, V);
+    for (int i = 0; i < n; i++) {
+        P.d[c0 + i] = (float)sig[i];
+        for (int r = 0; r < n; r++) P.qt[(size_t)(c0 + i) * N + c0 + r] = (float)U[r * n + i];
+    }
+    for (int i = 0; i < m; i++)
+        for (int r = 0; r < m; r++) P.wt[(size_t)(c0 + i) * N + c0 + r] = (float)V[r * m + i];
+}
+
+// Merge t of a level: owning problem and columns (device-derived schedule).
+struct MergeCtx {
+    int prob, c0, c1, c2;
+    MergeState *S;
+};
+__device__ inline MergeCtx merge_ctx(const ProbList &pl, const LevelArgs &la, int t)
+{
+    MergeCtx c;
+    c.prob = find_task(la.task_off, pl.count, t);
+    const int j = t - off_at(la.task_off, c.prob);
+    merge_cols(pl.get(c.prob).N, la.r, j, c.c0, c.c1, c.c2);
+    c.S = la.states + t;
+    return c;
+}
+__host__ __device__ inline int strips_per_task(const LevelArgs &la) { return (la.stride + DT - 1) / DT; }
+
+// ---------------------------------------------------------------------------
+// deflate: one workgroup per merge (Deflation, svd.cpp:325-975; dlasd2 structure)
+// dynamic LDS: pd[n] (double), pz[n] (double), flag[n] (int), scan[DT+1] (int)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DT) void dc_deflate_kernel(ProbList pl, LevelArgs la)
+{
+    const MergeCtx T = merge_ctx(pl, la, blockIdx.x);
+    const ProbDesc P = pl.get(T.prob);
+    const int N = P.N, c0 = T.c0, c1 = T.c1, c2 = T.c2;
+    const int n1 = c1 - c0 - 1, m1 = c1 - c0, m2 = c2 - c1, n2 = (c2 == N) ? m2 : m2 - 1;
+    const int n = n1 + 1 + n2, m = m1 + m2, sq = m - n;
+    const int tid = threadIdx.x;
+    extern __shared__ double dsm[];
+    double *pd = dsm, *pz = pd + n;
+    int *flag = reinterpret_cast<int *>(pz + n);
+    int *scan = flag + n;
+    __shared__ double s_tol;
+    __shared__ int s_nrot;
+
+    const float *d = P.d, *wt = P.wt;
+    const double alpha = P.d[c1 - 1], beta = P.e[c1 - 1];
+    double orgnrm = fmax(fabs(alpha), fabs(beta));
+    double dmax = 0.0;
+    if (n1 > 0) dmax = fmax(dmax, (double)d[c0]);
+    if (n2 > 0) dmax = fmax(dmax, (double)d[c1]);
+    orgnrm = fmax(orgnrm, dmax);
+    const double scale = orgnrm > 0.0 ? orgnrm : 1.0;
+    const double al = alpha / scale, be = beta / scale;
+    Arena Ar = make_arena(P.idx, c0, n);
+
+    // merged ascending order of the two descending lists (ties: left first)
+    for (int p = tid; p < n1; p += DT) {
+        const float x = d[c0 + p];
+        int lo = 0, hi = n2;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (d[c1 + mid] < x)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        const int t = 1 + (n1 - 1 - p) + (n2 - lo);
+        pd[t] = (double)x / scale;
+        pz[t] = al * (double)wt[(size_t)(c0 + p) * N + (c1 - 1)];
+        Ar.src[t] = p;
+    }
+    for (int q = tid; q < n2; q += DT) {
+        const float x = d[c1 + q];
+        int lo = 0, hi = n1;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (d[c0 + mid] <= x)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        const int t = 1 + (n2 - 1 - q) + (n1 - lo);
+        pd[t] = (double)x / scale;
+        pz[t] = be * (double)wt[(size_t)(c1 + q) * N + c1];
+        Ar.src[t] = -1 - q;
+    }
+    if (tid == 0) {
+        const double tol = 8.0 * (double)FLT_EPSILON * fmax(fmax(fabs(al), fabs(be)), dmax / scale);
+        const double zn1 = al * (double)wt[(size_t)(c0 + n1) * N + (c1 - 1)];
+        double nc = 1.0, ns = 0.0;
+        if (sq) {
+            const double zn2 = be * (double)wt[(size_t)(c1 + n2) * N + c1];
+            double r = hypot(zn1, zn2);
+            if (r <= tol) {
+                r = tol;
+            } else {
+                nc = zn1 / r;
+                ns = zn2 / r;
+            }
+            pz[0] = r;
+        } else {
+            pz[0] = fabs(zn1) <= tol ? tol : zn1;
+        }
+        pd[0] = 0.0;
+        flag[0] = 0;
+        Ar.src[0] = SRC_POLE0;
+        s_tol = tol;
+        MergeState &S = *T.S;
+        S.scale = scale;
+        S.tol = tol;
+        S.nc = nc;
+        S.ns = ns;
+        S.n = n;
+        S.m = m;
+        S.n1 = n1;
+        S.m1 = m1;
+        S.n2 = n2;
+        S.m2 = m2;
+        S.sq = sq;
+    }
+    __syncthreads();
+    // sequential deflation scan (svd.cpp:637-742): small z, then close poles by rotation
+    if (tid == 0) {
+        const double tol = s_tol;
+        int jp = -1, nrot = 0;
+        for (int j = 1; j < n; ++j) {
+            if (fabs(pz[j]) <= tol) {
+                flag[j] = 1;
+                continue;
+            }
+            if (jp >= 0 && fabs(pd[j] - pd[jp]) <= tol) {
+                const double tau = hypot(pz[jp], pz[j]);
+                const double c = pz[j] / tau, s = pz[jp] / tau;
+                pz[j] = tau;
+                pz[jp] = 0.0;
+                flag[jp] = 1;
+                Ar.rpair[nrot] = jp | (j << 16);
+                Ar.rc[nrot] = (float)c;
+                Ar.rs[nrot] = (float)s;
+                ++nrot;
+            } else if (jp >= 0) {
+                flag[jp] = 0;
+            }
+            jp = j;
+        }
+        if (jp >= 0) flag[jp] = 0;
+        s_nrot = nrot;
+    }
+    __syncthreads();
+    // compaction: kept poles (ascending) then deflated poles (ascending)
+    const int chunk = (n + DT - 1) / DT, j0 = min(n, tid * chunk), j1 = min(n, j0 + chunk);
+    int cnt = 0;
+    for (int j = j0; j < j1; ++j) cnt += (flag[j] == 0);
+    scan[tid] = cnt;
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int t = 0; t < DT; ++t) {
+            int v = scan[t];
+            scan[t] = acc;
+            acc += v;
+        }
+        scan[DT] = acc;
+    }
+    __syncthreads();
+    const int k = scan[DT];
+    int kb = scan[tid];
+    for (int j = j0; j < j1; ++j) {
+        if (flag[j] == 0) {
+            Ar.order[kb] = j;
+            Ar.dk[kb] = pd[j];
+            Ar.zk[kb] = pz[j];
+            ++kb;
+        } else {
+            const int t = k + (j - kb);
+            Ar.order[t] = j;
+            Ar.dk[t] = pd[j];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (k > 1 && Ar.dk[1] <= 0.5 * s_tol) Ar.dk[1] = 0.5 * s_tol;
+        T.S->k = k;
+        T.S->nrot = s_nrot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// gather: bases of the merged problem in sorted-pole order + deflation rotations.
+// One thread per column x of the merged basis; strips of DT columns.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DT) void dc_gather_kernel(ProbList pl, LevelArgs la)
+{
+    const int spt = strips_per_task(la);
+    const int strip = blockIdx.x % spt;
+    const MergeCtx T = merge_ctx(pl, la, blockIdx.x / spt);
+    const ProbDesc P = pl.get(T.prob);
+    const MergeState S = *T.S;
+    const int N = P.N, c0 = T.c0, c1 = T.c1;
+    const int n = S.n, m = S.m, n1 = S.n1, m1 = S.m1, n2 = S.n2, m2 = S.m2;
+    const int x = strip * DT + threadIdx.x;
+    if (x >= m) return;
+    const Arena Ar = make_arena(P.idx, c0, n);
+    const float *qt = P.qt, *wt = P.wt;
+    float *st = P.qt + (size_t)N * N, *gt = P.wt + (size_t)N * N;
+    const float nc = (float)S.nc, ns = (float)S.ns;
+    const float v1n = (x < m1) ? wt[(size_t)(c0 + n1) * N + c0 + x] : 0.f;
+    const float v2n = (S.sq && x >= m1) ? wt[(size_t)(c1 + n2) * N + c1 + (x - m1)] : 0.f;
+    for (int t = 0; t < n; ++t) {
+        const int s = Ar.src[t];
+        float qv = 0.f, wv = 0.f;
+        if (s == SRC_POLE0) {
+            qv = (x == n1) ? 1.f : 0.f;
+            wv = S.sq ? (x < m1 ? nc * v1n : ns * v2n) : v1n;
+        } else if (s >= 0) {
+            if (x < n1) qv = qt[(size_t)(c0 + s) * N + c0 + x];
+            if (x < m1) wv = wt[(size_t)(c0 + s) * N + c0 + x];
+        } else {
+            const int q = -1 - s;
+            if (x > n1 && x < n) qv = qt[(size_t)(c1 + q) * N + c1 + (x - n1 - 1)];
+            if (x >= m1) wv = wt[(size_t)(c1 + q) * N + c1 + (x - m1)];
+        }
+        if (x < n) st[(size_t)(c0 + t) * N + c0 + x] = qv;
+        gt[(size_t)(c0 + t) * N + c0 + x] = wv;
+    }
+    if (S.sq) gt[(size_t)(c0 + n) * N + c0 + x] = (x < m1) ? -ns * v1n : nc * v2n;
+    for (int r = 0; r < S.nrot; ++r) {
+        const int pr = Ar.rpair[r], jp = pr & 0xffff, j = pr >> 16;
+        const float c = Ar.rc[r], s = Ar.rs[r];
+        if (x < n) {
+            float *a = st + (size_t)(c0 + jp) * N + c0 + x, *b = st + (size_t)(c0 + j) * N + c0 + x;
+            const float av = *a, bv = *b;
+            *a = c * av - s * bv;
+            *b = s * av + c * bv;
+        }
+        float *a = gt + (size_t)(c0 + jp) * N + c0 + x, *b = gt + (size_t)(c0 + j) * N + c0 + x;
+        const float av = *a, bv = *b;
+        *a = c * av - s * bv;
+        *b = s * av + c * bv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// secular roots: one wavefront per root (replaces SecularEquationSolver,
+// svd.cpp:977-1336): FP64, origin chosen by the midpoint test, two-pole rational
+// model, bracket with bisection fallback, convergence flag into info.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DT) void dc_secular_kernel(ProbList pl, LevelArgs la)
+{
+    const int gw = (blockIdx.x * DT + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (gw >= la.ntask * la.stride) return;
+    const int i = gw % la.stride;
+    const MergeCtx T = merge_ctx(pl, la, gw / la.stride);
+    const MergeState &S = *T.S;
+    const int k = S.k;
+    if (i >= k) return;
+    const ProbDesc P = pl.get(T.prob);
+    const Arena Ar = make_arena(P.idx, T.c0, S.n);
+    const double *d = Ar.dk, *z = Ar.zk;
+    const double eps = DBL_EPSILON;
+    if (k == 1) {
+        if (lane == 0) {
+            Ar.Kx[0] = 0;
+            Ar.mu[0] = z[0] * z[0];
+        }
+        return;
+    }
+    int K;
+    double lo, hi;
+    if (i == k - 1) {
+        double zz = 0;
+        for (int j = lane; j < k; j += 64) zz += z[j] * z[j];
+        zz = wave_sum(zz);
+        K = k - 1;
+        lo = 0.0;
+        hi = zz;
+    } else {
+        const double dl = d[i], dr = d[i + 1], om = 0.5 * (dl + dr);
+        double f = 0;
+        for (int j = lane; j < k; j += 64) f += z[j] * z[j] / ((d[j] - om) * (d[j] + om));
+        f = 1.0 + wave_sum(f);
+        if (f > 0) {
+            K = i;
+            lo = 0.0;
+            hi = (om - dl) * (om + dl);
+        } else {
+            K = i + 1;
+            lo = (om - dr) * (om + dr);
+            hi = 0.0;
+        }
+    }
+    const double dK = d[K];
+    double mu = 0.5 * (lo + hi);
+    int it;
+    for (it = 1; it <= 200; ++it) {
+        double psi = 0, dpsi = 0, phi = 0, dphi = 0, erretm = 0;
+        for (int j = lane; j < k; j += 64) {
+            const double t = (d[j] - dK) * (d[j] + dK) - mu;
+            const double q = z[j] / t, term = z[j] * q;
+            if (j <= i) {
+                psi += term;
+                dpsi += q * q;
+            } else {
+                phi += term;
+                dphi += q * q;
+            }
+            erretm += fabs(term);
+        }
+        psi = wave_sum(psi);
+        dpsi = wave_sum(dpsi);
+        phi = wave_sum(phi);
+        dphi = wave_sum(dphi);
+        erretm = wave_sum(erretm);
+        const double w = 1.0 + psi + phi;
+        if (w == 0.0 || fabs(w) <= 8.0 * eps * (1.0 + erretm)) break;
+        if (w < 0)
+            lo = mu;
+        else
+            hi = mu;
+        const double a1 = (d[i] - dK) * (d[i] + dK) - mu;
+        double next;
+        if (i == k - 1) {
+            const double s = a1 * a1 * dpsi, c = w - s / a1;
+            next = (c > 0) ? mu + a1 + s / c : 0.5 * (lo + hi);
+        } else {
+            const double a2 = (d[i + 1] - dK) * (d[i + 1] + dK) - mu;
+            const double s = a1 * a1 * dpsi, t = a2 * a2 * dphi;
+            const double c = w - s / a1 - t / a2;
+            const double B = c * (a1 + a2) + s + t, C = a1 * a2 * w;
+            double eta;
+            if (c == 0.0) {
+                eta = C / B;
+            } else {
+                double disc = B * B - 4.0 * c * C;
+                if (disc < 0) disc = 0;
+                const double sqd = sqrt(disc);
+                const double qq = 0.5 * (B + (B >= 0 ? sqd : -sqd));
+                const double r1 = qq / c, r2 = (qq != 0.0) ? C / qq : 0.0;
+                eta = (r1 > a1 && r1 < a2) ? r1 : r2;
+            }
+            next = mu + eta;
+        }
+        if (!(next > lo && next < hi)) next = 0.5 * (lo + hi);
+        const double step = fabs(next - mu);
+        mu = next;
+        if (step <= 2.0 * eps * fabs(mu) || (hi - lo) <= 4.0 * eps * fmax(fabs(lo), fabs(hi))) break;
+    }
+    if (lane == 0) {
+        Ar.Kx[i] = K;
+        Ar.mu[i] = mu;
+        if (it > 200) atomicAdd(P.info, 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Loewner z (svd.cpp:1424-1438, in FP64) and output positions; one thread per pole.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DT) void dc_lowner_kernel(ProbList pl, LevelArgs la)
+{
+    const int g = blockIdx.x * DT + threadIdx.x;
+    if (g >= la.ntask * la.stride) return;
+    const int j = g % la.stride;
+    const MergeCtx T = merge_ctx(pl, la, g / la.stride);
+    const MergeState &S = *T.S;
+    const int k = S.k, n = S.n;
+    if (j >= n) return;
+    const ProbDesc P = pl.get(T.prob);
+    const Arena Ar = make_arena(P.idx, T.c0, n);
+    const double *d = Ar.dk;
+    int rank;
+    if (j < k) {
+        const double dj = d[j];
+        double prod;
+        {
+            const int K = Ar.Kx[k - 1];
+            prod = Ar.mu[k - 1] - (dj - d[K]) * (dj + d[K]);
+        }
+        for (int i = 0; i < j; ++i) {
+            const int K = Ar.Kx[i];
+            const double df = (dj - d[K]) * (dj + d[K]) - Ar.mu[i];
+            prod *= df / ((dj - d[i]) * (dj + d[i]));
+        }
+        for (int i = j; i < k - 1; ++i) {
+            const int K = Ar.Kx[i];
+            const double df = (dj - d[K]) * (dj + d[K]) - Ar.mu[i];
+            prod *= df / ((dj - d[i + 1]) * (dj + d[i + 1]));
+        }
+        const double zj = Ar.zk[j];
+        Ar.zk[j] = copysign(sqrt(fabs(prod)), zj);
+        // rank of root j among deflated poles (those with d^2 < sigma_j^2)
+        const int K = Ar.Kx[j];
+        const double s2 = d[K] * d[K] + Ar.mu[j];
+        int lo = k, hi = n;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (d[mid] * d[mid] < s2)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        rank = j + (lo - k);
+    } else {
+        const double dd2 = d[j] * d[j];
+        int lo = 0, hi = k;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            const int K = Ar.Kx[mid];
+            if (d[K] * d[K] + Ar.mu[mid] <= dd2)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        rank = (j - k) + lo;
+    }
+    Ar.pos[j] = n - 1 - rank;
+}
+
+// ---------------------------------------------------------------------------
+// secular vectors (svd.cpp:1440-1496): one wavefront per root i; FP64, stored FP32.
+// UMt (left, in a) and VMt (right, in vmt), row i = coefficients of root i.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DT) void dc_vectors_kernel(ProbList pl, LevelArgs la)
+{
+    const int gw = (blockIdx.x * DT + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (gw >= la.ntask * la.stride) return;
+    const int i = gw % la.stride;
+    const MergeCtx T = merge_ctx(pl, la, gw / la.stride);
+    const MergeState &S = *T.S;
+    const int k = S.k;
+    if (i >= k) return;
+    const ProbDesc P = pl.get(T.prob);
+    const int N = P.N, c0 = T.c0;
+    const Arena Ar = make_arena(P.idx, c0, S.n);
+    const double *d = Ar.dk, *z = Ar.zk;
+    float *um = P.a + (size_t)(c0 + i) * N + c0;
+    float *vm = P.vmt + (size_t)(c0 + i) * N + c0;
+    if (k == 1 && z[0] == 0.0) {
+        if (lane == 0) {
+            um[0] = 1.f;
+            vm[0] = 1.f;
+        }
+        return;
+    }
+    const int K = Ar.Kx[i];
+    const double dK = d[K], mu = Ar.mu[i];
+    double su = 0, sv = 0;
+    for (int j = lane; j < k; j += 64) {
+        const double df = (d[j] - dK) * (d[j] + dK) - mu;
+        const double v = z[j] / df, u = (j == 0) ? -1.0 : d[j] * z[j] / df;
+        su += u * u;
+        sv += v * v;
+    }
+    su = wave_sum(su);
+    sv = wave_sum(sv);
+    const double nu = 1.0 / sqrt(su), nv = 1.0 / sqrt(sv);
+    for (int j = lane; j < k; j += 64) {
+        const double df = (d[j] - dK) * (d[j] + dK) - mu;
+        const double v = z[j] / df, u = (j == 0) ? -1.0 : d[j] * z[j] / df;
+        um[j] = (float)(u * nu);
+        vm[j] = (float)(v * nv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// finalize: deflated rows, the null vector, and sigma in descending order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(DT) void dc_finalize_kernel(ProbList pl, LevelArgs la)
+{
+    const int spt = strips_per_task(la);
+    const int strip = blockIdx.x % spt;
+    const MergeCtx T = merge_ctx(pl, la, blockIdx.x / spt);
+    const ProbDesc P = pl.get(T.prob);
+    const MergeState S = *T.S;
+    const int N = P.N, c0 = T.c0, n = S.n, m = S.m, k = S.k;
+    const Arena Ar = make_arena(P.idx, c0, n);
+    const float *st = P.qt + (size_t)N * N, *gt = P.wt + (size_t)N * N;
+    const int x = strip * DT + threadIdx.x;
+    if (x < m) {
+        for (int t = k; t < n; ++t) {
+            const int src = Ar.order[t], dst = Ar.pos[t];
+            if (x < n) P.qt[(size_t)(c0 + dst) * N + c0 + x] = st[(size_t)(c0 + src) * N + c0 + x];
+            P.wt[(size_t)(c0 + dst) * N + c0 + x] = gt[(size_t)(c0 + src) * N + c0 + x];
+        }
+        if (S.sq) P.wt[(size_t)(c0 + n) * N + c0 + x] = gt[(size_t)(c0 + n) * N + c0 + x];
+    }
+    if (strip == 0) {
+        for (int t = threadIdx.x; t < n; t += DT) {
+            double s;
+            if (t < k) {
+                const int K = Ar.Kx[t];
+                s = sqrt(Ar.dk[K] * Ar.dk[K] + Ar.mu[t]);
+            } else {
+                s = Ar.dk[t];
+            }
+            P.d[c0 + Ar.pos[t]] = (float)(s * S.scale);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// merge products (svd.cpp:1504-1516): rows pos[i] of the node block <-
+//   sum_j coef[i][j] * basis[order[j]], i, j < k, one 64x64 tile per workgroup.
+// side 0: coef = left secular vectors (in a), basis = gathered left bases, out qt;
+// side 1: coef = right secular vectors (vmt), basis = gathered right bases, out wt.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(GT) void dc_merge_gemm_kernel(ProbList pl, LevelArgs la, int side)
+{
+    const int tpt = (la.stride + 63) / 64, tiles = tpt * tpt;
+    const int tile = blockIdx.x % tiles;
+    const MergeCtx T = merge_ctx(pl, la, blockIdx.x / tiles);
+    const MergeState &S = *T.S;
+    const ProbDesc P = pl.get(T.prob);
+    const int N = P.N, c0 = T.c0, k = S.k;
+    const Arena Ar = make_arena(P.idx, c0, S.n);
+    TileCtx t;
+    const size_t base = (size_t)c0 * N + c0;
+    t.A = (side == 0 ? P.a : P.vmt) + base;
+    t.B = (side == 0 ? P.qt : P.wt) + (size_t)N * N + base;
+    t.C = (side == 0 ? P.qt : P.wt) + base;
+    t.brow = Ar.order;
+    t.crow = Ar.pos;
+    t.lda = t.ldb = t.ldc = N;
+    t.m = k;
+    t.K = k;
+    t.n = side == 0 ? S.n : S.m;
+    t.transA = t.transB = 0;
+    t.alpha = 1.f;
+    t.beta = 0.f;
+    t.m0 = (tile / tpt) * 64;
+    t.n0 = (tile % tpt) * 64;
+    if (t.m0 >= t.m || t.n0 >= t.n) return;
+    tile_gemm<MODE>(t);
+}
+
+// back-transforms (updateUVt, svd.cpp:1911-1981):
+//   side 0: a (M x N) <- u[:, :N] * qt^T        side 1: qt+N*N (N x N) <- wt * vt
+template <int MODE>
+__global__ __launch_bounds__(GT) void dc_bt_gemm_kernel(ProbList pl, const int *tile_off, int side)
+{
+    const int pid = find_task(tile_off, pl.count, blockIdx.x);
+    const int tile = blockIdx.x - off_at(tile_off, pid);
+    const ProbDesc P = pl.get(pid);
+    const int M = P.M, N = P.N, tn = (N + 63) / 64;
+    TileCtx t;
+    if (side == 0) {
+        t.A = P.u;
+        t.lda = M;
+        t.B = P.qt;
+        t.ldb = N;
+        t.transB = 1;
+        t.C = P.a;
+        t.ldc = N;
+        t.m = M;
+    } else {
+        t.A = P.wt;
+        t.lda = N;
+        t.B = P.vt;
+        t.ldb = N;
+        t.transB = 0;
+        t.C = P.qt + (size_t)N * N;
+        t.ldc = N;
+        t.m = N;
+    }
+    t.transA = 0;
+    t.brow = t.crow = nullptr;
+    t.n = N;
+    t.K = N;
+    t.alpha = 1.f;
+    t.beta = 0.f;
+    t.m0 = (tile / tn) * 64;
+    t.n0 = (tile % tn) * 64;
+    if (t.m0 >= t.m) return;
+    tile_gemm<MODE>(t);
+}
+
+// copy of the back-transform results into u[:, :N] (side 0) and vt (side 1)
+__global__ __launch_bounds__(DT) void dc_bt_copy_kernel(ProbList pl, const int *row_off, int side)
+{
+    const int pid = find_task(row_off, pl.count, blockIdx.x);
+    const int r = blockIdx.x - off_at(row_off, pid);
+    const ProbDesc P = pl.get(pid);
+    const int M = P.M, N = P.N;
+    if (side == 0) {
+        if (r >= M) return;
+        for (int c = threadIdx.x; c < N; c += DT) P.u[(size_t)r * M + c] = P.a[(size_t)r * N + c];
+    } else {
+        if (r >= N) return;
+        const float *src = P.qt + (size_t)N * N;
+        for (int c = threadIdx.x; c < N; c += DT) P.vt[(size_t)r * N + c] = src[(size_t)r * N + c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_dc_leaves(ProbList pl, const int *leaf_off, const uint32_t *blob_leaves, int nleaf, hipStream_t s)
+{
+    if (nleaf <= 0) return hipSuccess;
+    hipLaunchKernelGGL(dc_leaf_kernel, dim3((nleaf + DT - 1) / DT), dim3(DT), 0, s, pl, leaf_off, blob_leaves, nleaf);
+    return hipGetLastError();
+}
+
+template <int MODE>
+static void launch_merge_gemms(ProbList pl, LevelArgs la, hipStream_t s)
+{
+    const int tpt = (la.stride + 63) / 64;
+    const int grid = la.ntask * tpt * tpt;
+    hipLaunchKernelGGL(dc_merge_gemm_kernel<MODE>, dim3(grid), dim3(GT), 0, s, pl, la, 0);
+    hipLaunchKernelGGL(dc_merge_gemm_kernel<MODE>, dim3(grid), dim3(GT), 0, s, pl, la, 1);
+}
+
+hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)dc_deflate_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    if (la.ntask <= 0) return hipSuccess;
+    const size_t lds = (size_t)la.maxn * (2 * sizeof(double) + sizeof(int)) + (DT + 1) * sizeof(int);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const int spt = strips_per_task(la);
+    const int rows = la.ntask * la.stride;
+    const int wave_blocks = (rows + DT / 64 - 1) / (DT / 64);
+    hipLaunchKernelGGL(dc_deflate_kernel, dim3(la.ntask), dim3(DT), lds, s, pl, la);
+    hipLaunchKernelGGL(dc_gather_kernel, dim3(la.ntask * spt), dim3(DT), 0, s, pl, la);
+    hipLaunchKernelGGL(dc_secular_kernel, dim3(wave_blocks), dim3(DT), 0, s, pl, la);
+    hipLaunchKernelGGL(dc_lowner_kernel, dim3((rows + DT - 1) / DT), dim3(DT), 0, s, pl, la);
+    hipLaunchKernelGGL(dc_vectors_kernel, dim3(wave_blocks), dim3(DT), 0, s, pl, la);
+    const int mode = gemm_get_mode();
+    if (mode == 1)
+        launch_merge_gemms<1>(pl, la, s);
+    else if (mode == 2)
+        launch_merge_gemms<2>(pl, la, s);
+    else
+        launch_merge_gemms<0>(pl, la, s);
+    hipLaunchKernelGGL(dc_finalize_kernel, dim3(la.ntask * spt), dim3(DT), 0, s, pl, la);
+    return hipGetLastError();
+}
+
+template <int MODE>
+static void launch_bt(ProbList pl, const int *tou, int ntu, const int *tov, int ntv, hipStream_t s)
+{
+    if (ntu > 0) hipLaunchKernelGGL(dc_bt_gemm_kernel<MODE>, dim3(ntu), dim3(GT), 0, s, pl, tou, 0);
+    if (ntv > 0) hipLaunchKernelGGL(dc_bt_gemm_kernel<MODE>, dim3(ntv), dim3(GT), 0, s, pl, tov, 1);
+}
+
+hipError_t launch_dc_backtransform(ProbList pl, const int *tile_off_u, int ntiles_u, const int *tile_off_v,
+                                   int ntiles_v, const int *row_off_u, int nrows_u, const int *row_off_v,
+                                   int nrows_v, hipStream_t s)
+{
+    const int mode = gemm_get_mode();
+    if (mode == 1)
+        launch_bt<1>(pl, tile_off_u, ntiles_u, tile_off_v, ntiles_v, s);
+    else if (mode == 2)
+        launch_bt<2>(pl, tile_off_u, ntiles_u, tile_off_v, ntiles_v, s);
+    else
+        launch_bt<0>(pl, tile_off_u, ntiles_u, tile_off_v, ntiles_v, s);
+    if (nrows_u > 0) hipLaunchKernelGGL(dc_bt_copy_kernel, dim3(nrows_u), dim3(DT), 0, s, pl, row_off_u, 0);
+    if (nrows_v > 0) hipLaunchKernelGGL(dc_bt_copy_kernel, dim3(nrows_v), dim3(DT), 0, s, pl, row_off_v, 1);
+    return hipGetLastError();
+}
+
+} // namespace paco

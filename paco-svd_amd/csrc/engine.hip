@@ -1,0 +1,763 @@
+// Host orchestration of the gfx950 SVD engine and its C ABI (include/paco_svd.h).
+//
+// Plain entry points (paco_upper_bidiagonalization / paco_svd_DC) take one problem by
+// value: every schedule (leaves, merge rounds, tiles) is derived on the device from
+// (M, N), so no host plan is uploaded and the calls are pure stream-ordered launches
+// (main.cpp:126-160 launches the two reference kernels the same way).  The batched
+// path uploads one small plan (problem descriptors + per-problem prefix arrays) and
+// runs each phase as one launch over every matrix.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/paco_svd.h"
+#include "paco_kernels.hpp"
+
+namespace paco {
+
+static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------
+// tiny kernels of the engine
+// ---------------------------------------------------------------------------
+__global__ void zero_ints_kernel(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// a <- A (row-major M x N) for every problem; grid over (problem, row).
+__global__ __launch_bounds__(256) void copy_in_kernel(ProbList pl, const float *const *src, const int *row_off)
+{
+    const int pid = find_task(row_off, pl.count, blockIdx.x);
+    const int r = blockIdx.x - off_at(row_off, pid);
+    const ProbDesc P = pl.get(pid);
+    if (r >= P.M) return;
+    const float *s = src[pid] + (size_t)r * P.N;
+    float *d = P.a + (size_t)r * P.N;
+    for (int c = threadIdx.x; c < P.N; c += 256) d[c] = s[c];
+}
+
+// ---------------------------------------------------------------------------
+// single-problem workspace
+// ---------------------------------------------------------------------------
+struct WsLayout {
+    size_t vmt, tq, tp, info, states, grid, total;
+};
+
+static int total_merges(int N) { return leaf_count(N) - 1; }
+
+static WsLayout ws_layout(int M, int N)
+{
+    WsLayout L;
+    size_t o = 0;
+    L.vmt = o;
+    o = align_up(o + sizeof(float) * (size_t)N * N, 256);
+    L.tq = o;
+    o = align_up(o + sizeof(float) * 1024 * (size_t)cdiv(N, 32), 256);
+    L.tp = o;
+    o = align_up(o + sizeof(float) * 1024 * (size_t)cdiv(std::max(N - 1, 1), 32), 256);
+    L.info = o;
+    o = align_up(o + 64, 256);
+    L.states = o;
+    o = align_up(o + sizeof(MergeState) * (size_t)std::max(total_merges(N), 1), 256);
+    L.grid = o;
+    o = align_up(o + bidiag_grid_ws_bytes(M, N), 256);
+    L.total = o;
+    return L;
+}
+
+static std::mutex g_mu;
+static void *g_ws = nullptr;
+static size_t g_ws_size = 0;
+
+static void *internal_ws(size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (bytes > g_ws_size) {
+        if (g_ws) (void)hipFree(g_ws);
+        g_ws = nullptr;
+        g_ws_size = 0;
+        if (hipMalloc(&g_ws, bytes) != hipSuccess) return nullptr;
+        g_ws_size = bytes;
+    }
+    return g_ws;
+}
+
+static bool use_grid_bidiag(int M, int N)
+{
+    const char *env = getenv("PACO_BIDIAG");
+    if (env && !strcmp(env, "wg")) return false;
+    if (env && !strcmp(env, "grid")) return true;
+    return (size_t)M * N >= (size_t)1024 * 1024;
+}
+
+static ProbDesc bidiag_desc(int M, int N, float *a, float *u, float *vt, float *d, float *e, float *tauq,
+                            float *taup, char *ws, const WsLayout &L)
+{
+    ProbDesc P;
+    memset(&P, 0, sizeof(P));
+    P.M = M;
+    P.N = N;
+    P.a = a;
+    P.u = u;
+    P.vt = vt;
+    P.d = d;
+    P.e = e;
+    P.tauq = tauq;
+    P.taup = taup;
+    P.vmt = reinterpret_cast<float *>(ws + L.vmt);
+    P.tq = reinterpret_cast<float *>(ws + L.tq);
+    P.tp = reinterpret_cast<float *>(ws + L.tp);
+    P.info = reinterpret_cast<int *>(ws + L.info);
+    return P;
+}
+
+// Q and P^T from the reflectors (GetUVt, upper_bidiagonalization.cpp:774-893), blocked WY.
+static hipError_t form_q_pt(ProbList pl, const int *idq, int nidq, const int *idp, int nidp, const int *bq, int nbq,
+                            const int *bp, int nbp, const int *sq, int nsq, const int *sp, int nsp, const int *tpair,
+                            int ntp, hipStream_t s)
+{
+    hipError_t e;
+    if ((e = launch_identity(pl, idq, nidq, 0, s)) != hipSuccess) return e;
+    if ((e = launch_identity(pl, idp, nidp, 1, s)) != hipSuccess) return e;
+    if ((e = launch_tfactor(pl, bq, nbq, 0, s)) != hipSuccess) return e;
+    if ((e = launch_tfactor(pl, bp, nbp, 1, s)) != hipSuccess) return e;
+    if ((e = launch_wy_apply(pl, sq, nsq, 0, s)) != hipSuccess) return e;
+    if ((e = launch_wy_apply(pl, sp, nsp, 1, s)) != hipSuccess) return e;
+    return launch_transpose_vt(pl, tpair, ntp, s);
+}
+
+static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, float *e, float *tauq, float *taup,
+                         void *workspace, hipStream_t s)
+{
+    const WsLayout L = ws_layout(M, N);
+    char *ws = static_cast<char *>(workspace ? workspace : internal_ws(L.total));
+    if (!ws) return PACO_ENOMEM;
+    ProbList pl;
+    pl.ptr = nullptr;
+    pl.count = 1;
+    pl.single = bidiag_desc(M, N, a, u, vt, d, e, tauq, taup, ws, L);
+    hipError_t err;
+    if (use_grid_bidiag(M, N)) {
+        err = launch_bidiag_grid(pl.single, ws + L.grid, s);
+    } else {
+        const size_t lds = bidiag_wg_lds_bytes(M, N);
+        if (lds > 160 * 1024) return PACO_EINVAL;
+        err = launch_bidiag_wg(pl, nullptr, 1, lds, s);
+    }
+    if (err != hipSuccess) return (int)err;
+    const int nt = cdiv(N, 32);
+    err = form_q_pt(pl, nullptr, M, nullptr, N, nullptr, cdiv(N, 32), nullptr, cdiv(std::max(N - 1, 0), 32), nullptr,
+                    cdiv(M, 64), nullptr, cdiv(N, 64), nullptr, nt * (nt + 1) / 2, s);
+    return (int)err;
+}
+
+// D&C levels + back-transform for a problem list (single: offsets null).
+struct DcOffsets {
+    const int *leaf_off = nullptr;
+    int nleaf = 0;
+    std::vector<const int *> task_off; // per round
+    std::vector<int> ntask, maxn;
+    std::vector<MergeState *> states;
+    const int *bt_tu = nullptr, *bt_tv = nullptr, *bt_ru = nullptr, *bt_rv = nullptr;
+    int ntu = 0, ntv = 0, nru = 0, nrv = 0;
+};
+
+static hipError_t run_dc(ProbList pl, const DcOffsets &o, const uint32_t *blob_leaves, hipStream_t s)
+{
+    hipError_t e = launch_dc_leaves(pl, o.leaf_off, blob_leaves, o.nleaf, s);
+    if (e != hipSuccess) return e;
+    for (size_t r = 1; r <= o.ntask.size(); ++r) {
+        LevelArgs la;
+        la.task_off = o.task_off[r - 1];
+        la.states = o.states[r - 1];
+        la.ntask = o.ntask[r - 1];
+        la.r = (int)r;
+        la.stride = 3 << r;
+        la.maxn = o.maxn[r - 1];
+        if ((e = launch_dc_level(pl, la, s)) != hipSuccess) return e;
+    }
+    return launch_dc_backtransform(pl, o.bt_tu, o.ntu, o.bt_tv, o.ntv, o.bt_ru, o.nru, o.bt_rv, o.nrv, s);
+}
+
+static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, float *e, float *qt, float *wt,
+                     uint32_t *idx, void *workspace, const void *tiling_dev, hipStream_t s)
+{
+    const WsLayout L = ws_layout(M, N);
+    char *ws = static_cast<char *>(workspace ? workspace : internal_ws(L.total));
+    if (!ws) return PACO_ENOMEM;
+    ProbList pl;
+    pl.ptr = nullptr;
+    pl.count = 1;
+    ProbDesc &P = pl.single;
+    P = bidiag_desc(M, N, a, u, vt, d, e, nullptr, nullptr, ws, L);
+    P.qt = qt;
+    P.wt = wt;
+    P.idx = idx;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, P.info, 1);
+    DcOffsets o;
+    o.nleaf = leaf_count(N);
+    MergeState *st = reinterpret_cast<MergeState *>(ws + L.states);
+    const int R = rounds_of(N);
+    for (int r = 1; r <= R; ++r) {
+        o.task_off.push_back(nullptr);
+        o.ntask.push_back(merges_in_round(N, r));
+        o.maxn.push_back(std::min(3 << r, N));
+        o.states.push_back(st);
+        st += merges_in_round(N, r);
+    }
+    o.ntu = cdiv(M, 64) * cdiv(N, 64);
+    o.ntv = cdiv(N, 64) * cdiv(N, 64);
+    o.nru = M;
+    o.nrv = N;
+    const uint32_t *blob = nullptr;
+    if (tiling_dev) blob = reinterpret_cast<const uint32_t *>(static_cast<const char *>(tiling_dev) + 64);
+    return (int)run_dc(pl, o, blob, s);
+}
+
+// ---------------------------------------------------------------------------
+// batched path
+// ---------------------------------------------------------------------------
+struct BatchScratch {
+    size_t a, e, tauq, taup, qt, wt, idx, vmt, tq, tp, states, total;
+};
+
+static BatchScratch scratch_layout(int M, int N)
+{
+    BatchScratch B;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t r = o;
+        o = align_up(o + bytes, 256);
+        return r;
+    };
+    B.a = take(sizeof(float) * (size_t)M * N);
+    B.e = take(sizeof(float) * (size_t)std::max(N - 1, 1));
+    B.tauq = take(sizeof(float) * (size_t)N);
+    B.taup = take(sizeof(float) * (size_t)std::max(N - 1, 1));
+    B.qt = take(sizeof(float) * 2 * (size_t)N * N);
+    B.wt = take(sizeof(float) * 2 * (size_t)N * N);
+    B.idx = take(sizeof(uint32_t) * 16 * (size_t)N);
+    B.vmt = take(sizeof(float) * (size_t)N * N);
+    B.tq = take(sizeof(float) * 1024 * (size_t)cdiv(N, 32));
+    B.tp = take(sizeof(float) * 1024 * (size_t)cdiv(std::max(N - 1, 1), 32));
+    B.states = take(sizeof(MergeState) * (size_t)std::max(total_merges(N), 1));
+    B.total = o;
+    return B;
+}
+
+// Host-side plan of a batch: a byte blob (uploaded once) + launch sizes.
+struct BatchPlan {
+    std::vector<uint8_t> blob;
+    size_t scratch_bytes = 0;
+    int count = 0;
+    // offsets into blob (bytes)
+    size_t o_probs = 0, o_src = 0, o_order = 0, o_info = 0;
+    size_t o_rowin = 0, o_idq = 0, o_idp = 0, o_bq = 0, o_bp = 0, o_sq = 0, o_sp = 0, o_tp = 0;
+    size_t o_leaf = 0, o_tu = 0, o_tv = 0, o_ru = 0, o_rv = 0;
+    std::vector<size_t> o_task;
+    int nrowin = 0, nidq = 0, nidp = 0, nbq = 0, nbp = 0, nsq = 0, nsp = 0, ntp = 0, nleaf = 0;
+    int ntu = 0, ntv = 0, nru = 0, nrv = 0;
+    std::vector<int> ntask, maxn;
+    std::vector<size_t> state_off; // per round, in MergeState units from the states base
+    size_t lds_bidiag = 0;
+    bool fits = true;
+};
+
+template <class T>
+static size_t blob_put(std::vector<uint8_t> &b, const std::vector<T> &v)
+{
+    size_t o = align_up(b.size(), 256);
+    b.resize(o + sizeof(T) * v.size());
+    if (!v.empty()) memcpy(b.data() + o, v.data(), sizeof(T) * v.size());
+    return o;
+}
+
+// Prefix of per-problem counts: returns offsets (count+1 entries) and the total.
+template <class F>
+static std::vector<int> prefix(int count, F f, int &total)
+{
+    std::vector<int> off(count + 1);
+    int acc = 0;
+    for (int i = 0; i < count; ++i) {
+        off[i] = acc;
+        acc += f(i);
+    }
+    off[count] = acc;
+    total = acc;
+    return off;
+}
+
+static double svd_cost(int M, int N) { return 4.0 * M * M * N + 2.0 * M * N * N + 6.0 * (double)N * N * N; }
+
+static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::vector<size_t> &scr_off)
+{
+    bp.count = count;
+    scr_off.resize(count);
+    size_t so = 0;
+    int maxN = 0;
+    for (int i = 0; i < count; ++i) {
+        scr_off[i] = so;
+        so += scratch_layout(jobs[i].M, jobs[i].N).total;
+        maxN = std::max(maxN, jobs[i].N);
+        bp.lds_bidiag = std::max(bp.lds_bidiag, bidiag_wg_lds_bytes(jobs[i].M, jobs[i].N));
+    }
+    bp.scratch_bytes = so;
+    bp.fits = bp.lds_bidiag <= 160 * 1024 && maxN <= PACO_MAX_N;
+    auto &b = bp.blob;
+    b.clear();
+    b.resize(0);
+    // problem descriptors are patched with device addresses at upload time
+    std::vector<ProbDesc> probs(count);
+    bp.o_probs = blob_put(b, probs);
+    std::vector<const float *> src(count);
+    for (int i = 0; i < count; ++i) src[i] = jobs[i].A;
+    bp.o_src = blob_put(b, src);
+    std::vector<int> order(count);
+    for (int i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return svd_cost(jobs[x].M, jobs[x].N) > svd_cost(jobs[y].M, jobs[y].N);
+    });
+    bp.o_order = blob_put(b, order);
+    bp.o_rowin = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nrowin));
+    bp.o_idq = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nidq));
+    bp.o_idp = blob_put(b, prefix(count, [&](int i) { return jobs[i].N; }, bp.nidp));
+    bp.o_bq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, 32); }, bp.nbq));
+    bp.o_bp = blob_put(b, prefix(count, [&](int i) { return cdiv(std::max(jobs[i].N - 1, 0), 32); }, bp.nbp));
+    bp.o_sq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, 64); }, bp.nsq));
+    bp.o_sp = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, 64); }, bp.nsp));
+    bp.o_tp = blob_put(b, prefix(count, [&](int i) {
+        const int nt = cdiv(jobs[i].N, 32);
+        return nt * (nt + 1) / 2;
+    }, bp.ntp));
+    bp.o_leaf = blob_put(b, prefix(count, [&](int i) { return leaf_count(jobs[i].N); }, bp.nleaf));
+    int R = 0;
+    for (int i = 0; i < count; ++i) R = std::max(R, rounds_of(jobs[i].N));
+    bp.o_task.clear();
+    bp.ntask.clear();
+    bp.maxn.clear();
+    bp.state_off.clear();
+    size_t sacc = 0;
+    for (int r = 1; r <= R; ++r) {
+        int nt = 0;
+        bp.o_task.push_back(blob_put(b, prefix(count, [&](int i) { return merges_in_round(jobs[i].N, r); }, nt)));
+        bp.ntask.push_back(nt);
+        int mx = 0;
+        for (int i = 0; i < count; ++i)
+            if (merges_in_round(jobs[i].N, r) > 0) mx = std::max(mx, std::min(3 << r, jobs[i].N));
+        bp.maxn.push_back(mx);
+        bp.state_off.push_back(sacc);
+        sacc += nt;
+    }
+    bp.o_tu = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, 64) * cdiv(jobs[i].N, 64); }, bp.ntu));
+    bp.o_tv = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, 64) * cdiv(jobs[i].N, 64); }, bp.ntv));
+    bp.o_ru = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nru));
+    bp.o_rv = blob_put(b, prefix(count, [&](int i) { return jobs[i].N; }, bp.nrv));
+    std::vector<MergeState> states(std::max<size_t>(sacc, 1));
+    bp.o_info = blob_put(b, std::vector<int>(count, 0));
+    // merge states live after the blob in device memory (not uploaded)
+    b.resize(align_up(b.size(), 256));
+}
+
+// Device-side resources of a batch: plan buffer + scratch + states.
+struct BatchRes {
+    void *dev = nullptr;
+    size_t dev_size = 0;
+    void *pinned = nullptr;
+    size_t pinned_size = 0;
+    hipEvent_t ev = nullptr;
+};
+
+static hipError_t ensure(BatchRes &r, size_t dev_bytes, size_t pin_bytes)
+{
+    if (dev_bytes > r.dev_size) {
+        if (r.dev) (void)hipFree(r.dev);
+        r.dev = nullptr;
+        r.dev_size = 0;
+        hipError_t e = hipMalloc(&r.dev, dev_bytes);
+        if (e != hipSuccess) return e;
+        r.dev_size = dev_bytes;
+    }
+    if (pin_bytes > r.pinned_size) {
+        if (r.pinned) (void)hipHostFree(r.pinned);
+        r.pinned = nullptr;
+        r.pinned_size = 0;
+        hipError_t e = hipHostMalloc(&r.pinned, pin_bytes, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        r.pinned_size = pin_bytes;
+    }
+    if (!r.ev) return hipEventCreateWithFlags(&r.ev, hipEventDisableTiming);
+    return hipSuccess;
+}
+
+// Prepared batch: device layout fixed, descriptors patched, blob staged in pinned memory.
+struct Prepared {
+    size_t o_states = 0;
+    int *info = nullptr;
+};
+
+// Lay out device memory [blob | states | scratch...], patch descriptors and stage the
+// blob in pinned memory.  Allocates (outside any stream capture).
+static hipError_t prepare_batch(const paco_svd_job *jobs, int count, BatchPlan &bp, BatchRes &res, Prepared &pr)
+{
+    std::vector<size_t> scr_off;
+    build_plan(jobs, count, bp, scr_off);
+    if (!bp.fits) return hipErrorInvalidValue;
+    size_t nstates = 0;
+    for (int v : bp.ntask) nstates += v;
+    pr.o_states = align_up(bp.blob.size(), 256);
+    const size_t o_scratch = align_up(pr.o_states + sizeof(MergeState) * std::max<size_t>(nstates, 1), 256);
+    const size_t total = o_scratch + bp.scratch_bytes;
+    // the previous upload from the pinned buffer must be complete before overwriting it
+    if (res.ev) (void)hipEventSynchronize(res.ev);
+    hipError_t e = ensure(res, total, bp.blob.size());
+    if (e != hipSuccess) return e;
+    char *dev = static_cast<char *>(res.dev);
+    ProbDesc *probs = reinterpret_cast<ProbDesc *>(bp.blob.data() + bp.o_probs);
+    pr.info = reinterpret_cast<int *>(dev + bp.o_info);
+    for (int i = 0; i < count; ++i) {
+        const BatchScratch L = scratch_layout(jobs[i].M, jobs[i].N);
+        char *base = dev + o_scratch + scr_off[i];
+        ProbDesc &P = probs[i];
+        memset(&P, 0, sizeof(P));
+        P.M = jobs[i].M;
+        P.N = jobs[i].N;
+        P.a = reinterpret_cast<float *>(base + L.a);
+        P.u = jobs[i].U;
+        P.vt = jobs[i].Vt;
+        P.d = jobs[i].S;
+        P.e = reinterpret_cast<float *>(base + L.e);
+        P.tauq = reinterpret_cast<float *>(base + L.tauq);
+        P.taup = reinterpret_cast<float *>(base + L.taup);
+        P.qt = reinterpret_cast<float *>(base + L.qt);
+        P.wt = reinterpret_cast<float *>(base + L.wt);
+        P.idx = reinterpret_cast<uint32_t *>(base + L.idx);
+        P.vmt = reinterpret_cast<float *>(base + L.vmt);
+        P.tq = reinterpret_cast<float *>(base + L.tq);
+        P.tp = reinterpret_cast<float *>(base + L.tp);
+        P.info = pr.info + i;
+    }
+    memcpy(res.pinned, bp.blob.data(), bp.blob.size());
+    return hipSuccess;
+}
+
+// Upload + the whole launch sequence (capturable: no allocation, no host sync).
+static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepared &pr, hipStream_t s, bool record)
+{
+    const int count = bp.count;
+    char *dev = static_cast<char *>(res.dev);
+    hipError_t e;
+    if ((e = hipMemcpyAsync(dev, res.pinned, bp.blob.size(), hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    if (record && (e = hipEventRecord(res.ev, s)) != hipSuccess) return e;
+    auto I = [&](size_t o) { return reinterpret_cast<const int *>(dev + o); };
+    ProbList pl;
+    pl.ptr = reinterpret_cast<const ProbDesc *>(dev + bp.o_probs);
+    pl.count = count;
+    memset(&pl.single, 0, sizeof(pl.single));
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(cdiv(count, 256)), dim3(256), 0, s, pr.info, count);
+    hipLaunchKernelGGL(copy_in_kernel, dim3(bp.nrowin), dim3(256), 0, s, pl,
+                       reinterpret_cast<const float *const *>(dev + bp.o_src), I(bp.o_rowin));
+    if ((e = launch_bidiag_wg(pl, I(bp.o_order), count, bp.lds_bidiag, s)) != hipSuccess) return e;
+    if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
+                       I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)
+        return e;
+    DcOffsets o;
+    o.leaf_off = I(bp.o_leaf);
+    o.nleaf = bp.nleaf;
+    MergeState *st = reinterpret_cast<MergeState *>(dev + pr.o_states);
+    for (size_t r = 0; r < bp.ntask.size(); ++r) {
+        o.task_off.push_back(I(bp.o_task[r]));
+        o.ntask.push_back(bp.ntask[r]);
+        o.maxn.push_back(bp.maxn[r]);
+        o.states.push_back(st + bp.state_off[r]);
+    }
+    o.bt_tu = I(bp.o_tu);
+    o.ntu = bp.ntu;
+    o.bt_tv = I(bp.o_tv);
+    o.ntv = bp.ntv;
+    o.bt_ru = I(bp.o_ru);
+    o.nru = bp.nru;
+    o.bt_rv = I(bp.o_rv);
+    o.nrv = bp.nrv;
+    return run_dc(pl, o, nullptr, s);
+}
+
+static thread_local int *t_last_info = nullptr;
+static thread_local int t_last_count = 0;
+
+static BatchRes g_batch_res;
+
+struct GraphEntry {
+    BatchRes res;
+    BatchPlan plan;
+    hipGraphExec_t exec = nullptr;
+    int *info = nullptr;
+};
+static std::unordered_map<uint64_t, GraphEntry *> g_graphs;
+
+static uint64_t batch_signature(const paco_svd_job *jobs, int count)
+{
+    uint64_t h = 1469598103934665603ull ^ (uint64_t)count;
+    auto mix = [&](uint64_t v) {
+        h ^= v;
+        h *= 1099511628211ull;
+    };
+    for (int i = 0; i < count; ++i) {
+        mix((uint64_t)jobs[i].M);
+        mix((uint64_t)jobs[i].N);
+        mix((uint64_t)(uintptr_t)jobs[i].A);
+        mix((uint64_t)(uintptr_t)jobs[i].S);
+        mix((uint64_t)(uintptr_t)jobs[i].U);
+        mix((uint64_t)(uintptr_t)jobs[i].Vt);
+    }
+    mix((uint64_t)gemm_get_mode());
+    return h;
+}
+
+static bool job_ok(const paco_svd_job &j)
+{
+    return j.M >= j.N && j.N >= 1 && j.A && j.S && j.U && j.Vt && j.N <= PACO_MAX_N;
+}
+
+} // namespace paco
+
+using namespace paco;
+
+extern "C" {
+
+int paco_upper_bidiagonalization(unsigned blockDim, void *stream, int M, int N, float *a, float *u, float *vt,
+                                 float *d, float *e, float *tauq, float *taup, void *workspace)
+{
+    (void)blockDim;
+    if (M < N || N < 1 || !a || !u || !vt || !d || !tauq || (N > 1 && (!e || !taup))) return PACO_EINVAL;
+    return bidiag_single(M, N, a, u, vt, d, e, tauq, taup, workspace, static_cast<hipStream_t>(stream));
+}
+
+int paco_svd_DC(unsigned blockDim, void *stream, int M, int N, float *a, float *u, float *vt, float *d, float *e,
+                float *qt, float *wt, uint32_t *idx, void *workspace, const void *tiling_dev)
+{
+    (void)blockDim;
+    if (M < N || N < 1 || N > PACO_MAX_N || !a || !u || !vt || !d || !qt || !wt || !idx || (N > 1 && !e))
+        return PACO_EINVAL;
+    return dc_single(M, N, a, u, vt, d, e, qt, wt, idx, workspace, tiling_dev, static_cast<hipStream_t>(stream));
+}
+
+size_t paco_workspace_size(int M, int N)
+{
+    if (M < N || N < 1) return 0;
+    return ws_layout(M, N).total;
+}
+
+int paco_set_precision(int mode)
+{
+    const int old = gemm_get_mode();
+    if (mode >= 0 && mode <= 2) gemm_set_mode(mode);
+    return old;
+}
+
+int paco_svd_plain(const paco_svd_job *jobs, int count, void *stream)
+{
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (count <= 0) return 0;
+    size_t need = 0;
+    for (int i = 0; i < count; ++i) {
+        if (!job_ok(jobs[i])) return PACO_EINVAL;
+        need = std::max(need, scratch_layout(jobs[i].M, jobs[i].N).total + ws_layout(jobs[i].M, jobs[i].N).total);
+    }
+    static void *buf = nullptr;
+    static size_t bufsz = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (need > bufsz) {
+            if (buf) (void)hipFree(buf);
+            buf = nullptr;
+            bufsz = 0;
+            if (hipMalloc(&buf, need) != hipSuccess) return PACO_ENOMEM;
+            bufsz = need;
+        }
+    }
+    for (int i = 0; i < count; ++i) {
+        const int M = jobs[i].M, N = jobs[i].N;
+        const BatchScratch L = scratch_layout(M, N);
+        char *base = static_cast<char *>(buf);
+        char *ws = base + L.total;
+        float *a = reinterpret_cast<float *>(base + L.a);
+        hipError_t e = hipMemcpyAsync(a, jobs[i].A, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return (int)e;
+        int rc = paco_upper_bidiagonalization(40, s, M, N, a, jobs[i].U, jobs[i].Vt, jobs[i].S,
+                                              reinterpret_cast<float *>(base + L.e),
+                                              reinterpret_cast<float *>(base + L.tauq),
+                                              reinterpret_cast<float *>(base + L.taup), ws);
+        if (rc) return rc;
+        rc = paco_svd_DC(1, s, M, N, a, jobs[i].U, jobs[i].Vt, jobs[i].S, reinterpret_cast<float *>(base + L.e),
+                         reinterpret_cast<float *>(base + L.qt), reinterpret_cast<float *>(base + L.wt),
+                         reinterpret_cast<uint32_t *>(base + L.idx), ws, nullptr);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int paco_svd_batched(const paco_svd_job *jobs, int count, void *stream)
+{
+    if (count <= 0) return 0;
+    for (int i = 0; i < count; ++i)
+        if (!job_ok(jobs[i])) return PACO_EINVAL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchPlan bp;
+    Prepared pr;
+    hipError_t e = prepare_batch(jobs, count, bp, g_batch_res, pr);
+    if (e == hipErrorInvalidValue && !bp.fits) return PACO_EINVAL;
+    if (e != hipSuccess) return (int)e;
+    e = enqueue_batch(bp, g_batch_res, pr, static_cast<hipStream_t>(stream), true);
+    if (e != hipSuccess) return (int)e;
+    t_last_info = pr.info;
+    t_last_count = count;
+    return 0;
+}
+
+int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
+{
+    if (count <= 0) return 0;
+    for (int i = 0; i < count; ++i)
+        if (!job_ok(jobs[i])) return PACO_EINVAL;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(g_mu);
+    const uint64_t sig = batch_signature(jobs, count);
+    auto it = g_graphs.find(sig);
+    GraphEntry *g = nullptr;
+    if (it == g_graphs.end()) {
+        if (g_graphs.size() >= 8) {
+            for (auto &kv : g_graphs) {
+                (void)hipStreamSynchronize(s);
+                if (kv.second->exec) (void)hipGraphExecDestroy(kv.second->exec);
+                if (kv.second->res.dev) (void)hipFree(kv.second->res.dev);
+                if (kv.second->res.pinned) (void)hipHostFree(kv.second->res.pinned);
+                if (kv.second->res.ev) (void)hipEventDestroy(kv.second->res.ev);
+                delete kv.second;
+            }
+            g_graphs.clear();
+        }
+        g = new GraphEntry();
+        Prepared pr;
+        hipError_t e = prepare_batch(jobs, count, g->plan, g->res, pr);
+        if (e != hipSuccess) {
+            delete g;
+            return (e == hipErrorInvalidValue) ? PACO_EINVAL : (int)e;
+        }
+        g->info = pr.info;
+        hipGraph_t graph;
+        e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            delete g;
+            return (int)e;
+        }
+        e = enqueue_batch(g->plan, g->res, pr, s, false);
+        hipError_t e2 = hipStreamEndCapture(s, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            delete g;
+            return (int)(e != hipSuccess ? e : e2);
+        }
+        e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            delete g;
+            return (int)e;
+        }
+        g_graphs[sig] = g;
+    } else {
+        g = it->second;
+    }
+    hipError_t e = hipGraphLaunch(g->exec, s);
+    if (e != hipSuccess) return (int)e;
+    t_last_info = g->info;
+    t_last_count = count;
+    return 0;
+}
+
+size_t paco_batch_workspace_size(const int *M, const int *N, int count)
+{
+    size_t t = 0;
+    for (int i = 0; i < count; ++i) t += scratch_layout(M[i], N[i]).total;
+    return t + (size_t)count * (sizeof(ProbDesc) + 64 * sizeof(int)) + (1 << 20);
+}
+
+int paco_last_info(int *info_host, int count, void *stream)
+{
+    if (!t_last_info || !info_host) return 0;
+    const int n = std::min(count, t_last_count);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (hipMemcpyAsync(info_host, t_last_info, sizeof(int) * n, hipMemcpyDeviceToHost, s) != hipSuccess) return 0;
+    if (hipStreamSynchronize(s) != hipSuccess) return 0;
+    return n;
+}
+
+int paco_gemm_f32(void *stream, int transA, int transB, int m, int n, int k, float alpha, const float *A, int lda,
+                  const float *B, int ldb, float beta, float *C, int ldc)
+{
+    if (m < 0 || n < 0 || k < 0 || !A || !B || !C) return PACO_EINVAL;
+    if (m == 0 || n == 0) return 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    GemmDesc g;
+    memset(&g, 0, sizeof(g));
+    g.A = A;
+    g.B = B;
+    g.C = C;
+    g.lda = lda;
+    g.ldb = ldb;
+    g.ldc = ldc;
+    g.m = m;
+    g.n = n;
+    g.k = k;
+    g.transA = transA;
+    g.transB = transB;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.tile_off = 0;
+    g.tiles_n = cdiv(n, 64);
+    const int tiles = cdiv(m, 64) * g.tiles_n;
+    static GemmDesc *dg = nullptr;
+    static hipEvent_t ev = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (!dg && hipMalloc(&dg, sizeof(GemmDesc)) != hipSuccess) return PACO_ENOMEM;
+        if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return PACO_ENOMEM;
+        // the descriptor buffer is reused: the previous product must have consumed it
+        (void)hipEventSynchronize(ev);
+        hipError_t e = hipMemcpyAsync(dg, &g, sizeof(g), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return (int)e;
+        e = launch_gemm_grouped(dg, 1, tiles, s);
+        if (e != hipSuccess) return (int)e;
+        (void)hipEventRecord(ev, s);
+        (void)hipStreamSynchronize(s);
+    }
+    return 0;
+}
+
+void paco_release(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    (void)hipDeviceSynchronize();
+    for (auto &kv : g_graphs) {
+        if (kv.second->exec) (void)hipGraphExecDestroy(kv.second->exec);
+        if (kv.second->res.dev) (void)hipFree(kv.second->res.dev);
+        if (kv.second->res.pinned) (void)hipHostFree(kv.second->res.pinned);
+        if (kv.second->res.ev) (void)hipEventDestroy(kv.second->res.ev);
+        delete kv.second;
+    }
+    g_graphs.clear();
+    if (g_batch_res.dev) (void)hipFree(g_batch_res.dev);
+    if (g_batch_res.pinned) (void)hipHostFree(g_batch_res.pinned);
+    if (g_batch_res.ev) (void)hipEventDestroy(g_batch_res.ev);
+    g_batch_res = BatchRes();
+    if (g_ws) (void)hipFree(g_ws);
+    g_ws = nullptr;
+    g_ws_size = 0;
+}
+
+} // extern "C"

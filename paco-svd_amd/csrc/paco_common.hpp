@@ -1,0 +1,177 @@
+// Shared device/host definitions of the MI355X (gfx950) SVD engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PACO_WAVE 64
+
+namespace paco {
+
+// One SVD problem: the reference's global-memory buffers (main.cpp:33-48) plus
+// the engine's scratch.  All row-major FP32, M >= N.
+struct ProbDesc {
+    int M, N;
+    float *a;      // M*N   in: A; out of bidiag: reflectors; D&C scratch
+    float *u;      // M*M   Q, then left singular vectors
+    float *vt;     // N*N   P^T, then right singular vectors (rows)
+    float *d;      // N     diagonal, then sigma (descending)
+    float *e;      // N-1   superdiagonal
+    float *tauq;   // N
+    float *taup;   // N-1
+    float *qt;     // 2*N*N left vectors of D&C nodes | gathered bases
+    float *wt;     // 2*N*N right vectors of D&C nodes | gathered bases
+    uint32_t *idx; // 16*N  per-column arena of the merges
+    float *vmt;    // N*N   right secular vectors of the merges
+    float *tq;     // ceil(N/32)*1024 block reflector factors T (Q side)
+    float *tp;     // ceil(N/32)*1024 block reflector factors T (P side)
+    int *info;     // 1     LAPACK-style status: 0 ok, <0 bad input, >0 no convergence
+    int pad;
+};
+
+struct MergeState {
+    double scale;  // orgnrm
+    double tol;    // deflation tolerance (scaled units)
+    double nc, ns; // rotation combining the two null vectors
+    int k;         // non-deflated count
+    int nrot;      // deflation rotations
+    int n, m;      // merged rows / columns
+    int n1, m1, n2, m2;
+    int sq;
+    int pad[3];
+};
+
+static_assert(sizeof(MergeState) % 8 == 0, "MergeState alignment");
+
+// Grouped GEMM descriptor: C[crow(i)] = alpha * op(A)[i,:] * op(B)[:, j] (+ beta C)
+// with an optional gathered row index on B (row kk of B is brow[kk]) and a
+// scattered row index on C.  m and K may be read on the device (dyn), which is
+// how the merge GEMMs learn k after deflation.
+struct GemmDesc {
+    const float *A;
+    const float *B;
+    float *C;
+    const int *brow;
+    const int *crow;
+    const int *dyn_mk; // if set: m = K = *dyn_mk (bounded by m, k below)
+    int lda, ldb, ldc;
+    int m, n, k;
+    int transA, transB;
+    float alpha, beta;
+    int tile_off;      // prefix over descs of tiles
+    int tiles_n;       // tiles along n
+};
+
+// Arena carving of the idx buffer for the merge covering [c0, c0+n): all offsets in
+// 32-bit words from idx + 16*c0.  Doubles are 8-byte aligned because c0 and n are
+// multiplied by 16 and 2 respectively.
+struct Arena {
+    double *dk;   // [n] compacted poles (kept first, then deflated), scaled
+    double *zk;   // [n] compacted z for kept poles; Loewner z after that phase
+    double *mu;   // [n] per root: sigma^2 = dk[K]^2 + mu
+    int *Kx;      // [n] per root origin pole
+    int *src;     // [n] per sorted pole: basis source (see dc.hip)
+    int *order;   // [n] compacted index -> sorted pole index
+    int *pos;     // [n] compacted index -> output row (descending sigma)
+    int *rpair;   // [n] rotation t: (jp | j << 16)
+    float *rc;    // [n] rotation cosine
+    float *rs;    // [n] rotation sine
+    float *nu;    // [n] unused spare
+};
+
+__host__ __device__ inline Arena make_arena(uint32_t *idx, int c0, int n)
+{
+    uint32_t *base = idx + 16 * (size_t)c0;
+    Arena A;
+    A.dk = reinterpret_cast<double *>(base);
+    A.zk = reinterpret_cast<double *>(base + 2 * (size_t)n);
+    A.mu = reinterpret_cast<double *>(base + 4 * (size_t)n);
+    A.Kx = reinterpret_cast<int *>(base + 6 * (size_t)n);
+    A.src = reinterpret_cast<int *>(base + 7 * (size_t)n);
+    A.order = reinterpret_cast<int *>(base + 8 * (size_t)n);
+    A.pos = reinterpret_cast<int *>(base + 9 * (size_t)n);
+    A.rpair = reinterpret_cast<int *>(base + 10 * (size_t)n);
+    A.rc = reinterpret_cast<float *>(base + 11 * (size_t)n);
+    A.rs = reinterpret_cast<float *>(base + 12 * (size_t)n);
+    A.nu = reinterpret_cast<float *>(base + 13 * (size_t)n);
+    return A;
+}
+
+__device__ inline float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Binary search: index i of the group with offs[i] <= x < offs[i+1] (offs sorted).
+// A single group needs no offsets array (offs may be null).
+__host__ __device__ inline int find_task(const int *offs, int ntask, int x)
+{
+    if (ntask <= 1) return 0;
+    int lo = 0, hi = ntask - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (offs[mid] <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__host__ __device__ inline int off_at(const int *offs, int i) { return offs ? offs[i] : 0; }
+
+// Leaf partition of N columns (svd_tiling.cpp:83-130): L = N/3 + 1 leaves for N >= 3,
+// leaf l starts at 3l; the last starts one column early when N % 3 == 0.  N <= 2 is
+// one leaf.  Leaf l covers [leaf_start(l), leaf_start(l+1)) with leaf_start(L) = N.
+__host__ __device__ inline int leaf_count(int N) { return N <= 2 ? 1 : N / 3 + 1; }
+__host__ __device__ inline int leaf_start(int N, int l)
+{
+    const int L = leaf_count(N);
+    if (l >= L) return N;
+    if (L == 1) return 0;
+    if (l <= L - 2) return 3 * l;
+    return 3 * l - (N % 3 == 0 ? 1 : 0);
+}
+// Merges of round r >= 1 (pairs of nodes of round r-1, left to right, odd tail carried;
+// svd.cpp:168-189): node j of round r covers leaves [j 2^r, min((j+1) 2^r, L)); it is
+// a merge when its right half is not empty.
+__host__ __device__ inline int merges_in_round(int N, int r)
+{
+    const int L = leaf_count(N), span = 1 << r, half = span >> 1;
+    return L > half ? (L - half + span - 1) / span : 0;
+}
+__host__ __device__ inline int rounds_of(int N)
+{
+    int L = leaf_count(N), r = 0;
+    while ((1 << r) < L) ++r;
+    return r;
+}
+__host__ __device__ inline void merge_cols(int N, int r, int j, int &c0, int &c1, int &c2)
+{
+    const int L = leaf_count(N), span = 1 << r, half = span >> 1;
+    const int l0 = j * span, lm = l0 + half, l1 = (l0 + span < L) ? l0 + span : L;
+    c0 = leaf_start(N, l0);
+    c1 = leaf_start(N, lm);
+    c2 = leaf_start(N, l1);
+}
+
+} // namespace paco
+
+#define PACO_CHECK(x)                                                                              \
+    do {                                                                                           \
+        hipError_t _e = (x);                                                                       \
+        if (_e != hipSuccess) return (int)_e;                                                      \
+    } while (0)

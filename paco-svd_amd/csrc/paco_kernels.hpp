@@ -1,0 +1,61 @@
+// Kernel argument structs and host-side launchers of the gfx950 kernels.
+#pragma once
+#include "paco_common.hpp"
+
+namespace paco {
+
+// Problems of a launch: either one problem passed by value (plain entry points, no
+// device descriptor array needed) or a device array (batched path).
+struct ProbList {
+    const ProbDesc *ptr;
+    ProbDesc single;
+    int count;
+    __host__ __device__ inline ProbDesc get(int i) const { return ptr ? ptr[i] : single; }
+};
+
+// One D&C level (merge round r) over every problem of the launch.  Merge t belongs to
+// problem p with task_off[p] <= t < task_off[p+1] (task_off may be null for one
+// problem); its columns follow from (N, r, t - task_off[p]).  stride bounds the rows
+// and columns of every merge of the round (3 * 2^r).
+struct LevelArgs {
+    const int *task_off;
+    MergeState *states; // [ntask]
+    int ntask, r, stride;
+    int maxn; // largest merged row count of the level (sizes the deflation LDS)
+};
+
+struct CopyDesc {
+    const float *src;
+    float *dst;
+    int lds, ldd, rows, cols;
+    int row_off; // prefix over descs of rows
+};
+
+// bidiag.hip
+size_t bidiag_wg_lds_bytes(int M, int N);
+hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s);
+hipError_t launch_tfactor(ProbList pl, const int *blk_off, int nblocks, int side, hipStream_t s);
+hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int side, hipStream_t s);
+hipError_t launch_identity(ProbList pl, const int *row_off, int nrows, int side, hipStream_t s);
+hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hipStream_t s);
+
+// bidiag_grid.hip (large single matrices)
+size_t bidiag_grid_ws_bytes(int M, int N);
+hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s);
+
+// dc.hip
+hipError_t launch_dc_leaves(ProbList pl, const int *leaf_off, const uint32_t *blob_leaves, int nleaf,
+                            hipStream_t s);
+hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s);
+// back-transform: U[:, :N] <- U[:, :N] qt^T and Vt <- wt Vt for every problem
+hipError_t launch_dc_backtransform(ProbList pl, const int *tile_off_u, int ntiles_u, const int *tile_off_v,
+                                   int ntiles_v, const int *row_off_u, int nrows_u, const int *row_off_v,
+                                   int nrows_v, hipStream_t s);
+
+// gemm.hip
+hipError_t launch_gemm_grouped(const GemmDesc *descs, int ndesc, int ntiles, hipStream_t s);
+hipError_t launch_copy2d(const CopyDesc *descs, int ndesc, int nrows, hipStream_t s);
+void gemm_set_mode(int mode);
+int gemm_get_mode();
+
+} // namespace paco

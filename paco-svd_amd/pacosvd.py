@@ -1,0 +1,184 @@
+"""ctypes front end of libpacosvd.so for the Python harness (tests, bench, smoke).
+
+The product is the C ABI in include/paco_svd.h; this module only passes torch device
+pointers and the current HIP stream through it.  It never falls back to a CPU
+implementation: if the library is missing, loading raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpacosvd.so")
+
+EXPORTS = [
+    "paco_upper_bidiagonalization", "paco_svd_DC", "getTilingSize", "GenerateTiling",
+    "paco_workspace_size", "paco_set_precision", "paco_svd_plain", "paco_svd_batched",
+    "paco_svd_batched_graph", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
+]
+
+_lib = None
+_vp = ctypes.c_void_p
+
+
+class SvdJob(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int), ("A", _vp), ("S", _vp), ("U", _vp), ("Vt", _vp)]
+
+
+def lib():
+    """Load libpacosvd.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError("libpacosvd.so not built: run python paco-svd_amd/build.py (%s)" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    L.paco_upper_bidiagonalization.argtypes = [ctypes.c_uint, _vp, ctypes.c_int, ctypes.c_int] + [_vp] * 8
+    L.paco_svd_DC.argtypes = [ctypes.c_uint, _vp, ctypes.c_int, ctypes.c_int] + [_vp] * 10
+    L.getTilingSize.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.getTilingSize.restype = ctypes.c_int
+    L.GenerateTiling.argtypes = [ctypes.c_int, ctypes.c_int, _vp]
+    L.GenerateTiling.restype = None
+    L.paco_workspace_size.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.paco_workspace_size.restype = ctypes.c_size_t
+    L.paco_set_precision.argtypes = [ctypes.c_int]
+    L.paco_svd_plain.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, _vp]
+    L.paco_svd_batched.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, _vp]
+    L.paco_svd_batched_graph.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, _vp]
+    L.paco_batch_workspace_size.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    L.paco_batch_workspace_size.restype = ctypes.c_size_t
+    L.paco_last_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, _vp]
+    L.paco_gemm_f32.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_float, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, _vp, ctypes.c_int]
+    L.paco_release.argtypes = []
+    L.paco_release.restype = None
+    _lib = L
+    return L
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed with code %d" % (what, rc))
+
+
+def tiling_blob(N, svd_block_num=1):
+    size = lib().getTilingSize(N, svd_block_num)
+    buf = (ctypes.c_uint8 * size)()
+    lib().GenerateTiling(N, svd_block_num, buf)
+    return bytes(buf)
+
+
+def workspace(M, N, device="cuda"):
+    import torch
+    return torch.zeros(max(1, lib().paco_workspace_size(M, N)), dtype=torch.uint8, device=device)
+
+
+def upper_bidiagonalization(a, ws=None):
+    """Entry point (a) on device tensor a (M x N fp32, overwritten).  Returns dict of
+    device tensors: a, u, vt, d, e, tauq, taup."""
+    import torch
+    M, N = a.shape
+    dev = a.device
+    u = torch.zeros((M, M), dtype=torch.float32, device=dev)
+    vt = torch.zeros((N, N), dtype=torch.float32, device=dev)
+    d = torch.zeros(N, dtype=torch.float32, device=dev)
+    e = torch.zeros(max(N - 1, 1), dtype=torch.float32, device=dev)
+    tq = torch.zeros(N, dtype=torch.float32, device=dev)
+    tp = torch.zeros(max(N - 1, 1), dtype=torch.float32, device=dev)
+    if ws is None:
+        ws = workspace(M, N, dev)
+    rc = lib().paco_upper_bidiagonalization(40, _stream(), M, N, _ptr(a), _ptr(u), _ptr(vt), _ptr(d), _ptr(e),
+                                            _ptr(tq), _ptr(tp), _ptr(ws))
+    _check(rc, "paco_upper_bidiagonalization")
+    return dict(a=a, u=u, vt=vt, d=d, e=e[: N - 1], tauq=tq, taup=tp[: N - 1], ws=ws)
+
+
+def svd_DC(M, N, a, u, vt, d, e, ws=None, use_tiling=True):
+    """Entry point (b): d <- sigma, u, vt <- singular vectors (in place)."""
+    import torch
+    dev = u.device
+    qt = torch.zeros(2 * N * N, dtype=torch.float32, device=dev)
+    wt = torch.zeros(2 * N * N, dtype=torch.float32, device=dev)
+    idx = torch.zeros(16 * N, dtype=torch.int32, device=dev)
+    if ws is None:
+        ws = workspace(M, N, dev)
+    tiling = None
+    if use_tiling:
+        blob = tiling_blob(N)
+        tiling = torch.frombuffer(bytearray(blob), dtype=torch.uint8).to(dev)
+    if e.numel() < max(N - 1, 1):
+        e = torch.zeros(1, dtype=torch.float32, device=dev)
+    rc = lib().paco_svd_DC(1, _stream(), M, N, _ptr(a), _ptr(u), _ptr(vt), _ptr(d), _ptr(e), _ptr(qt), _ptr(wt),
+                           _ptr(idx), _ptr(ws), _ptr(tiling))
+    _check(rc, "paco_svd_DC")
+    return d, u, vt
+
+
+def _jobs(As, outs):
+    jobs = (SvdJob * len(As))()
+    for i, (A, (S, U, Vt)) in enumerate(zip(As, outs)):
+        jobs[i].M, jobs[i].N = A.shape
+        jobs[i].A = A.data_ptr()
+        jobs[i].S = S.data_ptr()
+        jobs[i].U = U.data_ptr()
+        jobs[i].Vt = Vt.data_ptr()
+    return jobs
+
+
+def alloc_outputs(As):
+    import torch
+    outs = []
+    for A in As:
+        M, N = A.shape
+        outs.append((torch.zeros(N, dtype=torch.float32, device=A.device),
+                     torch.zeros((M, M), dtype=torch.float32, device=A.device),
+                     torch.zeros((N, N), dtype=torch.float32, device=A.device)))
+    return outs
+
+
+def svd_plain(As, outs=None):
+    """Plain path: each matrix through the two entry points.  Returns [(S, U, Vt)]."""
+    outs = outs if outs is not None else alloc_outputs(As)
+    _check(lib().paco_svd_plain(_jobs(As, outs), len(As), _stream()), "paco_svd_plain")
+    return outs
+
+
+def svd_batched(As, outs=None, graph=False):
+    """Batched path (one launch per phase over all matrices)."""
+    outs = outs if outs is not None else alloc_outputs(As)
+    jobs = _jobs(As, outs)
+    fn = lib().paco_svd_batched_graph if graph else lib().paco_svd_batched
+    _check(fn(jobs, len(As), _stream()), "paco_svd_batched")
+    return outs
+
+
+def last_info(count):
+    buf = (ctypes.c_int * count)()
+    n = lib().paco_last_info(buf, count, _stream())
+    return list(buf[:n])
+
+
+def gemm(A, B, C=None, transA=False, transB=False, alpha=1.0, beta=0.0):
+    """C = alpha op(A) op(B) + beta C with the engine's GEMM tile (row-major fp32)."""
+    import torch
+    m = A.shape[1] if transA else A.shape[0]
+    k = A.shape[0] if transA else A.shape[1]
+    n = B.shape[0] if transB else B.shape[1]
+    if C is None:
+        C = torch.zeros((m, n), dtype=torch.float32, device=A.device)
+    rc = lib().paco_gemm_f32(_stream(), int(transA), int(transB), m, n, k, alpha, _ptr(A), A.stride(0), _ptr(B),
+                             B.stride(0), beta, _ptr(C), C.stride(0))
+    _check(rc, "paco_gemm_f32")
+    return C
+
+
+def set_precision(mode):
+    return lib().paco_set_precision(mode)

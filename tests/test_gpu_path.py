@@ -1,0 +1,203 @@
+"""GPU parity of the HIP path (through the C ABI) against the oracle and FP64 LAPACK.
+
+Bars (stated per result): sigma normwise error max|s - s*| / s*_max <= 1e-5, entry-wise
+orthogonality max|U^T U - I|, max|V^T V - I| <= 1e-5, residual ratio
+||A - U S Vt||_F / (||A||_F max(M, N) eps32) <= 10.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from svdcheck import measures
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+SIG_TOL = 1e-5
+ORTH_TOL = 1e-5
+RESID_TOL = 10.0
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    import pacosvd
+    pacosvd.lib()
+    oracle.build()
+    return torch
+
+
+def np_of(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("shape", [(64, 64, 64, False, False), (100, 37, 73, False, True), (130, 65, 129, True, False),
+                                   (5, 200, 17, True, True), (257, 96, 300, False, False)])
+def test_gemm_tile(torch_dev, mode, shape):
+    torch = torch_dev
+    import pacosvd
+    m, n, k, ta, tb = shape
+    say("gemm", shape, "mode", mode)
+    g = torch.Generator().manual_seed(m * 7 + n)
+    A = torch.randn((k, m) if ta else (m, k), generator=g)
+    B = torch.randn((n, k) if tb else (k, n), generator=g)
+    C0 = torch.randn((m, n), generator=g)
+    old = pacosvd.set_precision(mode)
+    try:
+        C = pacosvd.gemm(A.cuda(), B.cuda(), C0.clone().cuda(), transA=ta, transB=tb, alpha=0.5, beta=-1.0)
+        torch.cuda.synchronize()
+    finally:
+        pacosvd.set_precision(old)
+    Ad = (A.T if ta else A).double()
+    Bd = (B.T if tb else B).double()
+    ref = 0.5 * (Ad @ Bd) - C0.double()
+    err = (C.cpu().double() - ref).abs().max().item()
+    scale = (Ad.abs() @ Bd.abs()).max().item()
+    assert err <= (1e-6 if mode != 1 else 4e-6) * scale, err
+
+
+def run_entry_points(torch, A_np):
+    import pacosvd
+    M, N = A_np.shape
+    a = torch.from_numpy(A_np.copy()).cuda()
+    r = pacosvd.upper_bidiagonalization(a)
+    torch.cuda.synchronize()
+    bid = {k: np_of(v) for k, v in r.items() if k != "ws"}
+    d, u, vt = pacosvd.svd_DC(M, N, r["a"], r["u"], r["vt"], r["d"], r["e"], ws=r["ws"])
+    torch.cuda.synchronize()
+    return bid, np_of(d), np_of(u), np_of(vt)
+
+
+def test_c1_entry_points(torch_dev):
+    torch = torch_dev
+    say("c1 128x96 entry points")
+    g = np.load(os.path.join(GOLD, "c1.npz"))
+    A = g["A"]
+    bid, s, U, Vt = run_entry_points(torch, A)
+    orc = oracle.bidiag(A.astype(np.float64))
+    smax = g["sigma"][0]
+    # entry point (a): d, e unique (non-negative convention) -> compare with the oracle
+    assert np.max(np.abs(bid["d"] - orc["d"])) <= 5e-5 * smax
+    assert np.max(np.abs(bid["e"] - orc["e"])) <= 5e-5 * smax
+    assert np.max(np.abs(bid["tauq"] - orc["tauq"])) <= 1e-4
+    assert np.max(np.abs(bid["u"].T @ bid["u"] - np.eye(128))) <= ORTH_TOL
+    assert np.max(np.abs(bid["vt"] @ bid["vt"].T - np.eye(96))) <= ORTH_TOL
+    M, N = A.shape
+    B = np.zeros((M, N))
+    B[np.arange(N), np.arange(N)] = bid["d"]
+    B[np.arange(N - 1), np.arange(1, N)] = bid["e"]
+    assert np.linalg.norm(bid["u"] @ B @ bid["vt"] - A) <= 1e-5 * np.linalg.norm(A)
+    # entry point (b): whole SVD vs golden FP64 dgesdd and the oracle's sigma
+    m = measures(A, s, U, Vt, s_ref=g["sigma"])
+    say("c1 measures", m)
+    assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL
+    assert m["resid_ratio"] <= RESID_TOL
+    s_orc, _, _ = oracle.svd(A.astype(np.float64))
+    assert np.max(np.abs(s - s_orc)) <= SIG_TOL * smax
+    assert np.all(np.diff(s) <= 0)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 1), (2, 2), (3, 3), (4, 3), (5, 4), (6, 6), (7, 5), (9, 8), (33, 17),
+                                   (64, 64), (100, 1), (150, 149), (300, 200), (512, 512)])
+def test_entry_points_shapes(torch_dev, shape):
+    torch = torch_dev
+    say("entry points", shape)
+    rng = np.random.default_rng(shape[0] * 1000 + shape[1])
+    A = rng.standard_normal(shape).astype(np.float32)
+    _, s, U, Vt = run_entry_points(torch, A)
+    m = measures(A, s, U, Vt)
+    say("measures", m)
+    assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL
+    assert m["resid_ratio"] <= RESID_TOL
+
+
+@pytest.mark.parametrize("kind", ["zero", "rank1", "clustered", "graded"])
+def test_entry_points_special(torch_dev, kind):
+    torch = torch_dev
+    say("special", kind)
+    rng = np.random.default_rng(1)
+    M, N = 120, 90
+    if kind == "zero":
+        A = np.zeros((M, N), np.float32)
+    elif kind == "rank1":
+        A = np.outer(rng.standard_normal(M), rng.standard_normal(N)).astype(np.float32)
+    else:
+        if kind == "clustered":
+            sig = np.concatenate([1.0 + 1e-7 * rng.uniform(size=60), 10.0 ** rng.uniform(-3, 0, N - 60)])
+        else:
+            sig = 10.0 ** (-6 * np.arange(N) / (N - 1))
+        Q1, _ = np.linalg.qr(rng.standard_normal((M, N)))
+        Q2, _ = np.linalg.qr(rng.standard_normal((N, N)))
+        A = ((Q1 * sig) @ Q2.T).astype(np.float32)
+    _, s, U, Vt = run_entry_points(torch, A)
+    m = measures(A, s, U, Vt)
+    say("measures", m)
+    assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL
+    if kind != "zero":
+        assert m["resid_ratio"] <= RESID_TOL
+
+
+def mixed_batch(rng, count, mmax=200):
+    shapes = []
+    for _ in range(count):
+        M = int(rng.integers(16, mmax + 1))
+        N = int(rng.integers(8, M + 1))
+        shapes.append((M, N))
+    return [rng.standard_normal(sh).astype(np.float32) for sh in shapes]
+
+
+def test_batched_matches_plain_and_alone(torch_dev):
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(3)
+    mats = mixed_batch(rng, 24)
+    say("batched", [m.shape for m in mats])
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    outs_b = pacosvd.svd_batched(As)
+    outs_p = pacosvd.svd_plain(As)
+    torch.cuda.synchronize()
+    assert pacosvd.last_info(len(As)) == [0] * len(As)
+    for i, A in enumerate(mats):
+        sb, Ub, Vb = (np_of(x) for x in outs_b[i])
+        sp = np_of(outs_p[i][0])
+        mb = measures(A, sb, Ub, Vb)
+        assert mb["sigma_err"] <= SIG_TOL and mb["orth_u"] <= ORTH_TOL and mb["orth_v"] <= ORTH_TOL, (i, mb)
+        assert mb["resid_ratio"] <= RESID_TOL
+        assert np.max(np.abs(sb - sp)) <= SIG_TOL * max(sb[0], 1e-30)
+    # alone vs in the mix: a matrix's schedule depends only on its shape -> identical bits
+    for i in (0, 5, 11):
+        alone = pacosvd.svd_batched([As[i]])
+        torch.cuda.synchronize()
+        for x, y in zip(alone[0], outs_b[i]):
+            assert torch.equal(x, y), i
+
+
+def test_batched_graph_replays(torch_dev):
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(9)
+    mats = mixed_batch(rng, 12, mmax=120)
+    say("graph", [m.shape for m in mats])
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    eager = pacosvd.svd_batched(As)
+    torch.cuda.synchronize()
+    outs = pacosvd.alloc_outputs(As)
+    for rep in range(3):
+        for o in outs:
+            for t in o:
+                t.zero_()
+        pacosvd.svd_batched(As, outs=outs, graph=True)
+        torch.cuda.synchronize()
+        _ = (torch.randn(256, 256, device="cuda") @ torch.randn(256, 256, device="cuda")).sum().item()
+        for i in range(len(As)):
+            for x, y in zip(outs[i], eager[i]):
+                assert torch.equal(x, y), (rep, i)
