@@ -107,6 +107,15 @@ int paco_last_info(int *info_host, int count, void *stream);
 int paco_gemm_f32(void *stream, int transA, int transB, int m, int n, int k, float alpha, const float *A, int lda,
                   const float *B, int ldb, float beta, float *C, int ldc);
 
+/* Phase timing of the batched path with hipEvents recorded on the caller's stream
+ * (phases: 0 copy-in, 1 bidiagonalization, 2 Q/P^T formation, 3 D&C leaves + merges,
+ * 4 back-transforms).  paco_timing_enable(1) turns recording on for later
+ * paco_svd_batched calls; paco_timing_read synchronises on the last call's events and
+ * writes up to max durations in milliseconds; returns the number written.
+ * No counterpart in the reference (it has no timers, main.cpp:1). */
+int paco_timing_enable(int on);
+int paco_timing_read(float *ms, int max);
+
 /* Release the library's pools (device memory, pinned staging, graphs). */
 void paco_release(void);
 

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
 {
     const int pid = order ? order[blockIdx.x] : blockIdx.x;
     const ProbDesc P = pl.get(pid);
-    const int M = P.M, N = P.N;
+    const int M = P.M, N = P.N, ld = P.ld ? P.ld : P.N;
     float *__restrict__ A = P.a;
     extern __shared__ float sm[];
     float *vbuf = sm, *wbuf = vbuf + M, *ubuf = wbuf + N, *red = ubuf + N;
@@ -67,23 +67,23 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
         // ---- column reflector on A[i:, i]
         float part = 0.f;
         for (int r = i + 1 + tid; r < M; r += BT) {
-            float x = A[(size_t)r * N + i];
+            float x = A[(size_t)r * ld + i];
             vbuf[r] = x;
             part += x * x;
         }
         const float sigma = block_sum(part, red);
-        const float x1 = A[(size_t)i * N + i];
+        const float x1 = A[(size_t)i * ld + i];
         float tau, beta, inv;
         house_params(x1, sigma, &beta, &tau, &inv);
         if (tau != 0.f) {
             for (int r = i + 1 + tid; r < M; r += BT) {
                 float v = vbuf[r] * inv;
                 vbuf[r] = v;
-                A[(size_t)r * N + i] = v;
+                A[(size_t)r * ld + i] = v;
             }
         }
         if (tid == 0) {
-            A[(size_t)i * N + i] = beta;
+            A[(size_t)i * ld + i] = beta;
             P.d[i] = beta;
             P.tauq[i] = tau;
             vbuf[i] = 1.f;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
         } else if (nc >= BT) {
             for (int j = i + 1 + tid; j < N; j += BT) {
                 float acc = 0.f;
-                for (int r = i; r < M; ++r) acc += vbuf[r] * A[(size_t)r * N + j];
+                for (int r = i; r < M; ++r) acc += vbuf[r] * A[(size_t)r * ld + j];
                 wbuf[j] = tau * acc;
             }
         } else {
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
             float acc = 0.f;
             if (c < nc) {
                 const int j = i + 1 + c;
-                for (int r = i + g; r < M; r += rg) acc += vbuf[r] * A[(size_t)r * N + j];
+                for (int r = i + g; r < M; r += rg) acc += vbuf[r] * A[(size_t)r * ld + j];
             }
             red[tid] = acc;
             __syncthreads();
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
         // ---- row i: apply left reflector, then row reflector on A[i, i+1:]
         part = 0.f;
         for (int j = i + 1 + tid; j < N; j += BT) {
-            float x = A[(size_t)i * N + j] - wbuf[j];
+            float x = A[(size_t)i * ld + j] - wbuf[j];
             ubuf[j] = x;
             if (j > i + 1) part += x * x;
         }
@@ -134,12 +134,12 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
         __syncthreads();
         for (int j = i + 1 + tid; j < N; j += BT) {
             if (j == i + 1) {
-                A[(size_t)i * N + j] = beta2;
+                A[(size_t)i * ld + j] = beta2;
                 ubuf[j] = 1.f;
             } else {
                 float uval = (tau2 != 0.f) ? ubuf[j] * inv2 : ubuf[j];
                 ubuf[j] = uval;
-                A[(size_t)i * N + j] = uval;
+                A[(size_t)i * ld + j] = uval;
             }
         }
         if (tid == 0) {
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
         const bool in_regs = nc <= 8 * 64;
         for (int r = i + 1 + wv; r < M; r += BT / 64) {
             const float vr = (tau != 0.f) ? vbuf[r] : 0.f;
-            float *row = A + (size_t)r * N;
+            float *row = A + (size_t)r * ld;
             float acc = 0.f;
             float regs[8];
             if (in_regs) {
@@ -387,11 +387,8 @@ size_t bidiag_wg_lds_bytes(int M, int N) { return sizeof(float) * ((size_t)M + 2
 
 hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s)
 {
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)bidiag_wg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
-    }
+    static size_t maxdyn = allow_dynamic_lds((const void *)bidiag_wg_kernel);
+    if (lds > maxdyn) return hipErrorInvalidValue;
     hipLaunchKernelGGL(bidiag_wg_kernel, dim3(count), dim3(BT), lds, s, pl, order);
     return hipGetLastError();
 }

@@ -732,14 +732,10 @@ static void launch_merge_gemms(ProbList pl, LevelArgs la, hipStream_t s)
 
 hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s)
 {
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)dc_deflate_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
-    }
+    static size_t maxdyn = allow_dynamic_lds((const void *)dc_deflate_kernel);
     if (la.ntask <= 0) return hipSuccess;
     const size_t lds = (size_t)la.maxn * (2 * sizeof(double) + sizeof(int)) + (DT + 1) * sizeof(int);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (lds > maxdyn) return hipErrorInvalidValue;
     const int spt = strips_per_task(la);
     const int rows = la.ntask * la.stride;
     const int wave_blocks = (rows + DT / 64 - 1) / (DT / 64);

@@ -169,7 +169,7 @@ struct DcOffsets {
     int ntu = 0, ntv = 0, nru = 0, nrv = 0;
 };
 
-static hipError_t run_dc(ProbList pl, const DcOffsets &o, const uint32_t *blob_leaves, hipStream_t s)
+static hipError_t run_dc_levels(ProbList pl, const DcOffsets &o, const uint32_t *blob_leaves, hipStream_t s)
 {
     hipError_t e = launch_dc_leaves(pl, o.leaf_off, blob_leaves, o.nleaf, s);
     if (e != hipSuccess) return e;
@@ -183,7 +183,19 @@ static hipError_t run_dc(ProbList pl, const DcOffsets &o, const uint32_t *blob_l
         la.maxn = o.maxn[r - 1];
         if ((e = launch_dc_level(pl, la, s)) != hipSuccess) return e;
     }
+    return hipSuccess;
+}
+
+static hipError_t run_dc_backtransform(ProbList pl, const DcOffsets &o, hipStream_t s)
+{
     return launch_dc_backtransform(pl, o.bt_tu, o.ntu, o.bt_tv, o.ntv, o.bt_ru, o.nru, o.bt_rv, o.nrv, s);
+}
+
+static hipError_t run_dc(ProbList pl, const DcOffsets &o, const uint32_t *blob_leaves, hipStream_t s)
+{
+    hipError_t e = run_dc_levels(pl, o, blob_leaves, s);
+    if (e != hipSuccess) return e;
+    return run_dc_backtransform(pl, o, s);
 }
 
 static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, float *e, float *qt, float *wt,
@@ -447,9 +459,22 @@ static hipError_t prepare_batch(const paco_svd_job *jobs, int count, BatchPlan &
     return hipSuccess;
 }
 
+// Phase events of the batched path (recorded only outside graph capture).
+static const int kPhases = 5;
+static bool g_timing = false;
+static hipEvent_t g_tev[kPhases + 1] = {};
+static bool g_tev_recorded = false;
+
+static void tmark(hipStream_t s, int i, bool on)
+{
+    if (on && g_tev[i]) (void)hipEventRecord(g_tev[i], s);
+}
+
 // Upload + the whole launch sequence (capturable: no allocation, no host sync).
 static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepared &pr, hipStream_t s, bool record)
 {
+    const bool tm = record && g_timing;
+    if (tm) g_tev_recorded = true;
     const int count = bp.count;
     char *dev = static_cast<char *>(res.dev);
     hipError_t e;
@@ -460,13 +485,17 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     pl.ptr = reinterpret_cast<const ProbDesc *>(dev + bp.o_probs);
     pl.count = count;
     memset(&pl.single, 0, sizeof(pl.single));
+    tmark(s, 0, tm);
     hipLaunchKernelGGL(zero_ints_kernel, dim3(cdiv(count, 256)), dim3(256), 0, s, pr.info, count);
     hipLaunchKernelGGL(copy_in_kernel, dim3(bp.nrowin), dim3(256), 0, s, pl,
                        reinterpret_cast<const float *const *>(dev + bp.o_src), I(bp.o_rowin));
+    tmark(s, 1, tm);
     if ((e = launch_bidiag_wg(pl, I(bp.o_order), count, bp.lds_bidiag, s)) != hipSuccess) return e;
+    tmark(s, 2, tm);
     if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
                        I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)
         return e;
+    tmark(s, 3, tm);
     DcOffsets o;
     o.leaf_off = I(bp.o_leaf);
     o.nleaf = bp.nleaf;
@@ -485,7 +514,11 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     o.nru = bp.nru;
     o.bt_rv = I(bp.o_rv);
     o.nrv = bp.nrv;
-    return run_dc(pl, o, nullptr, s);
+    if ((e = run_dc_levels(pl, o, nullptr, s)) != hipSuccess) return e;
+    tmark(s, 4, tm);
+    e = run_dc_backtransform(pl, o, s);
+    tmark(s, 5, tm);
+    return e;
 }
 
 static thread_local int *t_last_info = nullptr;
@@ -737,6 +770,30 @@ int paco_gemm_f32(void *stream, int transA, int transB, int m, int n, int k, flo
         (void)hipStreamSynchronize(s);
     }
     return 0;
+}
+
+int paco_timing_enable(int on)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (on) {
+        for (int i = 0; i <= kPhases; ++i)
+            if (!g_tev[i] && hipEventCreate(&g_tev[i]) != hipSuccess) return PACO_ENOMEM;
+    }
+    g_timing = on != 0;
+    return 0;
+}
+
+int paco_timing_read(float *ms, int max)
+{
+    if (!g_tev_recorded || !ms) return 0;
+    if (hipEventSynchronize(g_tev[kPhases]) != hipSuccess) return 0;
+    int n = 0;
+    for (int i = 0; i < kPhases && i < max; ++i) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, g_tev[i], g_tev[i + 1]) != hipSuccess) t = -1.f;
+        ms[n++] = t;
+    }
+    return n;
 }
 
 void paco_release(void)

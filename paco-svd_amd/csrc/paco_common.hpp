@@ -25,7 +25,7 @@ struct ProbDesc {
     float *tq;     // ceil(N/32)*1024 block reflector factors T (Q side)
     float *tp;     // ceil(N/32)*1024 block reflector factors T (P side)
     int *info;     // 1     LAPACK-style status: 0 ok, <0 bad input, >0 no convergence
-    int pad;
+    int ld;        // row stride of a for the bidiagonalization (0: N)
 };
 
 struct MergeState {

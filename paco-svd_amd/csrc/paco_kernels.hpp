@@ -31,6 +31,24 @@ struct CopyDesc {
     int row_off; // prefix over descs of rows
 };
 
+// Raise a kernel's dynamic-LDS limit to what the device allows beyond the kernel's
+// static LDS; returns that limit (bytes).  Clears the runtime's last error.
+inline size_t allow_dynamic_lds(const void *func)
+{
+    int dev = 0, optin = 0, basic = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev);
+    (void)hipDeviceGetAttribute(&basic, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    const int maxb = optin > basic ? optin : basic;
+    hipFuncAttributes fa;
+    size_t stat = 0;
+    if (hipFuncGetAttributes(&fa, func) == hipSuccess) stat = fa.sharedSizeBytes;
+    const size_t dyn = maxb > (int)stat ? (size_t)maxb - stat : 0;
+    (void)hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    (void)hipGetLastError();
+    return dyn;
+}
+
 // bidiag.hip
 size_t bidiag_wg_lds_bytes(int M, int N);
 hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s);

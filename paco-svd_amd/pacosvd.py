@@ -14,7 +14,9 @@ EXPORTS = [
     "paco_upper_bidiagonalization", "paco_svd_DC", "getTilingSize", "GenerateTiling",
     "paco_workspace_size", "paco_set_precision", "paco_svd_plain", "paco_svd_batched",
     "paco_svd_batched_graph", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
+    "paco_timing_enable", "paco_timing_read",
 ]
+PHASES = ["copy_in", "bidiag", "form_qp", "dc", "backtransform"]
 
 _lib = None
 _vp = ctypes.c_void_p
@@ -49,6 +51,8 @@ def lib():
     L.paco_last_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, _vp]
     L.paco_gemm_f32.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_float, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, _vp, ctypes.c_int]
+    L.paco_timing_enable.argtypes = [ctypes.c_int]
+    L.paco_timing_read.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int]
     L.paco_release.argtypes = []
     L.paco_release.restype = None
     _lib = L
@@ -182,3 +186,14 @@ def gemm(A, B, C=None, transA=False, transB=False, alpha=1.0, beta=0.0):
 
 def set_precision(mode):
     return lib().paco_set_precision(mode)
+
+
+def timing_enable(on=True):
+    _check(lib().paco_timing_enable(1 if on else 0), "paco_timing_enable")
+
+
+def timing_read():
+    """Per-phase milliseconds of the last batched call (dict), via hipEvents on its stream."""
+    buf = (ctypes.c_float * len(PHASES))()
+    n = lib().paco_timing_read(buf, len(PHASES))
+    return {PHASES[i]: float(buf[i]) for i in range(n)}

@@ -120,6 +120,34 @@ def test_entry_points_shapes(torch_dev, shape):
     assert m["resid_ratio"] <= RESID_TOL
 
 
+@pytest.mark.parametrize("shape", [(1100, 1000), (2048, 700)])
+def test_grid_bidiag_vs_oracle(torch_dev, shape):
+    """Large matrices take the chip-wide panel bidiagonalization (M*N >= 2^20)."""
+    torch = torch_dev
+    import pacosvd
+    say("grid bidiag", shape)
+    rng = np.random.default_rng(shape[1])
+    A = rng.standard_normal(shape).astype(np.float32)
+    a = torch.from_numpy(A.copy()).cuda()
+    r = pacosvd.upper_bidiagonalization(a)
+    torch.cuda.synchronize()
+    d, e = np_of(r["d"]), np_of(r["e"])
+    orc = oracle.bidiag(A.astype(np.float64))
+    an = np.linalg.norm(A, 2)
+    say("d err", np.max(np.abs(d - orc["d"])) / an, "e err", np.max(np.abs(e - orc["e"])) / an)
+    assert np.max(np.abs(d - orc["d"])) <= 1e-4 * an
+    assert np.max(np.abs(e - orc["e"])) <= 1e-4 * an
+    assert np.max(np.abs(np_of(r["tauq"]) - orc["tauq"])) <= 1e-3
+    u, vt = np_of(r["u"]), np_of(r["vt"])
+    M, N = shape
+    assert np.max(np.abs(u.T @ u - np.eye(M))) <= ORTH_TOL
+    assert np.max(np.abs(vt @ vt.T - np.eye(N))) <= ORTH_TOL
+    B = np.zeros((M, N))
+    B[np.arange(N), np.arange(N)] = d
+    B[np.arange(N - 1), np.arange(1, N)] = e
+    assert np.linalg.norm(u @ B @ vt - A) <= 1e-5 * np.linalg.norm(A)
+
+
 @pytest.mark.parametrize("kind", ["zero", "rank1", "clustered", "graded"])
 def test_entry_points_special(torch_dev, kind):
     torch = torch_dev
