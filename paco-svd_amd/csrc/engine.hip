@@ -684,14 +684,21 @@ int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
             return (e == hipErrorInvalidValue) ? PACO_EINVAL : (int)e;
         }
         g->info = pr.info;
+        // record on a library stream (the caller's may be the legacy null stream, which
+        // cannot be captured); the graph is launched on the caller's stream below
+        static hipStream_t cap = nullptr;
+        if (!cap && (e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)) != hipSuccess) {
+            delete g;
+            return (int)e;
+        }
         hipGraph_t graph;
-        e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) {
             delete g;
             return (int)e;
         }
-        e = enqueue_batch(g->plan, g->res, pr, s, false);
-        hipError_t e2 = hipStreamEndCapture(s, &graph);
+        e = enqueue_batch(g->plan, g->res, pr, cap, false);
+        hipError_t e2 = hipStreamEndCapture(cap, &graph);
         if (e != hipSuccess || e2 != hipSuccess) {
             delete g;
             return (int)(e != hipSuccess ? e : e2);
