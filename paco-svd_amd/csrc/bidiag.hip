@@ -50,18 +50,60 @@ __device__ inline void house_params(float x1, float sigma, float *beta, float *t
 }
 
 // ---------------------------------------------------------------------------
-// One workgroup per matrix.  LDS: vbuf[M] (current column reflector), wbuf[N]
-// (tau_q * v^T A), ubuf[N] (current row reflector), red[BT].
+// One workgroup per matrix.  LDS: [matrix M x ldl (LDS-resident variant only)]
+// vbuf[M] (current column reflector), wbuf[N] (tau_q v^T A), ubuf[N] (current row
+// reflector), red[BT].  The LDS-resident variant stages the whole matrix in LDS
+// (odd leading dimension: column accesses hit distinct banks) and writes it back at
+// the end; the global variant streams from L2/HBM with 8-deep unrolled loads and four
+// rows per wavefront in flight.
 // ---------------------------------------------------------------------------
+template <int STEP>
+__device__ inline float col_dot(const float *A, int ld, int j, int r0, int r1, const float *v)
+{
+    float acc = 0.f;
+    int r = r0;
+    for (; r + 8 * STEP <= r1; r += 8 * STEP) {
+        float a[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = A[(size_t)(r + k * STEP) * ld + j];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[r + k * STEP] * a[k];
+    }
+    for (; r < r1; r += STEP) acc += v[r] * A[(size_t)r * ld + j];
+    return acc;
+}
+
+__device__ inline float col_dot_step(const float *A, int ld, int j, int r0, int r1, int step, const float *v)
+{
+    float acc = 0.f;
+    int r = r0;
+    for (; r + 7 * step < r1; r += 8 * step) {
+        float a[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = A[(size_t)(r + k * step) * ld + j];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[r + k * step] * a[k];
+    }
+    for (; r < r1; r += step) acc += v[r] * A[(size_t)r * ld + j];
+    return acc;
+}
+
+template <bool LDSRES>
 __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *order)
 {
     const int pid = order ? order[blockIdx.x] : blockIdx.x;
     const ProbDesc P = pl.get(pid);
-    const int M = P.M, N = P.N, ld = P.ld ? P.ld : P.N;
-    float *__restrict__ A = P.a;
+    const int M = P.M, N = P.N, ldg = P.ld ? P.ld : P.N;
     extern __shared__ float sm[];
-    float *vbuf = sm, *wbuf = vbuf + M, *ubuf = wbuf + N, *red = ubuf + N;
+    const int ld = LDSRES ? (N | 1) : ldg;
+    float *__restrict__ A = LDSRES ? sm : P.a;
+    float *vbuf = sm + (LDSRES ? (size_t)M * ld : 0), *wbuf = vbuf + M, *ubuf = wbuf + N, *red = ubuf + N;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (LDSRES) {
+        for (int r = wv; r < M; r += BT / 64)
+            for (int c = lane; c < N; c += 64) A[(size_t)r * ld + c] = P.a[(size_t)r * ldg + c];
+        __syncthreads();
+    }
 
     for (int i = 0; i < N; ++i) {
         // ---- column reflector on A[i:, i]
@@ -96,20 +138,13 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
         if (tau == 0.f) {
             for (int j = i + 1 + tid; j < N; j += BT) wbuf[j] = 0.f;
         } else if (nc >= BT) {
-            for (int j = i + 1 + tid; j < N; j += BT) {
-                float acc = 0.f;
-                for (int r = i; r < M; ++r) acc += vbuf[r] * A[(size_t)r * ld + j];
-                wbuf[j] = tau * acc;
-            }
+            for (int j = i + 1 + tid; j < N; j += BT) wbuf[j] = tau * col_dot<1>(A, ld, j, i, M, vbuf);
         } else {
             int tc = 1;
             while (tc < nc) tc <<= 1;
             const int rg = BT / tc, g = tid / tc, c = tid % tc;
             float acc = 0.f;
-            if (c < nc) {
-                const int j = i + 1 + c;
-                for (int r = i + g; r < M; r += rg) acc += vbuf[r] * A[(size_t)r * ld + j];
-            }
+            if (c < nc) acc = col_dot_step(A, ld, i + 1 + c, i + g, M, rg, vbuf);
             red[tid] = acc;
             __syncthreads();
             if (g == 0 && c < nc) {
@@ -148,43 +183,68 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
         }
         __syncthreads();
 
-        // ---- rows r > i: left update (v_r, w), then right update (u, tau2); one wave per row
-        const bool in_regs = nc <= 8 * 64;
-        for (int r = i + 1 + wv; r < M; r += BT / 64) {
-            const float vr = (tau != 0.f) ? vbuf[r] : 0.f;
-            float *row = A + (size_t)r * ld;
-            float acc = 0.f;
-            float regs[8];
-            if (in_regs) {
+        // ---- rows r > i: left update (v_r, w), then right update (u, tau2); each
+        // wavefront keeps four rows in flight
+        if (nc <= 8 * 64) {
+            for (int r = i + 1 + 4 * wv; r < M; r += 4 * (BT / 64)) {
+                float x[4][8], acc[4];
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const int j = i + 1 + lane + 64 * c;
-                    regs[c] = 0.f;
-                    if (j < N) {
-                        float x = row[j] - vr * wbuf[j];
-                        regs[c] = x;
-                        acc += x * ubuf[j];
+                for (int q = 0; q < 4; ++q) {
+                    const int rq = r + q;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const int j = i + 1 + lane + 64 * c;
+                        x[q][c] = (rq < M && j < N) ? A[(size_t)rq * ld + j] : 0.f;
                     }
                 }
-            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rq = r + q;
+                    const float vr = (tau != 0.f && rq < M) ? vbuf[rq] : 0.f;
+                    acc[q] = 0.f;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const int j = i + 1 + lane + 64 * c;
+                        if (j < N) {
+                            x[q][c] -= vr * wbuf[j];
+                            acc[q] += x[q][c] * ubuf[j];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = wave_sum(acc[q]) * tau2;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rq = r + q;
+                    if (rq < M) {
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) {
+                            const int j = i + 1 + lane + 64 * c;
+                            if (j < N) A[(size_t)rq * ld + j] = x[q][c] - acc[q] * ubuf[j];
+                        }
+                    }
+                }
+            }
+        } else {
+            for (int r = i + 1 + wv; r < M; r += BT / 64) {
+                const float vr = (tau != 0.f) ? vbuf[r] : 0.f;
+                float *row = A + (size_t)r * ld;
+                float acc = 0.f;
                 for (int j = i + 1 + lane; j < N; j += 64) {
                     float x = row[j] - vr * wbuf[j];
                     row[j] = x;
                     acc += x * ubuf[j];
                 }
-            }
-            acc = wave_sum(acc) * tau2;
-            if (in_regs) {
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const int j = i + 1 + lane + 64 * c;
-                    if (j < N) row[j] = regs[c] - acc * ubuf[j];
-                }
-            } else {
+                acc = wave_sum(acc) * tau2;
                 for (int j = i + 1 + lane; j < N; j += 64) row[j] = row[j] - acc * ubuf[j];
             }
         }
         __syncthreads();
+    }
+    if (LDSRES) {
+        __syncthreads();
+        for (int r = wv; r < M; r += BT / 64)
+            for (int c = lane; c < N; c += 64) P.a[(size_t)r * ldg + c] = A[(size_t)r * ld + c];
     }
 }
 
@@ -384,12 +444,27 @@ __global__ __launch_bounds__(BT) void transpose_vt_kernel(ProbList pl, const int
 // launchers
 // ---------------------------------------------------------------------------
 size_t bidiag_wg_lds_bytes(int M, int N) { return sizeof(float) * ((size_t)M + 2 * (size_t)N + BT); }
-
-hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s)
+size_t bidiag_wg_lds_resident_bytes(int M, int N)
 {
-    static size_t maxdyn = allow_dynamic_lds((const void *)bidiag_wg_kernel);
-    if (lds > maxdyn) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(bidiag_wg_kernel, dim3(count), dim3(BT), lds, s, pl, order);
+    return sizeof(float) * ((size_t)M * (size_t)(N | 1)) + bidiag_wg_lds_bytes(M, N);
+}
+size_t bidiag_wg_lds_limit()
+{
+    static size_t maxdyn = allow_dynamic_lds((const void *)bidiag_wg_kernel<true>);
+    return maxdyn;
+}
+
+hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s, bool resident)
+{
+    static size_t maxg = allow_dynamic_lds((const void *)bidiag_wg_kernel<false>);
+    if (count <= 0) return hipSuccess;
+    if (resident) {
+        if (lds > bidiag_wg_lds_limit()) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(bidiag_wg_kernel<true>, dim3(count), dim3(BT), lds, s, pl, order);
+    } else {
+        if (lds > maxg) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(bidiag_wg_kernel<false>, dim3(count), dim3(BT), lds, s, pl, order);
+    }
     return hipGetLastError();
 }
 

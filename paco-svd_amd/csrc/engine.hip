@@ -147,9 +147,11 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     if (use_grid_bidiag(M, N)) {
         err = launch_bidiag_grid(pl.single, ws + L.grid, s);
     } else {
-        const size_t lds = bidiag_wg_lds_bytes(M, N);
-        if (lds > 160 * 1024) return PACO_EINVAL;
-        err = launch_bidiag_wg(pl, nullptr, 1, lds, s);
+        const size_t lres = bidiag_wg_lds_resident_bytes(M, N);
+        if (lres <= bidiag_wg_lds_limit())
+            err = launch_bidiag_wg(pl, nullptr, 1, lres, s, true);
+        else
+            err = launch_bidiag_wg(pl, nullptr, 1, bidiag_wg_lds_bytes(M, N), s, false);
     }
     if (err != hipSuccess) return (int)err;
     const int nt = cdiv(N, 32);
@@ -270,7 +272,9 @@ struct BatchPlan {
     size_t scratch_bytes = 0;
     int count = 0;
     // offsets into blob (bytes)
-    size_t o_probs = 0, o_src = 0, o_order = 0, o_info = 0;
+    size_t o_probs = 0, o_src = 0, o_order = 0, o_order_res = 0, o_info = 0;
+    int n_glb = 0, n_res = 0;
+    size_t lds_res = 0;
     size_t o_rowin = 0, o_idq = 0, o_idp = 0, o_bq = 0, o_bp = 0, o_sq = 0, o_sp = 0, o_tp = 0;
     size_t o_leaf = 0, o_tu = 0, o_tv = 0, o_ru = 0, o_rv = 0;
     std::vector<size_t> o_task;
@@ -306,6 +310,15 @@ static std::vector<int> prefix(int count, F f, int &total)
     return off;
 }
 
+// LDS the resident bidiagonalization may use per workgroup (PACO_LDS_RESIDENT_MAX caps it)
+static size_t bidiag_res_limit()
+{
+    size_t lim = bidiag_wg_lds_limit();
+    const char *env = getenv("PACO_LDS_RESIDENT_MAX");
+    if (env) lim = std::min(lim, (size_t)strtoull(env, nullptr, 10));
+    return lim;
+}
+
 static double svd_cost(int M, int N) { return 4.0 * M * M * N + 2.0 * M * N * N + 6.0 * (double)N * N * N; }
 
 static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::vector<size_t> &scr_off)
@@ -318,10 +331,9 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         scr_off[i] = so;
         so += scratch_layout(jobs[i].M, jobs[i].N).total;
         maxN = std::max(maxN, jobs[i].N);
-        bp.lds_bidiag = std::max(bp.lds_bidiag, bidiag_wg_lds_bytes(jobs[i].M, jobs[i].N));
     }
     bp.scratch_bytes = so;
-    bp.fits = bp.lds_bidiag <= 160 * 1024 && maxN <= PACO_MAX_N;
+    bp.fits = maxN <= PACO_MAX_N;
     auto &b = bp.blob;
     b.clear();
     b.resize(0);
@@ -331,12 +343,31 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     std::vector<const float *> src(count);
     for (int i = 0; i < count; ++i) src[i] = jobs[i].A;
     bp.o_src = blob_put(b, src);
+    // bidiagonalization variant per matrix: LDS-resident when the matrix fits, else
+    // streamed; each list in longest-first order
     std::vector<int> order(count);
     for (int i = 0; i < count; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         return svd_cost(jobs[x].M, jobs[x].N) > svd_cost(jobs[y].M, jobs[y].N);
     });
-    bp.o_order = blob_put(b, order);
+    const size_t res_limit = bidiag_res_limit();
+    std::vector<int> og, orr;
+    bp.lds_bidiag = 0;
+    bp.lds_res = 0;
+    for (int i : order) {
+        const size_t lr = bidiag_wg_lds_resident_bytes(jobs[i].M, jobs[i].N);
+        if (lr <= res_limit) {
+            orr.push_back(i);
+            bp.lds_res = std::max(bp.lds_res, lr);
+        } else {
+            og.push_back(i);
+            bp.lds_bidiag = std::max(bp.lds_bidiag, bidiag_wg_lds_bytes(jobs[i].M, jobs[i].N));
+        }
+    }
+    bp.n_glb = (int)og.size();
+    bp.n_res = (int)orr.size();
+    bp.o_order = blob_put(b, og);
+    bp.o_order_res = blob_put(b, orr);
     bp.o_rowin = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nrowin));
     bp.o_idq = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nidq));
     bp.o_idp = blob_put(b, prefix(count, [&](int i) { return jobs[i].N; }, bp.nidp));
@@ -384,6 +415,8 @@ struct BatchRes {
     void *pinned = nullptr;
     size_t pinned_size = 0;
     hipEvent_t ev = nullptr;
+    hipStream_t side = nullptr;      // forked stream for the concurrent bidiagonalization
+    hipEvent_t evf = nullptr, evj = nullptr;
 };
 
 static hipError_t ensure(BatchRes &r, size_t dev_bytes, size_t pin_bytes)
@@ -404,7 +437,11 @@ static hipError_t ensure(BatchRes &r, size_t dev_bytes, size_t pin_bytes)
         if (e != hipSuccess) return e;
         r.pinned_size = pin_bytes;
     }
-    if (!r.ev) return hipEventCreateWithFlags(&r.ev, hipEventDisableTiming);
+    hipError_t e = hipSuccess;
+    if (!r.ev && (e = hipEventCreateWithFlags(&r.ev, hipEventDisableTiming)) != hipSuccess) return e;
+    if (!r.side && (e = hipStreamCreateWithFlags(&r.side, hipStreamNonBlocking)) != hipSuccess) return e;
+    if (!r.evf && (e = hipEventCreateWithFlags(&r.evf, hipEventDisableTiming)) != hipSuccess) return e;
+    if (!r.evj && (e = hipEventCreateWithFlags(&r.evj, hipEventDisableTiming)) != hipSuccess) return e;
     return hipSuccess;
 }
 
@@ -490,7 +527,16 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     hipLaunchKernelGGL(copy_in_kernel, dim3(bp.nrowin), dim3(256), 0, s, pl,
                        reinterpret_cast<const float *const *>(dev + bp.o_src), I(bp.o_rowin));
     tmark(s, 1, tm);
-    if ((e = launch_bidiag_wg(pl, I(bp.o_order), count, bp.lds_bidiag, s)) != hipSuccess) return e;
+    // the two bidiagonalization variants run concurrently: resident on a forked stream
+    if (bp.n_res > 0) {
+        if ((e = hipEventRecord(res.evf, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(res.side, res.evf, 0)) != hipSuccess) return e;
+        if ((e = launch_bidiag_wg(pl, I(bp.o_order_res), bp.n_res, bp.lds_res, res.side, true)) != hipSuccess)
+            return e;
+        if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
+    }
+    if ((e = launch_bidiag_wg(pl, I(bp.o_order), bp.n_glb, bp.lds_bidiag, s, false)) != hipSuccess) return e;
+    if (bp.n_res > 0 && (e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
     tmark(s, 2, tm);
     if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
                        I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)

@@ -232,3 +232,25 @@ def test_batched_graph_replays(torch_dev):
         for i in range(len(As)):
             for x, y in zip(outs[i], eager[i]):
                 assert torch.equal(x, y), (rep, i)
+
+
+def test_bidiag_variants_bitwise_equal(torch_dev):
+    """The LDS-resident and the streamed one-workgroup bidiagonalization run the same
+    arithmetic in the same order: their outputs must be identical."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(21)
+    mats = mixed_batch(rng, 16, mmax=180)
+    say("variants", [m.shape for m in mats])
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    res = pacosvd.svd_batched(As)
+    torch.cuda.synchronize()
+    os.environ["PACO_LDS_RESIDENT_MAX"] = "0"
+    try:
+        glb = pacosvd.svd_batched(As)
+        torch.cuda.synchronize()
+    finally:
+        del os.environ["PACO_LDS_RESIDENT_MAX"]
+    for i in range(len(As)):
+        for x, y in zip(res[i], glb[i]):
+            assert torch.equal(x, y), i
