@@ -18,19 +18,6 @@ namespace paco {
 
 #define BT 256
 
-__device__ inline float block_sum(float v, float *red)
-{
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    v = wave_sum(v);
-    __syncthreads();
-    if (lane == 0) red[wv] = v;
-    __syncthreads();
-    float r = 0.f;
-#pragma unroll
-    for (int w = 0; w < BT / 64; ++w) r += red[w];
-    return r;
-}
-
 // Reference sign convention (upper_bidiagonalization.cpp:182-218): beta = +mu unless
 // sigma == 0 (then tau = 0, beta = x1 signed).  inv = 1 / v1.
 __device__ inline void house_params(float x1, float sigma, float *beta, float *tau, float *inv)
@@ -50,82 +37,106 @@ __device__ inline void house_params(float x1, float sigma, float *beta, float *t
 }
 
 // ---------------------------------------------------------------------------
-// One workgroup per matrix.  LDS: [matrix M x ldl (LDS-resident variant only)]
-// vbuf[M] (current column reflector), wbuf[N] (tau_q v^T A), ubuf[N] (current row
-// reflector), red[BT].  The LDS-resident variant stages the whole matrix in LDS
-// (odd leading dimension: column accesses hit distinct banks) and writes it back at
-// the end; the global variant streams from L2/HBM with 8-deep unrolled loads and four
-// rows per wavefront in flight.
+// One workgroup of NT threads per matrix.  The bottom rows [split, M) of the matrix
+// are staged in LDS (as many as the launch's LDS budget allows; all rows when the
+// matrix fits) and the rest is streamed from L2/HBM; every access goes through the
+// row's pointer, and every reduction runs in the same order whatever the split, so
+// the result does not depend on how much of the matrix is in LDS.
+// LDS: [rows split..M-1, leading dimension N|1 (odd: column accesses hit distinct
+// banks)] vbuf[M] (column reflector) wbuf[N] (tau_q v^T A) ubuf[N] (row reflector)
+// red[NT].
 // ---------------------------------------------------------------------------
-template <int STEP>
-__device__ inline float col_dot(const float *A, int ld, int j, int r0, int r1, const float *v)
+template <int NT>
+__device__ inline float block_sum_t(float v, float *red)
 {
-    float acc = 0.f;
-    int r = r0;
-    for (; r + 8 * STEP <= r1; r += 8 * STEP) {
-        float a[8];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    float r = 0.f;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) a[k] = A[(size_t)(r + k * STEP) * ld + j];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) acc += v[r + k * STEP] * a[k];
-    }
-    for (; r < r1; r += STEP) acc += v[r] * A[(size_t)r * ld + j];
-    return acc;
+    for (int w = 0; w < NT / 64; ++w) r += red[w];
+    return r;
 }
 
-__device__ inline float col_dot_step(const float *A, int ld, int j, int r0, int r1, int step, const float *v)
+struct RowStore {
+    float *G; // global matrix
+    float *L; // LDS rows split..M-1
+    int ldg, ldl, split;
+    __device__ inline float *row(int r) const
+    {
+        return r >= split ? L + (size_t)(r - split) * ldl : G + (size_t)r * ldg;
+    }
+};
+
+// sum_{r = r0, r0+step, ... < r1} v[r] * A[r][j], eight loads in flight
+__device__ inline float col_dot(const RowStore &S, int j, int r0, int r1, int step, const float *v)
 {
     float acc = 0.f;
     int r = r0;
     for (; r + 7 * step < r1; r += 8 * step) {
         float a[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) a[k] = A[(size_t)(r + k * step) * ld + j];
+        for (int k = 0; k < 8; ++k) a[k] = S.row(r + k * step)[j];
 #pragma unroll
         for (int k = 0; k < 8; ++k) acc += v[r + k * step] * a[k];
     }
-    for (; r < r1; r += step) acc += v[r] * A[(size_t)r * ld + j];
+    for (; r < r1; r += step) acc += v[r] * S.row(r)[j];
     return acc;
 }
 
-template <bool LDSRES>
-__global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *order)
+// rows of A that fit in lds_bytes of dynamic LDS next to the vectors
+__host__ __device__ inline int bidiag_lds_rows(int M, int N, int nt, size_t lds_bytes)
+{
+    const size_t vec = sizeof(float) * ((size_t)M + 2 * (size_t)N + nt);
+    if (lds_bytes <= vec) return 0;
+    const size_t r = (lds_bytes - vec) / (sizeof(float) * (size_t)(N | 1));
+    return r > (size_t)M ? M : (int)r;
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *order, unsigned lds_bytes)
 {
     const int pid = order ? order[blockIdx.x] : blockIdx.x;
     const ProbDesc P = pl.get(pid);
     const int M = P.M, N = P.N, ldg = P.ld ? P.ld : P.N;
     extern __shared__ float sm[];
-    const int ld = LDSRES ? (N | 1) : ldg;
-    float *__restrict__ A = LDSRES ? sm : P.a;
-    float *vbuf = sm + (LDSRES ? (size_t)M * ld : 0), *wbuf = vbuf + M, *ubuf = wbuf + N, *red = ubuf + N;
+    const int nl = bidiag_lds_rows(M, N, NT, lds_bytes);
+    RowStore S;
+    S.G = P.a;
+    S.L = sm;
+    S.ldg = ldg;
+    S.ldl = N | 1;
+    S.split = M - nl;
+    float *vbuf = sm + (size_t)nl * S.ldl, *wbuf = vbuf + M, *ubuf = wbuf + N, *red = ubuf + N;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (LDSRES) {
-        for (int r = wv; r < M; r += BT / 64)
-            for (int c = lane; c < N; c += 64) A[(size_t)r * ld + c] = P.a[(size_t)r * ldg + c];
-        __syncthreads();
-    }
+    for (int r = S.split + wv; r < M; r += NT / 64)
+        for (int c = lane; c < N; c += 64) S.L[(size_t)(r - S.split) * S.ldl + c] = P.a[(size_t)r * ldg + c];
+    __syncthreads();
 
     for (int i = 0; i < N; ++i) {
         // ---- column reflector on A[i:, i]
         float part = 0.f;
-        for (int r = i + 1 + tid; r < M; r += BT) {
-            float x = A[(size_t)r * ld + i];
+        for (int r = i + 1 + tid; r < M; r += NT) {
+            float x = S.row(r)[i];
             vbuf[r] = x;
             part += x * x;
         }
-        const float sigma = block_sum(part, red);
-        const float x1 = A[(size_t)i * ld + i];
+        const float sigma = block_sum_t<NT>(part, red);
+        float *rowi = S.row(i);
+        const float x1 = rowi[i];
         float tau, beta, inv;
         house_params(x1, sigma, &beta, &tau, &inv);
         if (tau != 0.f) {
-            for (int r = i + 1 + tid; r < M; r += BT) {
+            for (int r = i + 1 + tid; r < M; r += NT) {
                 float v = vbuf[r] * inv;
                 vbuf[r] = v;
-                A[(size_t)r * ld + i] = v;
+                S.row(r)[i] = v;
             }
         }
         if (tid == 0) {
-            A[(size_t)i * ld + i] = beta;
+            rowi[i] = beta;
             P.d[i] = beta;
             P.tauq[i] = tau;
             vbuf[i] = 1.f;
@@ -136,15 +147,15 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
         // ---- w_j = tau * sum_{r>=i} v_r A[r][j], j in (i, N)
         const int nc = N - i - 1;
         if (tau == 0.f) {
-            for (int j = i + 1 + tid; j < N; j += BT) wbuf[j] = 0.f;
-        } else if (nc >= BT) {
-            for (int j = i + 1 + tid; j < N; j += BT) wbuf[j] = tau * col_dot<1>(A, ld, j, i, M, vbuf);
+            for (int j = i + 1 + tid; j < N; j += NT) wbuf[j] = 0.f;
+        } else if (nc >= NT) {
+            for (int j = i + 1 + tid; j < N; j += NT) wbuf[j] = tau * col_dot(S, j, i, M, 1, vbuf);
         } else {
             int tc = 1;
             while (tc < nc) tc <<= 1;
-            const int rg = BT / tc, g = tid / tc, c = tid % tc;
+            const int rg = NT / tc, g = tid / tc, c = tid % tc;
             float acc = 0.f;
-            if (c < nc) acc = col_dot_step(A, ld, i + 1 + c, i + g, M, rg, vbuf);
+            if (c < nc) acc = col_dot(S, i + 1 + c, i + g, M, rg, vbuf);
             red[tid] = acc;
             __syncthreads();
             if (g == 0 && c < nc) {
@@ -157,24 +168,24 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
 
         // ---- row i: apply left reflector, then row reflector on A[i, i+1:]
         part = 0.f;
-        for (int j = i + 1 + tid; j < N; j += BT) {
-            float x = A[(size_t)i * ld + j] - wbuf[j];
+        for (int j = i + 1 + tid; j < N; j += NT) {
+            float x = rowi[j] - wbuf[j];
             ubuf[j] = x;
             if (j > i + 1) part += x * x;
         }
-        const float sigma2 = block_sum(part, red);
+        const float sigma2 = block_sum_t<NT>(part, red);
         const float y1 = ubuf[i + 1];
         float tau2, beta2, inv2;
         house_params(y1, sigma2, &beta2, &tau2, &inv2);
         __syncthreads();
-        for (int j = i + 1 + tid; j < N; j += BT) {
+        for (int j = i + 1 + tid; j < N; j += NT) {
             if (j == i + 1) {
-                A[(size_t)i * ld + j] = beta2;
+                rowi[j] = beta2;
                 ubuf[j] = 1.f;
             } else {
                 float uval = (tau2 != 0.f) ? ubuf[j] * inv2 : ubuf[j];
                 ubuf[j] = uval;
-                A[(size_t)i * ld + j] = uval;
+                rowi[j] = uval;
             }
         }
         if (tid == 0) {
@@ -186,21 +197,22 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
         // ---- rows r > i: left update (v_r, w), then right update (u, tau2); each
         // wavefront keeps four rows in flight
         if (nc <= 8 * 64) {
-            for (int r = i + 1 + 4 * wv; r < M; r += 4 * (BT / 64)) {
+            for (int r = i + 1 + 4 * wv; r < M; r += 4 * (NT / 64)) {
                 float x[4][8], acc[4];
+                float *rp[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) rp[q] = (r + q < M) ? S.row(r + q) : nullptr;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int rq = r + q;
 #pragma unroll
                     for (int c = 0; c < 8; ++c) {
                         const int j = i + 1 + lane + 64 * c;
-                        x[q][c] = (rq < M && j < N) ? A[(size_t)rq * ld + j] : 0.f;
+                        x[q][c] = (rp[q] && j < N) ? rp[q][j] : 0.f;
                     }
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int rq = r + q;
-                    const float vr = (tau != 0.f && rq < M) ? vbuf[rq] : 0.f;
+                    const float vr = (tau != 0.f && rp[q]) ? vbuf[r + q] : 0.f;
                     acc[q] = 0.f;
 #pragma unroll
                     for (int c = 0; c < 8; ++c) {
@@ -215,20 +227,19 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
                 for (int q = 0; q < 4; ++q) acc[q] = wave_sum(acc[q]) * tau2;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int rq = r + q;
-                    if (rq < M) {
+                    if (rp[q]) {
 #pragma unroll
                         for (int c = 0; c < 8; ++c) {
                             const int j = i + 1 + lane + 64 * c;
-                            if (j < N) A[(size_t)rq * ld + j] = x[q][c] - acc[q] * ubuf[j];
+                            if (j < N) rp[q][j] = x[q][c] - acc[q] * ubuf[j];
                         }
                     }
                 }
             }
         } else {
-            for (int r = i + 1 + wv; r < M; r += BT / 64) {
+            for (int r = i + 1 + wv; r < M; r += NT / 64) {
                 const float vr = (tau != 0.f) ? vbuf[r] : 0.f;
-                float *row = A + (size_t)r * ld;
+                float *row = S.row(r);
                 float acc = 0.f;
                 for (int j = i + 1 + lane; j < N; j += 64) {
                     float x = row[j] - vr * wbuf[j];
@@ -241,11 +252,9 @@ __global__ __launch_bounds__(BT) void bidiag_wg_kernel(ProbList pl, const int *o
         }
         __syncthreads();
     }
-    if (LDSRES) {
-        __syncthreads();
-        for (int r = wv; r < M; r += BT / 64)
-            for (int c = lane; c < N; c += 64) P.a[(size_t)r * ldg + c] = A[(size_t)r * ld + c];
-    }
+    __syncthreads();
+    for (int r = S.split + wv; r < M; r += NT / 64)
+        for (int c = lane; c < N; c += 64) P.a[(size_t)r * ldg + c] = S.L[(size_t)(r - S.split) * S.ldl + c];
 }
 
 // ---------------------------------------------------------------------------
@@ -443,28 +452,34 @@ __global__ __launch_bounds__(BT) void transpose_vt_kernel(ProbList pl, const int
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-size_t bidiag_wg_lds_bytes(int M, int N) { return sizeof(float) * ((size_t)M + 2 * (size_t)N + BT); }
-size_t bidiag_wg_lds_resident_bytes(int M, int N)
+size_t bidiag_wg_lds_bytes(int M, int N, int nt) { return sizeof(float) * ((size_t)M + 2 * (size_t)N + nt); }
+size_t bidiag_wg_lds_resident_bytes(int M, int N, int nt)
 {
-    return sizeof(float) * ((size_t)M * (size_t)(N | 1)) + bidiag_wg_lds_bytes(M, N);
+    return sizeof(float) * ((size_t)M * (size_t)(N | 1)) + bidiag_wg_lds_bytes(M, N, nt);
 }
 size_t bidiag_wg_lds_limit()
 {
-    static size_t maxdyn = allow_dynamic_lds((const void *)bidiag_wg_kernel<true>);
-    return maxdyn;
+    static size_t lim = 0;
+    if (!lim) {
+        const size_t a = allow_dynamic_lds((const void *)bidiag_wg_kernel<256>);
+        const size_t b = allow_dynamic_lds((const void *)bidiag_wg_kernel<512>);
+        const size_t c = allow_dynamic_lds((const void *)bidiag_wg_kernel<1024>);
+        lim = a < b ? (a < c ? a : c) : (b < c ? b : c);
+    }
+    return lim;
 }
 
-hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s, bool resident)
+hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s, int nt)
 {
-    static size_t maxg = allow_dynamic_lds((const void *)bidiag_wg_kernel<false>);
     if (count <= 0) return hipSuccess;
-    if (resident) {
-        if (lds > bidiag_wg_lds_limit()) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(bidiag_wg_kernel<true>, dim3(count), dim3(BT), lds, s, pl, order);
-    } else {
-        if (lds > maxg) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(bidiag_wg_kernel<false>, dim3(count), dim3(BT), lds, s, pl, order);
-    }
+    if (lds > bidiag_wg_lds_limit()) return hipErrorInvalidValue;
+    const unsigned lb = (unsigned)lds;
+    if (nt == 1024)
+        hipLaunchKernelGGL(bidiag_wg_kernel<1024>, dim3(count), dim3(1024), lds, s, pl, order, lb);
+    else if (nt == 512)
+        hipLaunchKernelGGL(bidiag_wg_kernel<512>, dim3(count), dim3(512), lds, s, pl, order, lb);
+    else
+        hipLaunchKernelGGL(bidiag_wg_kernel<256>, dim3(count), dim3(256), lds, s, pl, order, lb);
     return hipGetLastError();
 }
 

@@ -407,7 +407,7 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s)
     pl.single.e = P.e + p;
     pl.single.tauq = P.tauq + p;
     pl.single.taup = P.taup + p;
-    return launch_bidiag_wg(pl, nullptr, 1, bidiag_wg_lds_bytes(M - p, N - p), s);
+    return launch_bidiag_wg(pl, nullptr, 1, bidiag_wg_lds_limit(), s, 1024);
 }
 
 } // namespace paco

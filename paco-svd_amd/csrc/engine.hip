@@ -72,6 +72,8 @@ static WsLayout ws_layout(int M, int N)
     return L;
 }
 
+int tuned_bidiag_threads(int M, int N);
+
 static std::mutex g_mu;
 static void *g_ws = nullptr;
 static size_t g_ws_size = 0;
@@ -118,6 +120,35 @@ static ProbDesc bidiag_desc(int M, int N, float *a, float *u, float *vt, float *
     return P;
 }
 
+// Launch shape of the one-workgroup bidiagonalization for an M x N matrix: matrices
+// that fit LDS whole run with 256 threads and exactly their LDS; larger ones with the
+// larger workgroup and the whole LDS budget (bottom rows cached).  The tuned table
+// (autotune.cpp) overrides the defaults per size bucket.
+struct BidiagChoice {
+    int nt;
+    size_t lds;
+    bool resident;
+};
+static BidiagChoice bidiag_choice(int M, int N)
+{
+    BidiagChoice c;
+    size_t lim = bidiag_wg_lds_limit();
+    const char *env = getenv("PACO_LDS_RESIDENT_MAX");
+    const size_t res_lim = env ? std::min(lim, (size_t)strtoull(env, nullptr, 10)) : lim;
+    const size_t lr = bidiag_wg_lds_resident_bytes(M, N, 256);
+    if (lr <= res_lim) {
+        c.nt = 256;
+        c.lds = lr;
+        c.resident = true;
+        return c;
+    }
+    c.nt = tuned_bidiag_threads(M, N);
+    c.lds = env ? bidiag_wg_lds_bytes(M, N, c.nt) : lim;
+    if (c.lds < bidiag_wg_lds_bytes(M, N, c.nt)) c.lds = bidiag_wg_lds_bytes(M, N, c.nt);
+    c.resident = false;
+    return c;
+}
+
 // Q and P^T from the reflectors (GetUVt, upper_bidiagonalization.cpp:774-893), blocked WY.
 static hipError_t form_q_pt(ProbList pl, const int *idq, int nidq, const int *idp, int nidp, const int *bq, int nbq,
                             const int *bp, int nbp, const int *sq, int nsq, const int *sp, int nsp, const int *tpair,
@@ -147,11 +178,8 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     if (use_grid_bidiag(M, N)) {
         err = launch_bidiag_grid(pl.single, ws + L.grid, s);
     } else {
-        const size_t lres = bidiag_wg_lds_resident_bytes(M, N);
-        if (lres <= bidiag_wg_lds_limit())
-            err = launch_bidiag_wg(pl, nullptr, 1, lres, s, true);
-        else
-            err = launch_bidiag_wg(pl, nullptr, 1, bidiag_wg_lds_bytes(M, N), s, false);
+        const BidiagChoice c = bidiag_choice(M, N);
+        err = launch_bidiag_wg(pl, nullptr, 1, c.lds, s, c.nt);
     }
     if (err != hipSuccess) return (int)err;
     const int nt = cdiv(N, 32);
@@ -273,7 +301,7 @@ struct BatchPlan {
     int count = 0;
     // offsets into blob (bytes)
     size_t o_probs = 0, o_src = 0, o_order = 0, o_order_res = 0, o_info = 0;
-    int n_glb = 0, n_res = 0;
+    int n_glb = 0, n_res = 0, nt_glb = 256;
     size_t lds_res = 0;
     size_t o_rowin = 0, o_idq = 0, o_idp = 0, o_bq = 0, o_bp = 0, o_sq = 0, o_sp = 0, o_tp = 0;
     size_t o_leaf = 0, o_tu = 0, o_tv = 0, o_ru = 0, o_rv = 0;
@@ -310,15 +338,6 @@ static std::vector<int> prefix(int count, F f, int &total)
     return off;
 }
 
-// LDS the resident bidiagonalization may use per workgroup (PACO_LDS_RESIDENT_MAX caps it)
-static size_t bidiag_res_limit()
-{
-    size_t lim = bidiag_wg_lds_limit();
-    const char *env = getenv("PACO_LDS_RESIDENT_MAX");
-    if (env) lim = std::min(lim, (size_t)strtoull(env, nullptr, 10));
-    return lim;
-}
-
 static double svd_cost(int M, int N) { return 4.0 * M * M * N + 2.0 * M * N * N + 6.0 * (double)N * N * N; }
 
 static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::vector<size_t> &scr_off)
@@ -350,18 +369,19 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         return svd_cost(jobs[x].M, jobs[x].N) > svd_cost(jobs[y].M, jobs[y].N);
     });
-    const size_t res_limit = bidiag_res_limit();
     std::vector<int> og, orr;
     bp.lds_bidiag = 0;
     bp.lds_res = 0;
+    bp.nt_glb = 256;
     for (int i : order) {
-        const size_t lr = bidiag_wg_lds_resident_bytes(jobs[i].M, jobs[i].N);
-        if (lr <= res_limit) {
+        const BidiagChoice c = bidiag_choice(jobs[i].M, jobs[i].N);
+        if (c.resident) {
             orr.push_back(i);
-            bp.lds_res = std::max(bp.lds_res, lr);
+            bp.lds_res = std::max(bp.lds_res, c.lds);
         } else {
             og.push_back(i);
-            bp.lds_bidiag = std::max(bp.lds_bidiag, bidiag_wg_lds_bytes(jobs[i].M, jobs[i].N));
+            bp.lds_bidiag = std::max(bp.lds_bidiag, c.lds);
+            bp.nt_glb = std::max(bp.nt_glb, c.nt);
         }
     }
     bp.n_glb = (int)og.size();
@@ -531,11 +551,11 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     if (bp.n_res > 0) {
         if ((e = hipEventRecord(res.evf, s)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(res.side, res.evf, 0)) != hipSuccess) return e;
-        if ((e = launch_bidiag_wg(pl, I(bp.o_order_res), bp.n_res, bp.lds_res, res.side, true)) != hipSuccess)
+        if ((e = launch_bidiag_wg(pl, I(bp.o_order_res), bp.n_res, bp.lds_res, res.side, 256)) != hipSuccess)
             return e;
         if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
     }
-    if ((e = launch_bidiag_wg(pl, I(bp.o_order), bp.n_glb, bp.lds_bidiag, s, false)) != hipSuccess) return e;
+    if ((e = launch_bidiag_wg(pl, I(bp.o_order), bp.n_glb, bp.lds_bidiag, s, bp.nt_glb)) != hipSuccess) return e;
     if (bp.n_res > 0 && (e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
     tmark(s, 2, tm);
     if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,

@@ -50,11 +50,12 @@ inline size_t allow_dynamic_lds(const void *func)
 }
 
 // bidiag.hip
-size_t bidiag_wg_lds_bytes(int M, int N);          // global-streaming variant
-size_t bidiag_wg_lds_resident_bytes(int M, int N); // matrix staged in LDS
-size_t bidiag_wg_lds_limit();                      // dynamic LDS the resident variant may use
-hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s,
-                            bool resident = false);
+// one-workgroup bidiagonalization with nt threads; lds bytes of dynamic LDS hold the
+// vectors and as many bottom rows of each matrix as fit
+size_t bidiag_wg_lds_bytes(int M, int N, int nt = 256);          // vectors only
+size_t bidiag_wg_lds_resident_bytes(int M, int N, int nt = 256); // whole matrix in LDS
+size_t bidiag_wg_lds_limit();
+hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s, int nt = 256);
 hipError_t launch_tfactor(ProbList pl, const int *blk_off, int nblocks, int side, hipStream_t s);
 hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int side, hipStream_t s);
 hipError_t launch_identity(ProbList pl, const int *row_off, int nrows, int side, hipStream_t s);

@@ -246,11 +246,34 @@ def test_bidiag_variants_bitwise_equal(torch_dev):
     res = pacosvd.svd_batched(As)
     torch.cuda.synchronize()
     os.environ["PACO_LDS_RESIDENT_MAX"] = "0"
+    os.environ["PACO_BIDIAG_THREADS"] = "256"
     try:
         glb = pacosvd.svd_batched(As)
         torch.cuda.synchronize()
     finally:
         del os.environ["PACO_LDS_RESIDENT_MAX"]
+        del os.environ["PACO_BIDIAG_THREADS"]
     for i in range(len(As)):
         for x, y in zip(res[i], glb[i]):
             assert torch.equal(x, y), i
+
+
+@pytest.mark.parametrize("threads", ["256", "512", "1024"])
+def test_bidiag_workgroup_sizes(torch_dev, threads):
+    """Matrices too large for LDS run with 256, 512 or 1024 threads (rows partly cached)."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(int(threads))
+    mats = [rng.standard_normal(sh).astype(np.float32) for sh in [(300, 250), (512, 512), (480, 130), (260, 259)]]
+    say("threads", threads, [m.shape for m in mats])
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    os.environ["PACO_BIDIAG_THREADS"] = threads
+    try:
+        outs = pacosvd.svd_batched(As)
+        torch.cuda.synchronize()
+    finally:
+        del os.environ["PACO_BIDIAG_THREADS"]
+    for i, A in enumerate(mats):
+        m = measures(A, *(np_of(x) for x in outs[i]))
+        assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL, (i, m)
+        assert m["resid_ratio"] <= RESID_TOL
