@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--plain-budget", type=float, default=20.0, help="seconds for the plain-path measurement")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-batched", action="store_true",
+                    help="only run the batched step (warmup + steps), for counter collection")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: independent SVDs shard by matrix across GPUs with no "
@@ -154,6 +156,10 @@ def main():
     ph = {k: float(np.mean([p[k] for p in phase_ms])) for k in phase_ms[0]}
     print("[bench] batched %.1f ms/step, phases %s" % (ms_step, ph), flush=True)
 
+    if args.profile_batched:
+        print(json.dumps({"profile_batched_ms_per_step": ms_step, "phase_ms": ph}), flush=True)
+        return
+
     # ---- batched path replayed from a hipGraph
     pacosvd.svd_batched(As, outs=outs, graph=True)
     torch.cuda.synchronize()
@@ -179,8 +185,8 @@ def main():
         pacosvd.svd_plain(chunk)
         torch.cuda.synchronize()
         done += len(chunk)
-    dt_p = time.perf_counter() - t0
-    plain_svds_s = done / dt_p
+    dt_p = max(time.perf_counter() - t0, 1e-9)
+    plain_svds_s = max(done, 1) / dt_p
     del plain_outs
     print("[bench] plain %d matrices %.2f s" % (done, dt_p), flush=True)
 
