@@ -37,14 +37,16 @@ __device__ inline void house_params(float x1, float sigma, float *beta, float *t
 }
 
 // ---------------------------------------------------------------------------
-// One workgroup of NT threads per matrix.  The bottom rows [split, M) of the matrix
-// are staged in LDS (as many as the launch's LDS budget allows; all rows when the
-// matrix fits) and the rest is streamed from L2/HBM; every access goes through the
-// row's pointer, and every reduction runs in the same order whatever the split, so
-// the result does not depend on how much of the matrix is in LDS.
-// LDS: [rows split..M-1, leading dimension N|1 (odd: column accesses hit distinct
-// banks)] vbuf[M] (column reflector) wbuf[N] (tau_q v^T A) ubuf[N] (row reflector)
-// red[NT].
+// One workgroup of NT threads per matrix (N <= 512).  Each column costs one streaming
+// pass over the trailing rows: while a wavefront holds an updated row in registers it
+// also accumulates the next column's Householder data -- that column's entries c_r,
+// sum c_r^2, and sum_r c_r A'[r, :] -- so the next left GEMV needs no pass of its own
+// (its scaling by 1/v_1 is applied after the reduction):
+//   w_{i+1} = tau (A'[i+1, :] + (1/v_1) sum_{r > i+1} c_r A'[r, :]).
+// Rows [split, M) are kept in LDS (all rows when the matrix fits), the rest is
+// streamed; every reduction runs in an order that does not depend on the split.
+// LDS: [cached rows, leading dimension N|1] cbuf[M] wbuf[N] ubuf[N] rowbuf[N] pw[N]
+//      pwr[NW][N] red[NT] (+ scalars).
 // ---------------------------------------------------------------------------
 template <int NT>
 __device__ inline float block_sum_t(float v, float *red)
@@ -70,34 +72,116 @@ struct RowStore {
     }
 };
 
-// sum_{r = r0, r0+step, ... < r1} v[r] * A[r][j], eight loads in flight
-__device__ inline float col_dot(const RowStore &S, int j, int r0, int r1, int step, const float *v)
+constexpr int WG_MAXN = 512; // 8 chunks of 64 columns per lane
+
+__host__ __device__ inline size_t bidiag_vec_floats(int M, int N, int nt)
 {
-    float acc = 0.f;
-    int r = r0;
-    for (; r + 7 * step < r1; r += 8 * step) {
-        float a[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) a[k] = S.row(r + k * step)[j];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) acc += v[r + k * step] * a[k];
-    }
-    for (; r < r1; r += step) acc += v[r] * S.row(r)[j];
-    return acc;
+    return (size_t)M + 5 * (size_t)N + (size_t)(nt / 64) * N + nt + 8;
 }
 
 // rows of A that fit in lds_bytes of dynamic LDS next to the vectors
 __host__ __device__ inline int bidiag_lds_rows(int M, int N, int nt, size_t lds_bytes)
 {
-    const size_t vec = sizeof(float) * ((size_t)M + 2 * (size_t)N + nt);
+    const size_t vec = sizeof(float) * bidiag_vec_floats(M, N, nt);
     if (lds_bytes <= vec) return 0;
     const size_t r = (lds_bytes - vec) / (sizeof(float) * (size_t)(N | 1));
     return r > (size_t)M ? M : (int)r;
 }
 
+// Fused pass over rows r > i (i = -1: read-only initial pass).  Row update
+//   t = A[r, j] - v_r w_j, x_r = tau2 t.u, A'[r, j] = t_j - x_r u_j   (j > i)
+// then, for column k = i + 1: row k -> rowbuf; rows r > k: cbuf[r] = A'[r, k],
+// sumsq partial, pw partial (columns > k).  Wave partials of pw go to pwr[wave].
+template <int NT>
+__device__ inline float fused_pass(const RowStore &S, int M, int N, int i, float tau, float inv, float tau2,
+                                   float *cbuf, const float *wbuf, const float *ubuf, float *rowbuf, float *pwr)
+{
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, k = i + 1;
+    const bool upd = i >= 0;
+    float pwp[8], ss = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) pwp[c] = 0.f;
+    for (int r = k + 4 * wv; r < M; r += 4 * NW) {
+        float x[4][8], acc[4];
+        float *rp[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rp[q] = (r + q < M) ? S.row(r + q) : nullptr;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int j = k + lane + 64 * c;
+                x[q][c] = (rp[q] && j < N) ? rp[q][j] : 0.f;
+            }
+        if (upd) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float vr = (rp[q] && tau != 0.f) ? cbuf[r + q] * inv : 0.f;
+                acc[q] = 0.f;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int j = k + lane + 64 * c;
+                    if (j < N) {
+                        x[q][c] -= vr * wbuf[j];
+                        acc[q] += x[q][c] * ubuf[j];
+                    }
+                }
+                if (rp[q] && lane == 0) rp[q][i] = (tau != 0.f) ? vr : cbuf[r + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = wave_sum(acc[q]) * tau2;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (rp[q]) {
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const int j = k + lane + 64 * c;
+                        if (j < N) {
+                            x[q][c] -= acc[q] * ubuf[j];
+                            rp[q][j] = x[q][c];
+                        }
+                    }
+                }
+            }
+        }
+        // next column's data
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rq = r + q;
+            if (rq >= M) break;
+            const float cv = __shfl(x[q][0], 0, 64);
+            if (rq == k) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int j = k + lane + 64 * c;
+                    if (j < N) rowbuf[j] = x[q][c];
+                }
+            } else {
+                if (lane == 0) {
+                    cbuf[rq] = cv;
+                    ss += cv * cv;
+                }
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int j = k + lane + 64 * c;
+                    if (j > k && j < N) pwp[c] += cv * x[q][c];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int j = k + lane + 64 * c;
+        if (j < N) pwr[(size_t)wv * N + j] = pwp[c];
+    }
+    return ss; // lane 0 of each wave holds its partial
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *order, unsigned lds_bytes)
 {
+    constexpr int NW = NT / 64;
     const int pid = order ? order[blockIdx.x] : blockIdx.x;
     const ProbDesc P = pl.get(pid);
     const int M = P.M, N = P.N, ldg = P.ld ? P.ld : P.N;
@@ -109,83 +193,66 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
     S.ldg = ldg;
     S.ldl = N | 1;
     S.split = M - nl;
-    float *vbuf = sm + (size_t)nl * S.ldl, *wbuf = vbuf + M, *ubuf = wbuf + N, *red = ubuf + N;
+    float *cbuf = sm + (size_t)nl * S.ldl, *wbuf = cbuf + M, *ubuf = wbuf + N, *rowbuf = ubuf + N, *pw = rowbuf + N;
+    float *pwr = pw + N, *red = pwr + (size_t)NW * N, *scal = red + NT;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int r = S.split + wv; r < M; r += NT / 64)
+    for (int r = S.split + wv; r < M; r += NW)
         for (int c = lane; c < N; c += 64) S.L[(size_t)(r - S.split) * S.ldl + c] = P.a[(size_t)r * ldg + c];
     __syncthreads();
 
+    // initial read-only pass: column 0 data
+    float ss = fused_pass<NT>(S, M, N, -1, 0.f, 1.f, 0.f, cbuf, wbuf, ubuf, rowbuf, pwr);
     for (int i = 0; i < N; ++i) {
-        // ---- column reflector on A[i:, i]
-        float part = 0.f;
-        for (int r = i + 1 + tid; r < M; r += NT) {
-            float x = S.row(r)[i];
-            vbuf[r] = x;
-            part += x * x;
+        // ---- reduce the pass partials: sum c_r^2 and pw
+        if (lane == 0) red[wv] = ss;
+        __syncthreads();
+        for (int j = i + 1 + tid; j < N; j += NT) {
+            float a = 0.f;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) a += pwr[(size_t)w * N + j];
+            pw[j] = a;
         }
-        const float sigma = block_sum_t<NT>(part, red);
-        float *rowi = S.row(i);
-        const float x1 = rowi[i];
+        if (tid == 0) {
+            float sg = 0.f;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) sg += red[w];
+            scal[0] = sg;
+        }
+        __syncthreads();
+        // ---- column reflector of column i
         float tau, beta, inv;
-        house_params(x1, sigma, &beta, &tau, &inv);
-        if (tau != 0.f) {
-            for (int r = i + 1 + tid; r < M; r += NT) {
-                float v = vbuf[r] * inv;
-                vbuf[r] = v;
-                S.row(r)[i] = v;
-            }
-        }
+        house_params(rowbuf[i], scal[0], &beta, &tau, &inv);
+        float *rowi = S.row(i);
         if (tid == 0) {
             rowi[i] = beta;
             P.d[i] = beta;
             P.tauq[i] = tau;
-            vbuf[i] = 1.f;
         }
-        __syncthreads();
-        if (i == N - 1) break;
-
-        // ---- w_j = tau * sum_{r>=i} v_r A[r][j], j in (i, N)
-        const int nc = N - i - 1;
-        if (tau == 0.f) {
-            for (int j = i + 1 + tid; j < N; j += NT) wbuf[j] = 0.f;
-        } else if (nc >= NT) {
-            for (int j = i + 1 + tid; j < N; j += NT) wbuf[j] = tau * col_dot(S, j, i, M, 1, vbuf);
-        } else {
-            int tc = 1;
-            while (tc < nc) tc <<= 1;
-            const int rg = NT / tc, g = tid / tc, c = tid % tc;
-            float acc = 0.f;
-            if (c < nc) acc = col_dot(S, i + 1 + c, i + g, M, rg, vbuf);
-            red[tid] = acc;
-            __syncthreads();
-            if (g == 0 && c < nc) {
-                float s = 0.f;
-                for (int q = 0; q < rg; ++q) s += red[q * tc + c];
-                wbuf[i + 1 + c] = tau * s;
-            }
+        if (i == N - 1) {
+            for (int r = i + 1 + tid; r < M; r += NT) S.row(r)[i] = (tau != 0.f) ? cbuf[r] * inv : cbuf[r];
+            break;
         }
-        __syncthreads();
-
-        // ---- row i: apply left reflector, then row reflector on A[i, i+1:]
-        part = 0.f;
+        // ---- w and the updated row i; row reflector
+        float part = 0.f;
         for (int j = i + 1 + tid; j < N; j += NT) {
-            float x = rowi[j] - wbuf[j];
-            ubuf[j] = x;
-            if (j > i + 1) part += x * x;
+            const float w = (tau != 0.f) ? tau * (rowbuf[j] + inv * pw[j]) : 0.f;
+            wbuf[j] = w;
+            const float y = rowbuf[j] - w;
+            ubuf[j] = y;
+            if (j > i + 1) part += y * y;
         }
         const float sigma2 = block_sum_t<NT>(part, red);
-        const float y1 = ubuf[i + 1];
         float tau2, beta2, inv2;
-        house_params(y1, sigma2, &beta2, &tau2, &inv2);
+        house_params(ubuf[i + 1], sigma2, &beta2, &tau2, &inv2);
         __syncthreads();
         for (int j = i + 1 + tid; j < N; j += NT) {
             if (j == i + 1) {
                 rowi[j] = beta2;
                 ubuf[j] = 1.f;
             } else {
-                float uval = (tau2 != 0.f) ? ubuf[j] * inv2 : ubuf[j];
-                ubuf[j] = uval;
-                rowi[j] = uval;
+                const float uv = (tau2 != 0.f) ? ubuf[j] * inv2 : ubuf[j];
+                ubuf[j] = uv;
+                rowi[j] = uv;
             }
         }
         if (tid == 0) {
@@ -193,67 +260,12 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
             P.taup[i] = tau2;
         }
         __syncthreads();
-
-        // ---- rows r > i: left update (v_r, w), then right update (u, tau2); each
-        // wavefront keeps four rows in flight
-        if (nc <= 8 * 64) {
-            for (int r = i + 1 + 4 * wv; r < M; r += 4 * (NT / 64)) {
-                float x[4][8], acc[4];
-                float *rp[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) rp[q] = (r + q < M) ? S.row(r + q) : nullptr;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-#pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        const int j = i + 1 + lane + 64 * c;
-                        x[q][c] = (rp[q] && j < N) ? rp[q][j] : 0.f;
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float vr = (tau != 0.f && rp[q]) ? vbuf[r + q] : 0.f;
-                    acc[q] = 0.f;
-#pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        const int j = i + 1 + lane + 64 * c;
-                        if (j < N) {
-                            x[q][c] -= vr * wbuf[j];
-                            acc[q] += x[q][c] * ubuf[j];
-                        }
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] = wave_sum(acc[q]) * tau2;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (rp[q]) {
-#pragma unroll
-                        for (int c = 0; c < 8; ++c) {
-                            const int j = i + 1 + lane + 64 * c;
-                            if (j < N) rp[q][j] = x[q][c] - acc[q] * ubuf[j];
-                        }
-                    }
-                }
-            }
-        } else {
-            for (int r = i + 1 + wv; r < M; r += NT / 64) {
-                const float vr = (tau != 0.f) ? vbuf[r] : 0.f;
-                float *row = S.row(r);
-                float acc = 0.f;
-                for (int j = i + 1 + lane; j < N; j += 64) {
-                    float x = row[j] - vr * wbuf[j];
-                    row[j] = x;
-                    acc += x * ubuf[j];
-                }
-                acc = wave_sum(acc) * tau2;
-                for (int j = i + 1 + lane; j < N; j += 64) row[j] = row[j] - acc * ubuf[j];
-            }
-        }
+        // ---- fused pass: rows > i updated, column i+1 data gathered
+        ss = fused_pass<NT>(S, M, N, i, tau, inv, tau2, cbuf, wbuf, ubuf, rowbuf, pwr);
         __syncthreads();
     }
     __syncthreads();
-    for (int r = S.split + wv; r < M; r += NT / 64)
+    for (int r = S.split + wv; r < M; r += NW)
         for (int c = lane; c < N; c += 64) P.a[(size_t)r * ldg + c] = S.L[(size_t)(r - S.split) * S.ldl + c];
 }
 
@@ -452,7 +464,7 @@ __global__ __launch_bounds__(BT) void transpose_vt_kernel(ProbList pl, const int
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-size_t bidiag_wg_lds_bytes(int M, int N, int nt) { return sizeof(float) * ((size_t)M + 2 * (size_t)N + nt); }
+size_t bidiag_wg_lds_bytes(int M, int N, int nt) { return sizeof(float) * bidiag_vec_floats(M, N, nt); }
 size_t bidiag_wg_lds_resident_bytes(int M, int N, int nt)
 {
     return sizeof(float) * ((size_t)M * (size_t)(N | 1)) + bidiag_wg_lds_bytes(M, N, nt);
@@ -473,6 +485,7 @@ hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds
 {
     if (count <= 0) return hipSuccess;
     if (lds > bidiag_wg_lds_limit()) return hipErrorInvalidValue;
+    if (!pl.ptr && pl.single.N > WG_MAXN) return hipErrorInvalidValue;
     const unsigned lb = (unsigned)lds;
     if (nt == 1024)
         hipLaunchKernelGGL(bidiag_wg_kernel<1024>, dim3(count), dim3(1024), lds, s, pl, order, lb);

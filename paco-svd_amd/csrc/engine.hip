@@ -94,9 +94,9 @@ static void *internal_ws(size_t bytes)
 static bool use_grid_bidiag(int M, int N)
 {
     const char *env = getenv("PACO_BIDIAG");
-    if (env && !strcmp(env, "wg")) return false;
+    if (env && !strcmp(env, "wg") && N <= BIDIAG_WG_MAXN) return false;
     if (env && !strcmp(env, "grid")) return true;
-    return (size_t)M * N >= (size_t)1024 * 1024;
+    return N > BIDIAG_WG_MAXN || (size_t)M * N >= (size_t)1024 * 1024;
 }
 
 static ProbDesc bidiag_desc(int M, int N, float *a, float *u, float *vt, float *d, float *e, float *tauq,
@@ -702,11 +702,32 @@ int paco_svd_plain(const paco_svd_job *jobs, int count, void *stream)
     return 0;
 }
 
+// Jobs wider than the one-workgroup bidiagonalization go through the chip-wide path.
+static bool split_wide(const paco_svd_job *jobs, int count, std::vector<paco_svd_job> &narrow,
+                       std::vector<paco_svd_job> &wide)
+{
+    for (int i = 0; i < count; ++i) (jobs[i].N > BIDIAG_WG_MAXN ? wide : narrow).push_back(jobs[i]);
+    return !wide.empty();
+}
+
+static int batched_narrow(const paco_svd_job *jobs, int count, void *stream);
+
 int paco_svd_batched(const paco_svd_job *jobs, int count, void *stream)
 {
     if (count <= 0) return 0;
     for (int i = 0; i < count; ++i)
         if (!job_ok(jobs[i])) return PACO_EINVAL;
+    std::vector<paco_svd_job> narrow, wide;
+    if (split_wide(jobs, count, narrow, wide)) {
+        int rc = narrow.empty() ? 0 : batched_narrow(narrow.data(), (int)narrow.size(), stream);
+        if (rc) return rc;
+        return paco_svd_plain(wide.data(), (int)wide.size(), stream);
+    }
+    return batched_narrow(jobs, count, stream);
+}
+
+static int batched_narrow(const paco_svd_job *jobs, int count, void *stream)
+{
     std::lock_guard<std::mutex> lk(g_mu);
     BatchPlan bp;
     Prepared pr;
@@ -725,6 +746,10 @@ int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
     if (count <= 0) return 0;
     for (int i = 0; i < count; ++i)
         if (!job_ok(jobs[i])) return PACO_EINVAL;
+    {
+        std::vector<paco_svd_job> narrow, wide;
+        if (split_wide(jobs, count, narrow, wide)) return paco_svd_batched(jobs, count, stream);
+    }
     hipStream_t s = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(g_mu);
     const uint64_t sig = batch_signature(jobs, count);

@@ -55,6 +55,7 @@ inline size_t allow_dynamic_lds(const void *func)
 size_t bidiag_wg_lds_bytes(int M, int N, int nt = 256);          // vectors only
 size_t bidiag_wg_lds_resident_bytes(int M, int N, int nt = 256); // whole matrix in LDS
 size_t bidiag_wg_lds_limit();
+constexpr int BIDIAG_WG_MAXN = 512; // widest matrix of the one-workgroup kernel
 hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s, int nt = 256);
 hipError_t launch_tfactor(ProbList pl, const int *blk_off, int nblocks, int side, hipStream_t s);
 hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int side, hipStream_t s);
