@@ -112,6 +112,87 @@ def accuracy_sample(torch, host, offs, Ms, Ns, outs, n=40):
     return worst
 
 
+def serving_requests(count, rate=2000.0, seed=5):
+    """Request stream of the serving configuration: Poisson arrivals at `rate` req/s,
+    M log-uniform in [32, 4096], aspect M/N uniform in [1, 4], 10% of requests with
+    clustered sigma (min(1000, N) values 1 + 1e-7 u), the rest log-uniform in [1e-3, 1].
+    Drawn in order per request from numpy default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t = 0.0
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        M = int(round(np.exp(rng.uniform(np.log(32), np.log(4096)))))
+        a = rng.uniform(1.0, 4.0)
+        N = max(1, int(round(M / a)))
+        clustered = bool(rng.uniform() < 0.1)
+        sig = 10.0 ** rng.uniform(-3.0, 0.0, N)
+        if clustered:
+            k = min(1000, N)
+            sig[:k] = 1.0 + 1e-7 * rng.uniform(size=k)
+        reqs.append(dict(t=t, M=M, N=N, clustered=clustered, sigma=np.sort(sig)[::-1].copy()))
+    return reqs
+
+
+def haar_matrix(torch, M, N, sigma, seed):
+    """A = Q1 diag(sigma) Q2^T with Haar factors (QR of Gaussian matrices with the sign
+    of diag(R) folded in), generated on the GPU in FP32."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    G1 = torch.randn((M, N), generator=g, device="cuda")
+    G2 = torch.randn((N, N), generator=g, device="cuda")
+    Q1, R1 = torch.linalg.qr(G1)
+    Q2, R2 = torch.linalg.qr(G2)
+    Q1 = Q1 * torch.sign(torch.diagonal(R1))[None, :]
+    Q2 = Q2 * torch.sign(torch.diagonal(R2))[None, :]
+    s = torch.from_numpy(sigma.astype(np.float32)).cuda()
+    return ((Q1 * s[None, :]) @ Q2.T.contiguous()).contiguous()
+
+
+def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0):
+    from svdcheck import measures
+    reqs = serving_requests(count, rate)
+    print("[bench] serving: generating %d requests" % count, flush=True)
+    As = [haar_matrix(torch, r["M"], r["N"], r["sigma"], 500000 + i) for i, r in enumerate(reqs)]
+    outs = pacosvd.alloc_outputs(As)
+    torch.cuda.synchronize()
+    srv = pacosvd.Server(int(budget_gb * (1 << 30)), max_batch=512, batch_lanes=2, wide_lanes=2,
+                         max_M=4096, max_N=4096)
+    try:
+        print("[bench] serving: submitting", flush=True)
+        t0 = time.perf_counter()
+        for i, r in enumerate(reqs):
+            while time.perf_counter() - t0 < r["t"]:
+                pass
+            rc = srv.submit(As[i], *outs[i], i)
+            if rc != 0:
+                raise RuntimeError("submit failed for request %d (%dx%d): %d" % (i, r["M"], r["N"], rc))
+        t_sub = time.perf_counter() - t0
+        srv.drain()
+        t_all = time.perf_counter() - t0
+        st = srv.stats()
+    finally:
+        srv.close()
+    torch.cuda.synchronize()
+    # accuracy on a sample of the requests vs FP64 LAPACK
+    worst = dict(sigma_err=0.0, resid_ratio=0.0, orth_u=0.0, orth_v=0.0)
+    sample = [i for i in range(count) if reqs[i]["M"] <= 1200][:: max(1, count // 40)][:12]
+    for i in sample:
+        m = measures(As[i].cpu().numpy(), *(t.cpu().numpy() for t in outs[i]))
+        for k in worst:
+            worst[k] = max(worst[k], m[k])
+    worst["matrices_checked"] = len(sample)
+    flops = sum(svd_flops(r["M"], r["N"]) for r in reqs)
+    return dict(requests=count, offered_rate=rate, submit_span_s=round(t_sub, 3), total_s=round(t_all, 3),
+                achieved_svds_per_s=round(st.get("svds_per_s", 0.0), 1),
+                p50_ms=round(st.get("p50_ms", 0.0), 2), p99_ms=round(st.get("p99_ms", 0.0), 2),
+                mean_ms=round(st.get("mean_ms", 0.0), 2), batches=int(st.get("batches", 0)),
+                mean_batch=round(st.get("mean_batch", 0.0), 1), max_lanes_busy=int(st.get("max_lanes_busy", 0)),
+                wide_requests=int(st.get("wide_requests", 0)), clustered=sum(r["clustered"] for r in reqs),
+                offered_tflops=round(flops * rate / count / 1e12, 2),
+                achieved_tflops=round(flops / max(t_all, 1e-9) / 1e12, 3),
+                accuracy_vs_fp64_lapack=worst)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +202,8 @@ def main():
     ap.add_argument("--plain-budget", type=float, default=20.0, help="seconds for the plain-path measurement")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400, help="requests of the serving run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--profile-batched", action="store_true",
                     help="only run the batched step (warmup + steps), for counter collection")
     args = ap.parse_args()
@@ -190,6 +273,11 @@ def main():
     del plain_outs
     print("[bench] plain %d matrices %.2f s" % (done, dt_p), flush=True)
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = serving_bench(torch, pacosvd, args.serve_requests, args.serve_rate)
+        print("[bench] serving %s" % serving, flush=True)
+
     # ---- roofline of the bidiagonalization kernel (one launch over all matrices)
     bb = sum(bidiag_bytes(int(m), int(n)) for m, n in zip(Ms, Ns))
     t_bid = ph["bidiag"] / 1000.0
@@ -225,8 +313,11 @@ def main():
         "plain_svds_per_s": round(plain_svds_s, 1),
         "plain_matrices_timed": done,
         "speedup_vs_plain": round(best / plain_svds_s, 2),
-        "latency_ms": {"p50": round(ms_step, 2), "p99": round(ms_step, 2),
-                       "note": "batched: every SVD of a step completes with the step"},
+        "latency_ms": ({"p50": serving["p50_ms"], "p99": serving["p99_ms"],
+                        "source": "serving run (Poisson arrivals, continuous batching)"} if serving else
+                       {"p50": round(ms_step, 2), "p99": round(ms_step, 2),
+                        "source": "batched step: every SVD completes with the step"}),
+        "serving": serving,
         "phase_ms": {k: round(v, 3) for k, v in ph.items()},
         "achieved_tflops": round(flops / (1000.0 * len(As) / best / 1000.0) / 1e12 / 1.0, 3),
         "accuracy_vs_fp64_lapack": acc,

@@ -116,6 +116,31 @@ int paco_gemm_f32(void *stream, int transA, int transB, int m, int n, int k, flo
 int paco_timing_enable(int on);
 int paco_timing_read(float *ms, int max);
 
+/* ---- serving loop (no counterpart in the reference; main.cpp runs one matrix) ----
+ * Continuous batching of SVD requests whose buffers stay resident on the device.
+ * hbm_budget_bytes bounds the library scratch of admitted requests (their footprint
+ * is charged at admission and released at completion); narrow requests are packed in
+ * arrival order into batches of at most max_batch on batch_lanes streams, wide ones
+ * (N > 512 or M*N >= 2^20) run one at a time on wide_lanes streams through the two
+ * entry points; lanes overlap on the device.  max_M, max_N size the wide lanes.
+ * Returns an opaque handle or NULL. */
+void *paco_server_create(size_t hbm_budget_bytes, int max_batch, int batch_lanes, int wide_lanes, int max_M,
+                         int max_N);
+/* Queue one request (thread-safe, returns at once).  tag is returned by poll.
+ * PACO_EINVAL when the request could never be admitted (bad shape, over the budget). */
+int paco_server_submit(void *server, const paco_svd_job *job, uint64_t tag);
+/* Copy up to max finished requests (tag, latency in ms from submit to observed
+ * completion) and forget them; returns the count. */
+int paco_server_poll(void *server, uint64_t *tags, double *latency_ms, int max);
+/* Block until every queued request has finished; returns the first error or 0. */
+int paco_server_drain(void *server);
+/* Statistics since creation: [completed, p50 ms, p99 ms, mean ms, SVDs/s over first
+ * submit .. last completion, batches, mean batch size, most lanes busy at once,
+ * wide requests]; returns the count written (at most 9). */
+int paco_server_stats(void *server, double *out, int max);
+/* Drain and release the server. */
+void paco_server_destroy(void *server);
+
 /* Release the library's pools (device memory, pinned staging, graphs). */
 void paco_release(void);
 

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/paco_svd.h"
+#include "lane.hpp"
 #include "paco_kernels.hpp"
 
 namespace paco {
@@ -622,6 +623,80 @@ static uint64_t batch_signature(const paco_svd_job *jobs, int count)
 static bool job_ok(const paco_svd_job &j)
 {
     return j.M >= j.N && j.N >= 1 && j.A && j.S && j.U && j.Vt && j.N <= PACO_MAX_N;
+}
+
+// ---------------------------------------------------------------------------
+// lanes for the serving loop (lane.hpp)
+// ---------------------------------------------------------------------------
+struct Lane {
+    BatchRes res;
+    size_t capacity = 0;
+    int max_jobs = 0;
+};
+
+size_t job_footprint(int M, int N)
+{
+    return scratch_layout(M, N).total + sizeof(ProbDesc) + 64 * sizeof(int) +
+           sizeof(MergeState) * (size_t)std::max(total_merges(N), 1) + 4096;
+}
+
+bool job_is_wide(int M, int N) { return use_grid_bidiag(M, N); }
+
+Lane *lane_create(size_t capacity, int max_jobs)
+{
+    Lane *l = new Lane();
+    l->capacity = capacity;
+    l->max_jobs = max_jobs;
+    // plan blob bound: descriptors + per-job prefix entries of every array (+ rounds)
+    const size_t pin = (size_t)max_jobs * (sizeof(ProbDesc) + 64 * sizeof(int)) + (1 << 20);
+    if (ensure(l->res, capacity + pin, pin) != hipSuccess) {
+        delete l;
+        return nullptr;
+    }
+    return l;
+}
+
+void lane_destroy(Lane *l)
+{
+    if (!l) return;
+    if (l->res.dev) (void)hipFree(l->res.dev);
+    if (l->res.pinned) (void)hipHostFree(l->res.pinned);
+    if (l->res.ev) (void)hipEventDestroy(l->res.ev);
+    if (l->res.evf) (void)hipEventDestroy(l->res.evf);
+    if (l->res.evj) (void)hipEventDestroy(l->res.evj);
+    if (l->res.side) (void)hipStreamDestroy(l->res.side);
+    delete l;
+}
+
+int lane_enqueue(Lane *l, const paco_svd_job *jobs, int count, hipStream_t s)
+{
+    if (count <= 0) return 0;
+    if (count > l->max_jobs) return PACO_EINVAL;
+    BatchPlan bp;
+    Prepared pr;
+    hipError_t e = prepare_batch(jobs, count, bp, l->res, pr);
+    if (e != hipSuccess) return (e == hipErrorInvalidValue) ? PACO_EINVAL : (int)e;
+    e = enqueue_batch(bp, l->res, pr, s, true);
+    return (int)e;
+}
+
+size_t wide_scratch_bytes(int M, int N) { return scratch_layout(M, N).total + ws_layout(M, N).total; }
+
+int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s)
+{
+    const int M = job.M, N = job.N;
+    const BatchScratch L = scratch_layout(M, N);
+    char *base = static_cast<char *>(scratch);
+    char *ws = base + L.total;
+    float *a = reinterpret_cast<float *>(base + L.a);
+    hipError_t e = hipMemcpyAsync(a, job.A, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return (int)e;
+    int rc = bidiag_single(M, N, a, job.U, job.Vt, job.S, reinterpret_cast<float *>(base + L.e),
+                           reinterpret_cast<float *>(base + L.tauq), reinterpret_cast<float *>(base + L.taup), ws, s);
+    if (rc) return rc;
+    return dc_single(M, N, a, job.U, job.Vt, job.S, reinterpret_cast<float *>(base + L.e),
+                     reinterpret_cast<float *>(base + L.qt), reinterpret_cast<float *>(base + L.wt),
+                     reinterpret_cast<uint32_t *>(base + L.idx), ws, nullptr, s);
 }
 
 } // namespace paco

@@ -1,0 +1,28 @@
+// Internal interface between the batched engine (engine.hip) and the serving loop
+// (serve.hip): a lane owns preallocated device scratch, a pinned plan buffer and a
+// forked stream, so enqueueing a batch on it allocates nothing.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "../../include/paco_svd.h"
+
+namespace paco {
+
+struct Lane;
+
+// Device bytes one job needs in a lane (scratch + its share of the plan and states).
+size_t job_footprint(int M, int N);
+// True when the job goes through the chip-wide path instead of a batch.
+bool job_is_wide(int M, int N);
+// A lane able to hold `capacity` bytes of job footprints and `max_jobs` jobs.
+Lane *lane_create(size_t capacity, int max_jobs);
+void lane_destroy(Lane *l);
+// Enqueue the batched pipeline of the jobs on stream s (stream-ordered, no host sync
+// beyond the lane's previous plan upload).  Returns 0 or an error code.
+int lane_enqueue(Lane *l, const paco_svd_job *jobs, int count, hipStream_t s);
+// Scratch for one wide job through the two entry points.
+size_t wide_scratch_bytes(int M, int N);
+int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s);
+
+} // namespace paco

@@ -15,6 +15,8 @@ EXPORTS = [
     "paco_workspace_size", "paco_set_precision", "paco_svd_plain", "paco_svd_batched",
     "paco_svd_batched_graph", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
     "paco_timing_enable", "paco_timing_read",
+    "paco_server_create", "paco_server_submit", "paco_server_poll", "paco_server_drain", "paco_server_stats",
+    "paco_server_destroy",
 ]
 PHASES = ["copy_in", "bidiag", "form_qp", "dc", "backtransform"]
 
@@ -53,6 +55,15 @@ def lib():
                                 ctypes.c_float, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, _vp, ctypes.c_int]
     L.paco_timing_enable.argtypes = [ctypes.c_int]
     L.paco_timing_read.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+    L.paco_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int]
+    L.paco_server_create.restype = _vp
+    L.paco_server_submit.argtypes = [_vp, ctypes.POINTER(SvdJob), ctypes.c_uint64]
+    L.paco_server_poll.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    L.paco_server_drain.argtypes = [_vp]
+    L.paco_server_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    L.paco_server_destroy.argtypes = [_vp]
+    L.paco_server_destroy.restype = None
     L.paco_release.argtypes = []
     L.paco_release.restype = None
     _lib = L
@@ -197,3 +208,40 @@ def timing_read():
     buf = (ctypes.c_float * len(PHASES))()
     n = lib().paco_timing_read(buf, len(PHASES))
     return {PHASES[i]: float(buf[i]) for i in range(n)}
+
+
+SERVER_STATS = ["completed", "p50_ms", "p99_ms", "mean_ms", "svds_per_s", "batches", "mean_batch", "max_lanes_busy",
+                "wide_requests"]
+
+
+class Server:
+    """Continuous-batching SVD server (paco_server_*).  Requests reference device
+    tensors that must stay alive until they are reported finished."""
+
+    def __init__(self, hbm_budget_bytes, max_batch=256, batch_lanes=2, wide_lanes=1, max_M=4096, max_N=4096):
+        self.h = lib().paco_server_create(hbm_budget_bytes, max_batch, batch_lanes, wide_lanes, max_M, max_N)
+        if not self.h:
+            raise RuntimeError("paco_server_create failed")
+
+    def submit(self, A, S, U, Vt, tag):
+        job = SvdJob(A.shape[0], A.shape[1], A.data_ptr(), S.data_ptr(), U.data_ptr(), Vt.data_ptr())
+        return lib().paco_server_submit(self.h, ctypes.byref(job), tag)
+
+    def poll(self, max_n=4096):
+        tags = (ctypes.c_uint64 * max_n)()
+        lat = (ctypes.c_double * max_n)()
+        n = lib().paco_server_poll(self.h, tags, lat, max_n)
+        return [(int(tags[i]), float(lat[i])) for i in range(n)]
+
+    def drain(self):
+        _check(lib().paco_server_drain(self.h), "paco_server_drain")
+
+    def stats(self):
+        buf = (ctypes.c_double * len(SERVER_STATS))()
+        n = lib().paco_server_stats(self.h, buf, len(SERVER_STATS))
+        return {SERVER_STATS[i]: float(buf[i]) for i in range(n)}
+
+    def close(self):
+        if self.h:
+            lib().paco_server_destroy(self.h)
+            self.h = None

@@ -277,3 +277,45 @@ def test_bidiag_workgroup_sizes(torch_dev, threads):
         m = measures(A, *(np_of(x) for x in outs[i]))
         assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL, (i, m)
         assert m["resid_ratio"] <= RESID_TOL
+
+
+def test_server_alone_vs_mix_and_admission(torch_dev):
+    """Continuous batching: every request's result equals the one computed alone."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(17)
+    shapes = [(40, 30), (200, 150), (1100, 1000), (64, 64), (300, 120), (17, 9), (600, 550), (128, 96)]
+    mats = [rng.standard_normal(sh).astype(np.float32) for sh in shapes]
+    say("server", shapes)
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    outs = pacosvd.alloc_outputs(As)
+    srv = pacosvd.Server(2 << 30, max_batch=4, batch_lanes=2, wide_lanes=1, max_M=1100, max_N=1100)
+    try:
+        for i, (A, (S, U, Vt)) in enumerate(zip(As, outs)):
+            assert srv.submit(A, S, U, Vt, i) == 0
+        srv.drain()
+        done = srv.poll()
+        assert sorted(t for t, _ in done) == list(range(len(As)))
+        st = srv.stats()
+        say("stats", st)
+        assert st["completed"] == len(As) and st["wide_requests"] == 2
+        # a request that can never be admitted is refused at submission
+        big = torch.zeros((2000, 1500), device="cuda")
+        o = pacosvd.alloc_outputs([big])[0]
+        assert srv.submit(big, *o, 99) == -1
+    finally:
+        srv.close()
+    for i, A in enumerate(mats):
+        m = measures(A, *(np_of(x) for x in outs[i]))
+        assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL, (i, m)
+    # alone: a fresh server, one request at a time
+    srv = pacosvd.Server(2 << 30, max_batch=4, batch_lanes=1, wide_lanes=1, max_M=1100, max_N=1100)
+    try:
+        for i in (0, 2, 4, 6):
+            alone = pacosvd.alloc_outputs([As[i]])[0]
+            assert srv.submit(As[i], *alone, 100 + i) == 0
+            srv.drain()
+            for x, y in zip(alone, outs[i]):
+                assert torch.equal(x, y), i
+    finally:
+        srv.close()
