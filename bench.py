@@ -193,6 +193,65 @@ def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0):
                 accuracy_vs_fp64_lapack=worst)
 
 
+def single_svd(torch, pacosvd, A):
+    """Both entry points on one device matrix, with per-phase hipEvent times."""
+    M, N = A.shape
+    a = A.clone()
+    pacosvd.timing_enable(True)
+    r = pacosvd.upper_bidiagonalization(a)
+    d, u, vt = pacosvd.svd_DC(M, N, r["a"], r["u"], r["vt"], r["d"], r["e"], ws=r["ws"])
+    ph = pacosvd.timing_read()
+    torch.cuda.synchronize()
+    return d, u, vt, ph
+
+
+def plain_large_bench(torch, pacosvd, M=4096, N=4096, seed=1, check=True):
+    """Plain path on one FP32 M x N N(0,1) matrix (default_rng(seed)): whole SVD through
+    the two entry points; bidiagonalization in HBM GB/s, back-transform GEMMs in TFLOP/s."""
+    from svdcheck import measures
+    A_np = np.random.default_rng(seed).standard_normal((M, N)).astype(np.float32)
+    A = torch.from_numpy(A_np).cuda()
+    single_svd(torch, pacosvd, A)  # warmup
+    t0 = time.perf_counter()
+    d, u, vt, ph = single_svd(torch, pacosvd, A)
+    wall = time.perf_counter() - t0
+    ph.pop("copy_in", None)
+    bb = bidiag_bytes(M, N)
+    bt_flops = 2.0 * M * N * N + 2.0 * N ** 3
+    out = dict(shape=[M, N], wall_ms=round(1000 * wall, 2), phase_ms={k: round(v, 3) for k, v in ph.items()},
+               bidiag_gbs=round(bb / (ph["bidiag"] / 1000) / 1e9, 1),
+               bidiag_hbm_frac=round(bb / (ph["bidiag"] / 1000) / 1e9 / HBM_PEAK_GBS, 4),
+               backtransform_tflops=round(bt_flops / (ph["backtransform"] / 1000) / 1e12, 2),
+               backtransform_mfma_frac=round(bt_flops / (ph["backtransform"] / 1000) / 1e12 / FP32_MFMA_PEAK_TFS, 4))
+    if check:
+        out["accuracy_vs_fp64_lapack"] = measures(A_np, d.cpu().numpy(), u.cpu().numpy(), vt.cpu().numpy())
+    return out
+
+
+def precision_bench(torch, pacosvd, M=8192, N=4096, seed=2):
+    """Graded sigma_i = 10^(-6 i / (N-1)), A = Q1 diag(sigma) Q2^T (Haar factors from the
+    GPU QR of Gaussian matrices, torch generator seed `seed`); FP32 matrix-core GEMMs vs
+    BF16x3 split, both against FP64 LAPACK dgesdd of the FP32 matrix."""
+    from svdcheck import measures, lapack_sigma
+    sig = 10.0 ** (-6.0 * np.arange(N) / (N - 1))
+    A = haar_matrix(torch, M, N, sig, seed)
+    A_np = A.cpu().numpy()
+    s_ref = lapack_sigma(A_np)
+    res = {}
+    for mode, name in [(0, "fp32_mfma"), (1, "bf16x3_split")]:
+        old = pacosvd.set_precision(mode)
+        try:
+            d, u, vt, ph = single_svd(torch, pacosvd, A)
+        finally:
+            pacosvd.set_precision(old)
+        s = d.cpu().numpy()
+        m = measures(A_np, s, u.cpu().numpy(), vt.cpu().numpy(), s_ref=s_ref)
+        m["sigma_rel_err_per_value_max"] = float(np.max(np.abs(s - s_ref) / s_ref))
+        m["phase_ms"] = {k: round(v, 3) for k, v in ph.items()}
+        res[name] = m
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +263,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--plain-large", action="store_true", help="also run the 4096^2 plain-path configuration")
+    ap.add_argument("--precision", action="store_true", help="also run the 8192x4096 precision configuration")
     ap.add_argument("--profile-batched", action="store_true",
                     help="only run the batched step (warmup + steps), for counter collection")
     args = ap.parse_args()
@@ -278,6 +339,15 @@ def main():
         serving = serving_bench(torch, pacosvd, args.serve_requests, args.serve_rate)
         print("[bench] serving %s" % serving, flush=True)
 
+    plain_large = None
+    if args.plain_large:
+        plain_large = plain_large_bench(torch, pacosvd)
+        print("[bench] plain 4096^2 %s" % plain_large, flush=True)
+    precision = None
+    if args.precision:
+        precision = precision_bench(torch, pacosvd)
+        print("[bench] precision %s" % precision, flush=True)
+
     # ---- roofline of the bidiagonalization kernel (one launch over all matrices)
     bb = sum(bidiag_bytes(int(m), int(n)) for m, n in zip(Ms, Ns))
     t_bid = ph["bidiag"] / 1000.0
@@ -318,6 +388,8 @@ def main():
                        {"p50": round(ms_step, 2), "p99": round(ms_step, 2),
                         "source": "batched step: every SVD completes with the step"}),
         "serving": serving,
+        "plain_4096x4096": plain_large,
+        "precision_8192x4096": precision,
         "phase_ms": {k: round(v, 3) for k, v in ph.items()},
         "achieved_tflops": round(flops / (1000.0 * len(As) / best / 1000.0) / 1e12 / 1.0, 3),
         "accuracy_vs_fp64_lapack": acc,

@@ -121,6 +121,17 @@ static ProbDesc bidiag_desc(int M, int N, float *a, float *u, float *vt, float *
     return P;
 }
 
+// Phase events (batched path outside graph capture, and the two entry points).
+static const int kPhases = 5;
+static bool g_timing = false;
+static hipEvent_t g_tev[kPhases + 1] = {};
+static bool g_tev_recorded = false;
+
+static void tmark(hipStream_t s, int i, bool on)
+{
+    if (on && g_tev[i]) (void)hipEventRecord(g_tev[i], s);
+}
+
 // Launch shape of the one-workgroup bidiagonalization for an M x N matrix: matrices
 // that fit LDS whole run with 256 threads and exactly their LDS; larger ones with the
 // larger workgroup and the whole LDS budget (bottom rows cached).  The tuned table
@@ -176,6 +187,12 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     pl.count = 1;
     pl.single = bidiag_desc(M, N, a, u, vt, d, e, tauq, taup, ws, L);
     hipError_t err;
+    const bool tm = g_timing;
+    if (tm) {
+        g_tev_recorded = true;
+        tmark(s, 0, tm);
+        tmark(s, 1, tm);
+    }
     if (use_grid_bidiag(M, N)) {
         err = launch_bidiag_grid(pl.single, ws + L.grid, s);
     } else {
@@ -183,9 +200,11 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
         err = launch_bidiag_wg(pl, nullptr, 1, c.lds, s, c.nt);
     }
     if (err != hipSuccess) return (int)err;
+    tmark(s, 2, tm);
     const int nt = cdiv(N, 32);
     err = form_q_pt(pl, nullptr, M, nullptr, N, nullptr, cdiv(N, 32), nullptr, cdiv(std::max(N - 1, 0), 32), nullptr,
                     cdiv(M, 64), nullptr, cdiv(N, 64), nullptr, nt * (nt + 1) / 2, s);
+    tmark(s, 3, tm);
     return (int)err;
 }
 
@@ -261,7 +280,17 @@ static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, floa
     o.nrv = N;
     const uint32_t *blob = nullptr;
     if (tiling_dev) blob = reinterpret_cast<const uint32_t *>(static_cast<const char *>(tiling_dev) + 64);
-    return (int)run_dc(pl, o, blob, s);
+    const bool tm = g_timing;
+    if (tm) {
+        g_tev_recorded = true;
+        tmark(s, 3, tm);
+    }
+    hipError_t err = run_dc_levels(pl, o, blob, s);
+    if (err != hipSuccess) return (int)err;
+    tmark(s, 4, tm);
+    err = run_dc_backtransform(pl, o, s);
+    tmark(s, 5, tm);
+    return (int)err;
 }
 
 // ---------------------------------------------------------------------------
@@ -515,17 +544,6 @@ static hipError_t prepare_batch(const paco_svd_job *jobs, int count, BatchPlan &
     }
     memcpy(res.pinned, bp.blob.data(), bp.blob.size());
     return hipSuccess;
-}
-
-// Phase events of the batched path (recorded only outside graph capture).
-static const int kPhases = 5;
-static bool g_timing = false;
-static hipEvent_t g_tev[kPhases + 1] = {};
-static bool g_tev_recorded = false;
-
-static void tmark(hipStream_t s, int i, bool on)
-{
-    if (on && g_tev[i]) (void)hipEventRecord(g_tev[i], s);
 }
 
 // Upload + the whole launch sequence (capturable: no allocation, no host sync).
