@@ -338,27 +338,31 @@ __global__ __launch_bounds__(BT) void tfactor_kernel(ProbList pl, const int *blk
 }
 
 // X <- (H_0 ... H_{n-1}) X, X initially the identity: X = Q (side 0, X = u, M x M) or
-// X = P (side 1, X = vt, N x N, transposed afterwards).  One workgroup per strip of
-// 64 columns; 4 row groups; blocks applied in descending order.
+// X = P (side 1, X = vt, N x N, transposed afterwards).  One workgroup per strip of SW
+// columns (SW = 64 for batches; 16 for a single large matrix so every CU has a strip),
+// 256 / SW row groups; blocks applied in descending order.
+template <int SW>
 __global__ __launch_bounds__(BT) void wy_apply_kernel(ProbList pl, const int *strip_off, int side)
 {
+    constexpr int G = BT / SW;
     const int pid = find_task(strip_off, pl.count, blockIdx.x);
     const int strip = blockIdx.x - off_at(strip_off, pid);
     const ProbDesc P = pl.get(pid);
     const int L = refl_len(P, side), nref = refl_count(P, side);
     float *X = side == 0 ? P.u : P.vt;
     const int ldx = L;
-    const int col = strip * 64 + (threadIdx.x & 63), grp = threadIdx.x >> 6;
+    const int cl = threadIdx.x % SW, grp = threadIdx.x / SW;
+    const int col = strip * SW + cl;
     const bool colok = col < L;
     __shared__ float Vc[64][33];
     __shared__ float Ts[32][33];
-    __shared__ float Wr[4][32][65];
+    __shared__ float Wr[G][32][SW + 1];
     const float *Tbase = side == 0 ? P.tq : P.tp;
     const int nblk = (nref + 31) / 32;
     for (int b = nblk - 1; b >= 0; --b) {
         const int i0 = b * 32, nb = min(32, nref - i0);
         const int rfirst = i0 + side;
-        if (strip * 64 + 63 < i0 + side) continue; // every column of the strip is untouched
+        if (strip * SW + SW - 1 < i0 + side) continue; // every column of the strip is untouched
         for (int q = threadIdx.x; q < 1024; q += BT) Ts[q / 32][q % 32] = Tbase[(size_t)b * 1024 + q];
         float w[32];
 #pragma unroll
@@ -371,7 +375,7 @@ __global__ __launch_bounds__(BT) void wy_apply_kernel(ProbList pl, const int *st
             }
             __syncthreads();
             if (colok) {
-                for (int rr = grp; rr < 64 && r0 + rr < L; rr += 4) {
+                for (int rr = grp; rr < 64 && r0 + rr < L; rr += G) {
                     const float x = X[(size_t)(r0 + rr) * ldx + col];
 #pragma unroll
                     for (int t = 0; t < 32; ++t) w[t] += Vc[rr][t] * x;
@@ -379,22 +383,24 @@ __global__ __launch_bounds__(BT) void wy_apply_kernel(ProbList pl, const int *st
             }
         }
 #pragma unroll
-        for (int t = 0; t < 32; ++t) Wr[grp][t][threadIdx.x & 63] = w[t];
+        for (int t = 0; t < 32; ++t) Wr[grp][t][cl] = w[t];
         __syncthreads();
-        // W = sum over groups; W2 = T W (T upper triangular)
-        float wsum[32];
+        // W = sum over row groups; W2 = T W (T upper triangular)
 #pragma unroll
-        for (int t = 0; t < 32; ++t)
-            wsum[t] = Wr[0][t][threadIdx.x & 63] + Wr[1][t][threadIdx.x & 63] + Wr[2][t][threadIdx.x & 63] +
-                      Wr[3][t][threadIdx.x & 63];
+        for (int t = 0; t < 32; ++t) {
+            float a = 0.f;
+#pragma unroll
+            for (int g = 0; g < G; ++g) a += Wr[g][t][cl];
+            w[t] = a;
+        }
         float w2[32];
 #pragma unroll
-        for (int s = 0; s < 32; ++s) {
+        for (int s2 = 0; s2 < 32; ++s2) {
             float acc = 0.f;
 #pragma unroll
             for (int t = 0; t < 32; ++t)
-                if (t >= s) acc += Ts[s][t] * wsum[t];
-            w2[s] = acc;
+                if (t >= s2) acc += Ts[s2][t] * w[t];
+            w2[s2] = acc;
         }
         for (int r0 = rfirst; r0 < L; r0 += 64) {
             __syncthreads();
@@ -404,7 +410,7 @@ __global__ __launch_bounds__(BT) void wy_apply_kernel(ProbList pl, const int *st
             }
             __syncthreads();
             if (colok) {
-                for (int rr = grp; rr < 64 && r0 + rr < L; rr += 4) {
+                for (int rr = grp; rr < 64 && r0 + rr < L; rr += G) {
                     float acc = 0.f;
 #pragma unroll
                     for (int t = 0; t < 32; ++t) acc += Vc[rr][t] * w2[t];
@@ -503,10 +509,13 @@ hipError_t launch_tfactor(ProbList pl, const int *blk_off, int nblocks, int side
     return hipGetLastError();
 }
 
-hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int side, hipStream_t s)
+hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int side, hipStream_t s, int sw)
 {
     if (nstrips <= 0) return hipSuccess;
-    hipLaunchKernelGGL(wy_apply_kernel, dim3(nstrips), dim3(BT), 0, s, pl, strip_off, side);
+    if (sw == 16)
+        hipLaunchKernelGGL(wy_apply_kernel<16>, dim3(nstrips), dim3(BT), 0, s, pl, strip_off, side);
+    else
+        hipLaunchKernelGGL(wy_apply_kernel<64>, dim3(nstrips), dim3(BT), 0, s, pl, strip_off, side);
     return hipGetLastError();
 }
 

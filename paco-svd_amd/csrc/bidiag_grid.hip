@@ -149,7 +149,9 @@ __global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
             float g2 = 0.f;
             for (int q = 0; q < GC; ++q) g2 += W.g2p[(size_t)q * M + row];
             float acc = A[(size_t)row * N + ip + 1] + inv2 * g2;
+#pragma unroll 8
             for (int t = 0; t <= jp; ++t) acc -= Vel(A, N, p, t, row) * t3s[t];
+#pragma unroll 8
             for (int t = 0; t < jp; ++t) acc -= W.X[(size_t)row * NB + t] * t4s[t];
             const float x = tau2 * acc;
             xs[row - r0] = x;
@@ -181,6 +183,7 @@ __global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
     float part = 0.f;
     for (int row = max(r0, i) + tid; row < r1; row += GBT) {
         float val = A[(size_t)row * N + i];
+#pragma unroll 8
         for (int t = 0; t < j; ++t) {
             const float xv = (t == j - 1) ? xs[row - r0] : W.X[(size_t)row * NB + t];
             val -= Vel(A, N, p, t, row) * ys[t] + xv * us[t];
@@ -199,18 +202,33 @@ __global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
         for (int row = rs; row < r1; ++row) acc += wc[row - r0] * A[(size_t)row * N + col];
         W.g1p[(size_t)rb * N + col] = acc;
     }
-    // t1p[rb][t] = sum_{row > i} V[row][t] wc[row], t2p[rb][t] = sum X[row][t] wc[row], t < j
-    if (cb == 0 && tid < NB) {
-        const int t = tid;
+    // t1p[rb][t] = sum_{row > i} V[row][t] wc[row], t2p[rb][t] = sum X[row][t] wc[row], t < j;
+    // 32 column lanes x 8 row groups, reduced through LDS
+    if (cb == 0) {
+        __shared__ float r1s[8][NB + 1], r2s[8][NB + 1];
+        const int t = tid % NB, g = tid / NB;
         float s1 = 0.f, s2 = 0.f;
-        if (t < j)
-            for (int row = rs; row < r1; ++row) {
+        if (t < j) {
+#pragma unroll 4
+            for (int row = rs + g; row < r1; row += 8) {
                 const float w = wc[row - r0];
                 s1 += Vel(A, N, p, t, row) * w;
                 s2 += ((t == j - 1) ? xs[row - r0] : W.X[(size_t)row * NB + t]) * w;
             }
-        W.t1p[rb * NB + t] = s1;
-        W.t2p[rb * NB + t] = s2;
+        }
+        r1s[g][t] = s1;
+        r2s[g][t] = s2;
+        __syncthreads();
+        if (tid < NB) {
+            float a1 = 0.f, a2 = 0.f;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                a1 += r1s[q][tid];
+                a2 += r2s[q][tid];
+            }
+            W.t1p[rb * NB + tid] = a1;
+            W.t2p[rb * NB + tid] = a2;
+        }
     }
 }
 
@@ -272,11 +290,13 @@ __global__ __launch_bounds__(GBT) void gb_row_kernel(PanelArgs a)
         float g1 = 0.f;
         for (int q = 0; q < GR; ++q) g1 += W.g1p[(size_t)q * N + col];
         float acc = A[(size_t)i * N + col] + inv * g1;
+#pragma unroll 8
         for (int t = 0; t < j; ++t) acc -= W.Y[(size_t)col * NB + t] * t1s[t] + Uel(A, N, p, t, col) * t2s[t];
         const float y = tau * acc;
         yl[col - c0] = y;
         if (rb == 0) W.Y[(size_t)col * NB + j] = y;
         float w = A[(size_t)i * N + col] - y * vi[j];
+#pragma unroll 8
         for (int t = 0; t < j; ++t) w -= W.Y[(size_t)col * NB + t] * vi[t] + Uel(A, N, p, t, col) * xi[t];
         wl[col - c0] = w;
         if (rb == 0) W.wrow[col] = w;
@@ -292,11 +312,14 @@ __global__ __launch_bounds__(GBT) void gb_row_kernel(PanelArgs a)
         acc = wave_sum(acc);
         if (lane == 0) W.g2p[(size_t)cb * M + row] = acc;
     }
-    // t3p[cb][t] = sum_{col > i+1} Y[col][t] w (t <= j), t4p[cb][t] = sum U[t][col] w (t < j)
-    if (rb == 0 && tid < NB) {
-        const int t = tid;
+    // t3p[cb][t] = sum_{col > i+1} Y[col][t] w (t <= j), t4p[cb][t] = sum U[t][col] w (t < j);
+    // 32 lanes over t x 8 column groups, reduced through LDS
+    if (rb == 0) {
+        __shared__ float r3s[8][NB + 1], r4s[8][NB + 1];
+        const int t = tid % NB, g = tid / NB;
         float s3 = 0.f, s4 = 0.f;
-        for (int col = cs; col < c1; ++col) {
+#pragma unroll 4
+        for (int col = cs + g; col < c1; col += 8) {
             const float w = wl[col - c0];
             if (t < j) {
                 s3 += W.Y[(size_t)col * NB + t] * w;
@@ -305,8 +328,19 @@ __global__ __launch_bounds__(GBT) void gb_row_kernel(PanelArgs a)
                 s3 += yl[col - c0] * w;
             }
         }
-        W.t3p[cb * NB + t] = s3;
-        W.t4p[cb * NB + t] = s4;
+        r3s[g][t] = s3;
+        r4s[g][t] = s4;
+        __syncthreads();
+        if (tid < NB) {
+            float a3 = 0.f, a4 = 0.f;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                a3 += r3s[q][tid];
+                a4 += r4s[q][tid];
+            }
+            W.t3p[cb * NB + tid] = a3;
+            W.t4p[cb * NB + tid] = a4;
+        }
     }
 }
 

@@ -472,6 +472,113 @@ __global__ __launch_bounds__(DT) void dc_secular_kernel(ProbList pl, LevelArgs l
     }
 }
 
+// Same iteration, one thread per root with serial pole loops: used for the small merges
+// of the low levels (k <= 96), where a wavefront per root would leave most lanes idle.
+__device__ inline int secular_serial(int k, const double *d, const double *z, int i, int &Kout, double &muout)
+{
+    const double eps = DBL_EPSILON;
+    if (k == 1) {
+        Kout = 0;
+        muout = z[0] * z[0];
+        return 0;
+    }
+    int K;
+    double lo, hi;
+    if (i == k - 1) {
+        double zz = 0;
+        for (int j = 0; j < k; ++j) zz += z[j] * z[j];
+        K = k - 1;
+        lo = 0.0;
+        hi = zz;
+    } else {
+        const double dl = d[i], dr = d[i + 1], om = 0.5 * (dl + dr);
+        double f = 1.0;
+        for (int j = 0; j < k; ++j) f += z[j] * z[j] / ((d[j] - om) * (d[j] + om));
+        if (f > 0) {
+            K = i;
+            lo = 0.0;
+            hi = (om - dl) * (om + dl);
+        } else {
+            K = i + 1;
+            lo = (om - dr) * (om + dr);
+            hi = 0.0;
+        }
+    }
+    const double dK = d[K];
+    double mu = 0.5 * (lo + hi);
+    int it;
+    for (it = 1; it <= 200; ++it) {
+        double psi = 0, dpsi = 0, phi = 0, dphi = 0, erretm = 0;
+        for (int j = 0; j < k; ++j) {
+            const double t = (d[j] - dK) * (d[j] + dK) - mu;
+            const double q = z[j] / t, term = z[j] * q;
+            if (j <= i) {
+                psi += term;
+                dpsi += q * q;
+            } else {
+                phi += term;
+                dphi += q * q;
+            }
+            erretm += fabs(term);
+        }
+        const double w = 1.0 + psi + phi;
+        if (w == 0.0 || fabs(w) <= 8.0 * eps * (1.0 + erretm)) break;
+        if (w < 0)
+            lo = mu;
+        else
+            hi = mu;
+        const double a1 = (d[i] - dK) * (d[i] + dK) - mu;
+        double next;
+        if (i == k - 1) {
+            const double s = a1 * a1 * dpsi, c = w - s / a1;
+            next = (c > 0) ? mu + a1 + s / c : 0.5 * (lo + hi);
+        } else {
+            const double a2 = (d[i + 1] - dK) * (d[i + 1] + dK) - mu;
+            const double s = a1 * a1 * dpsi, t = a2 * a2 * dphi;
+            const double c = w - s / a1 - t / a2;
+            const double B = c * (a1 + a2) + s + t, C = a1 * a2 * w;
+            double eta;
+            if (c == 0.0) {
+                eta = C / B;
+            } else {
+                double disc = B * B - 4.0 * c * C;
+                if (disc < 0) disc = 0;
+                const double sqd = sqrt(disc);
+                const double qq = 0.5 * (B + (B >= 0 ? sqd : -sqd));
+                const double r1 = qq / c, r2 = (qq != 0.0) ? C / qq : 0.0;
+                eta = (r1 > a1 && r1 < a2) ? r1 : r2;
+            }
+            next = mu + eta;
+        }
+        if (!(next > lo && next < hi)) next = 0.5 * (lo + hi);
+        const double step = fabs(next - mu);
+        mu = next;
+        if (step <= 2.0 * eps * fabs(mu) || (hi - lo) <= 4.0 * eps * fmax(fabs(lo), fabs(hi))) break;
+    }
+    Kout = K;
+    muout = mu;
+    return it;
+}
+
+__global__ __launch_bounds__(DT) void dc_secular_thread_kernel(ProbList pl, LevelArgs la)
+{
+    const int g = blockIdx.x * DT + threadIdx.x;
+    if (g >= la.ntask * la.stride) return;
+    const int i = g % la.stride;
+    const MergeCtx T = merge_ctx(pl, la, g / la.stride);
+    const MergeState &S = *T.S;
+    const int k = S.k;
+    if (i >= k) return;
+    const ProbDesc P = pl.get(T.prob);
+    const Arena Ar = make_arena(P.idx, T.c0, S.n);
+    int K;
+    double mu;
+    const int it = secular_serial(k, Ar.dk, Ar.zk, i, K, mu);
+    Ar.Kx[i] = K;
+    Ar.mu[i] = mu;
+    if (it > 200) atomicAdd(P.info, 1);
+}
+
 // ---------------------------------------------------------------------------
 // Loewner z (svd.cpp:1424-1438, in FP64) and output positions; one thread per pole.
 // ---------------------------------------------------------------------------
@@ -741,7 +848,10 @@ hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s)
     const int wave_blocks = (rows + DT / 64 - 1) / (DT / 64);
     hipLaunchKernelGGL(dc_deflate_kernel, dim3(la.ntask), dim3(DT), lds, s, pl, la);
     hipLaunchKernelGGL(dc_gather_kernel, dim3(la.ntask * spt), dim3(DT), 0, s, pl, la);
-    hipLaunchKernelGGL(dc_secular_kernel, dim3(wave_blocks), dim3(DT), 0, s, pl, la);
+    if (la.stride <= 96)
+        hipLaunchKernelGGL(dc_secular_thread_kernel, dim3((rows + DT - 1) / DT), dim3(DT), 0, s, pl, la);
+    else
+        hipLaunchKernelGGL(dc_secular_kernel, dim3(wave_blocks), dim3(DT), 0, s, pl, la);
     hipLaunchKernelGGL(dc_lowner_kernel, dim3((rows + DT - 1) / DT), dim3(DT), 0, s, pl, la);
     hipLaunchKernelGGL(dc_vectors_kernel, dim3(wave_blocks), dim3(DT), 0, s, pl, la);
     const int mode = gemm_get_mode();

@@ -164,15 +164,15 @@ static BidiagChoice bidiag_choice(int M, int N)
 // Q and P^T from the reflectors (GetUVt, upper_bidiagonalization.cpp:774-893), blocked WY.
 static hipError_t form_q_pt(ProbList pl, const int *idq, int nidq, const int *idp, int nidp, const int *bq, int nbq,
                             const int *bp, int nbp, const int *sq, int nsq, const int *sp, int nsp, const int *tpair,
-                            int ntp, hipStream_t s)
+                            int ntp, hipStream_t s, int sw = 64)
 {
     hipError_t e;
     if ((e = launch_identity(pl, idq, nidq, 0, s)) != hipSuccess) return e;
     if ((e = launch_identity(pl, idp, nidp, 1, s)) != hipSuccess) return e;
     if ((e = launch_tfactor(pl, bq, nbq, 0, s)) != hipSuccess) return e;
     if ((e = launch_tfactor(pl, bp, nbp, 1, s)) != hipSuccess) return e;
-    if ((e = launch_wy_apply(pl, sq, nsq, 0, s)) != hipSuccess) return e;
-    if ((e = launch_wy_apply(pl, sp, nsp, 1, s)) != hipSuccess) return e;
+    if ((e = launch_wy_apply(pl, sq, nsq, 0, s, sw)) != hipSuccess) return e;
+    if ((e = launch_wy_apply(pl, sp, nsp, 1, s, sw)) != hipSuccess) return e;
     return launch_transpose_vt(pl, tpair, ntp, s);
 }
 
@@ -202,8 +202,10 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     if (err != hipSuccess) return (int)err;
     tmark(s, 2, tm);
     const int nt = cdiv(N, 32);
+    // a single matrix: 16-column strips unless 64-column ones already give every CU a strip
+    const int sw = cdiv(M, 64) >= 512 ? 64 : 16;
     err = form_q_pt(pl, nullptr, M, nullptr, N, nullptr, cdiv(N, 32), nullptr, cdiv(std::max(N - 1, 0), 32), nullptr,
-                    cdiv(M, 64), nullptr, cdiv(N, 64), nullptr, nt * (nt + 1) / 2, s);
+                    cdiv(M, sw), nullptr, cdiv(N, sw), nullptr, nt * (nt + 1) / 2, s, sw);
     tmark(s, 3, tm);
     return (int)err;
 }

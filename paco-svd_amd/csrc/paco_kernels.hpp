@@ -58,7 +58,8 @@ size_t bidiag_wg_lds_limit();
 constexpr int BIDIAG_WG_MAXN = 512; // widest matrix of the one-workgroup kernel
 hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s, int nt = 256);
 hipError_t launch_tfactor(ProbList pl, const int *blk_off, int nblocks, int side, hipStream_t s);
-hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int side, hipStream_t s);
+// strips of sw columns (64 or 16); strip_off counts strips of that width
+hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int side, hipStream_t s, int sw = 64);
 hipError_t launch_identity(ProbList pl, const int *row_off, int nrows, int side, hipStream_t s);
 hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hipStream_t s);
 
