@@ -196,11 +196,65 @@ __global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
     if (cb == 0 && tid == 0) W.ssp[rb] = ss;
     // ---- GEMV1 partials: g1p[rb][col] = sum_{row > i} wc[row] A[row][col], col > i
     const int rs = max(r0, i + 1);
-    for (int col = max(c0, i + 1) + tid; col < c1; col += GBT) {
-        float acc = 0.f;
+    if ((N & 3) == 0) {
+        // float4 column groups x row groups, 8 loads in flight per thread, LDS reduction
+        __shared__ float4 red4[GBT];
+        const int ca = max(c0, i + 1) & ~3, ncg = (c1 - ca + 3) >> 2;
+        int cg = 1;
+        while (cg < ncg && cg < GBT) cg <<= 1;
+        const int RG = GBT / cg, g = tid / cg, c = tid % cg;
+        for (int cgb = 0; cgb < ncg; cgb += cg) {
+            const int col = ca + 4 * (cgb + c);
+            float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (cgb + c < ncg) {
+                int row = rs + g;
+                for (; row + 7 * RG < r1; row += 8 * RG) {
+                    float4 v[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) v[k] = *reinterpret_cast<const float4 *>(A + (size_t)(row + k * RG) * N + col);
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const float w = wc[row + k * RG - r0];
+                        acc.x += w * v[k].x;
+                        acc.y += w * v[k].y;
+                        acc.z += w * v[k].z;
+                        acc.w += w * v[k].w;
+                    }
+                }
+                for (; row < r1; row += RG) {
+                    const float4 v = *reinterpret_cast<const float4 *>(A + (size_t)row * N + col);
+                    const float w = wc[row - r0];
+                    acc.x += w * v.x;
+                    acc.y += w * v.y;
+                    acc.z += w * v.z;
+                    acc.w += w * v.w;
+                }
+            }
+            __syncthreads();
+            red4[tid] = acc;
+            __syncthreads();
+            if (g == 0 && cgb + c < ncg) {
+                float4 t = red4[c];
+                for (int q = 1; q < RG; ++q) {
+                    const float4 u = red4[q * cg + c];
+                    t.x += u.x;
+                    t.y += u.y;
+                    t.z += u.z;
+                    t.w += u.w;
+                }
+                const float tv[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) // own columns only (the aligned start may reach into the previous tile)
+                    if (col + e > i && col + e >= c0 && col + e < c1) W.g1p[(size_t)rb * N + col + e] = tv[e];
+            }
+        }
+    } else {
+        for (int col = max(c0, i + 1) + tid; col < c1; col += GBT) {
+            float acc = 0.f;
 #pragma unroll 8
-        for (int row = rs; row < r1; ++row) acc += wc[row - r0] * A[(size_t)row * N + col];
-        W.g1p[(size_t)rb * N + col] = acc;
+            for (int row = rs; row < r1; ++row) acc += wc[row - r0] * A[(size_t)row * N + col];
+            W.g1p[(size_t)rb * N + col] = acc;
+        }
     }
     // t1p[rb][t] = sum_{row > i} V[row][t] wc[row], t2p[rb][t] = sum X[row][t] wc[row], t < j;
     // 32 column lanes x 8 row groups, reduced through LDS
@@ -306,11 +360,28 @@ __global__ __launch_bounds__(GBT) void gb_row_kernel(PanelArgs a)
     if (rb == 0 && tid == 0) W.rsp[cb] = ss;
     // ---- GEMV2 partials: g2p[cb][row] = sum_{col > i+1} A[row][col] wl[col], row > i
     const int cs = max(c0, i + 2);
-    for (int row = max(r0, i + 1) + wv; row < r1; row += GBT / 64) {
-        float acc = 0.f;
-        for (int col = cs + lane; col < c1; col += 64) acc += A[(size_t)row * N + col] * wl[col - c0];
-        acc = wave_sum(acc);
-        if (lane == 0) W.g2p[(size_t)cb * M + row] = acc;
+    // eight rows per wavefront in flight; lanes over the tile's columns
+    for (int row = max(r0, i + 1) + 8 * wv; row < r1; row += 8 * (GBT / 64)) {
+        float acc[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] = 0.f;
+        for (int col = cs + lane; col < c1; col += 64) {
+            const float w = wl[col - c0];
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = (row + q < r1) ? A[(size_t)(row + q) * N + col] : 0.f;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[q] += v[q] * w;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] = wave_sum(acc[q]);
+        if (lane < 8 && row + lane < r1) {
+            float a = acc[0];
+#pragma unroll
+            for (int q = 1; q < 8; ++q)
+                if (lane == q) a = acc[q];
+            W.g2p[(size_t)cb * M + row + lane] = a;
+        }
     }
     // t3p[cb][t] = sum_{col > i+1} Y[col][t] w (t <= j), t4p[cb][t] = sum U[t][col] w (t < j);
     // 32 lanes over t x 8 column groups, reduced through LDS

@@ -123,7 +123,7 @@ def test_entry_points_shapes(torch_dev, shape):
     assert m["resid_ratio"] <= RESID_TOL
 
 
-@pytest.mark.parametrize("shape", [(1100, 1000), (2048, 700)])
+@pytest.mark.parametrize("shape", [(1100, 1000), (2048, 700), (1200, 1023)])
 def test_grid_bidiag_vs_oracle(torch_dev, shape):
     """Large matrices take the chip-wide panel bidiagonalization (M*N >= 2^20)."""
     torch = torch_dev
