@@ -352,9 +352,17 @@ def main():
     bb = sum(bidiag_bytes(int(m), int(n)) for m, n in zip(Ms, Ns))
     t_bid = ph["bidiag"] / 1000.0
     achieved = bb / t_bid / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_bidiag_batched_c4.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_launch")
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="bidiag_wg_kernel",
-                algorithmic_bytes_per_launch=bb, launch_ms=round(ph["bidiag"], 3))
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel="bidiag_wg_kernel",
+                algorithmic_bytes_per_launch=bb, launch_ms=round(ph["bidiag"], 3),
+                traffic_source="profiles/traffic_bidiag_batched_c4.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                               "calibrated with the copy kernel)" if traffic else None)
     flops = sum(svd_flops(int(m), int(n)) for m, n in zip(Ms, Ns))
 
     cpu = None
