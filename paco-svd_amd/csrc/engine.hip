@@ -155,7 +155,10 @@ static BidiagChoice bidiag_choice(int M, int N)
         return c;
     }
     c.nt = tuned_bidiag_threads(M, N);
-    c.lds = env ? bidiag_wg_lds_bytes(M, N, c.nt) : lim;
+    // PACO_BIDIAG_LDS caps the LDS of the streamed variant (tests of the
+    // tier-independence of the results)
+    const char *envl = getenv("PACO_BIDIAG_LDS");
+    c.lds = envl ? std::min(lim, (size_t)strtoull(envl, nullptr, 10)) : lim;
     if (c.lds < bidiag_wg_lds_bytes(M, N, c.nt)) c.lds = bidiag_wg_lds_bytes(M, N, c.nt);
     c.resident = false;
     return c;
@@ -332,8 +335,11 @@ struct BatchPlan {
     size_t scratch_bytes = 0;
     int count = 0;
     // offsets into blob (bytes)
-    size_t o_probs = 0, o_src = 0, o_order = 0, o_order_res = 0, o_info = 0;
-    int n_glb = 0, n_res = 0, nt_glb = 256;
+    size_t o_probs = 0, o_src = 0, o_order_res = 0, o_info = 0;
+    // streamed bidiagonalization: one launch per workgroup size (256, 512, 1024)
+    size_t o_order_nt[3] = {0, 0, 0}, lds_nt[3] = {0, 0, 0};
+    int n_nt[3] = {0, 0, 0};
+    int n_res = 0;
     size_t lds_res = 0;
     size_t o_rowin = 0, o_idq = 0, o_idp = 0, o_bq = 0, o_bp = 0, o_sq = 0, o_sp = 0, o_tp = 0;
     size_t o_leaf = 0, o_tu = 0, o_tv = 0, o_ru = 0, o_rv = 0;
@@ -342,7 +348,6 @@ struct BatchPlan {
     int ntu = 0, ntv = 0, nru = 0, nrv = 0;
     std::vector<int> ntask, maxn;
     std::vector<size_t> state_off; // per round, in MergeState units from the states base
-    size_t lds_bidiag = 0;
     bool fits = true;
 };
 
@@ -401,24 +406,25 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         return svd_cost(jobs[x].M, jobs[x].N) > svd_cost(jobs[y].M, jobs[y].N);
     });
-    std::vector<int> og, orr;
-    bp.lds_bidiag = 0;
+    // a matrix's launch shape (and so its arithmetic) depends on its shape only
+    std::vector<int> og[3], orr;
     bp.lds_res = 0;
-    bp.nt_glb = 256;
     for (int i : order) {
         const BidiagChoice c = bidiag_choice(jobs[i].M, jobs[i].N);
         if (c.resident) {
             orr.push_back(i);
             bp.lds_res = std::max(bp.lds_res, c.lds);
         } else {
-            og.push_back(i);
-            bp.lds_bidiag = std::max(bp.lds_bidiag, c.lds);
-            bp.nt_glb = std::max(bp.nt_glb, c.nt);
+            const int g = c.nt >= 1024 ? 2 : c.nt >= 512 ? 1 : 0;
+            og[g].push_back(i);
+            bp.lds_nt[g] = std::max(bp.lds_nt[g], c.lds);
         }
     }
-    bp.n_glb = (int)og.size();
+    for (int g = 0; g < 3; ++g) {
+        bp.n_nt[g] = (int)og[g].size();
+        bp.o_order_nt[g] = blob_put(b, og[g]);
+    }
     bp.n_res = (int)orr.size();
-    bp.o_order = blob_put(b, og);
     bp.o_order_res = blob_put(b, orr);
     bp.o_rowin = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nrowin));
     bp.o_idq = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nidq));
@@ -576,7 +582,9 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
             return e;
         if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
     }
-    if ((e = launch_bidiag_wg(pl, I(bp.o_order), bp.n_glb, bp.lds_bidiag, s, bp.nt_glb)) != hipSuccess) return e;
+    for (int g = 2; g >= 0; --g)
+        if ((e = launch_bidiag_wg(pl, I(bp.o_order_nt[g]), bp.n_nt[g], bp.lds_nt[g], s, 256 << g)) != hipSuccess)
+            return e;
     if (bp.n_res > 0 && (e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
     tmark(s, 2, tm);
     if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,

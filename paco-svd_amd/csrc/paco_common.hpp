@@ -96,12 +96,64 @@ __host__ __device__ inline Arena make_arena(uint32_t *idx, int c0, int n)
     return A;
 }
 
+// A descriptor read through a pointer that the kernel could also write is loaded with
+// vector loads: the compiler then treats its fields as per-lane values.  Every lane
+// reads the same descriptor, so the first lane's copy is taken into scalar registers.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <typename T>
+__device__ inline T *uni(T *p)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return reinterpret_cast<T *>(((uint64_t)hi << 32) | lo);
+}
+__device__ inline ProbDesc uniform_desc(const ProbDesc &P)
+{
+    ProbDesc U;
+    U.M = uni(P.M);
+    U.N = uni(P.N);
+    U.a = uni(P.a);
+    U.u = uni(P.u);
+    U.vt = uni(P.vt);
+    U.d = uni(P.d);
+    U.e = uni(P.e);
+    U.tauq = uni(P.tauq);
+    U.taup = uni(P.taup);
+    U.qt = uni(P.qt);
+    U.wt = uni(P.wt);
+    U.idx = uni(P.idx);
+    U.vmt = uni(P.vmt);
+    U.tq = uni(P.tq);
+    U.tp = uni(P.tp);
+    U.info = uni(P.info);
+    U.ld = uni(P.ld);
+    return U;
+}
+
 __device__ inline float wave_sum(float v)
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// Sum over the 64 lanes with DPP moves inside each row of 16 lanes (quad swaps, then
+// the half-row and row mirrors) and four lane reads across the rows: no LDS round trip.
+// Every lane returns the same value; the order of additions is fixed.
+#define PACO_DPP(v, ctrl) __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, false))
+__device__ inline float wave_sum_dpp(float v)
+{
+    v += PACO_DPP(v, 0xB1);  // quad_perm [1,0,3,2]
+    v += PACO_DPP(v, 0x4E);  // quad_perm [2,3,0,1]
+    v += PACO_DPP(v, 0x141); // row_half_mirror
+    v += PACO_DPP(v, 0x140); // row_mirror
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return (r0 + r1) + (r2 + r3);
+}
+#undef PACO_DPP
 __device__ inline double wave_sum(double v)
 {
 #pragma unroll

@@ -258,6 +258,40 @@ def test_bidiag_variants_bitwise_equal(torch_dev):
             assert torch.equal(x, y), i
 
 
+@pytest.mark.parametrize("threads", ["512", "1024"])
+def test_bidiag_tiers_bitwise_equal(torch_dev, threads):
+    """Rows streamed from global memory or cached in LDS take the same arithmetic in the
+    same order: every split of a matrix between the two gives the same bits."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(5)
+    shapes = [(512, 500), (300, 250), (480, 130), (260, 259), (200, 64), (100, 70), (511, 300), (64, 33),
+              (450, 420), (390, 380), (170, 129), (40, 9)]
+    mats = [rng.standard_normal(sh).astype(np.float32) for sh in shapes]
+    say("tiers", threads, shapes)
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    # LDS bytes of the streamed launch (None: all of it)
+    variants = [None, "0", "65536"]
+    results = []
+    for lds in variants:
+        os.environ["PACO_BIDIAG_THREADS"] = threads
+        os.environ["PACO_LDS_RESIDENT_MAX"] = "0"
+        if lds is not None:
+            os.environ["PACO_BIDIAG_LDS"] = lds
+        try:
+            results.append(pacosvd.svd_batched(As))
+            torch.cuda.synchronize()
+        finally:
+            for k in ("PACO_BIDIAG_THREADS", "PACO_LDS_RESIDENT_MAX", "PACO_BIDIAG_LDS"):
+                os.environ.pop(k, None)
+    for i, A in enumerate(mats):
+        m = measures(A, *(np_of(x) for x in results[0][i]))
+        assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL, (i, m)
+        for v in range(1, len(variants)):
+            for x, y in zip(results[0][i], results[v][i]):
+                assert torch.equal(x, y), (i, variants[v])
+
+
 @pytest.mark.parametrize("threads", ["256", "512", "1024"])
 def test_bidiag_workgroup_sizes(torch_dev, threads):
     """Matrices too large for LDS run with 256, 512 or 1024 threads (rows partly cached)."""
