@@ -11,30 +11,14 @@
 //    then one launch in which each workgroup owns a column strip of the target and
 //    applies all blocks to it (strips are independent), replacing the reference's
 //    N rank-1 passes with barriers.
+#include <algorithm>
+
 #include "paco_common.hpp"
 #include "paco_kernels.hpp"
 
 namespace paco {
 
 #define BT 256
-
-// Reference sign convention (upper_bidiagonalization.cpp:182-218): beta = +mu unless
-// sigma == 0 (then tau = 0, beta = x1 signed).  inv = 1 / v1.
-__device__ inline void house_params(float x1, float sigma, float *beta, float *tau, float *inv)
-{
-    if (sigma == 0.f) {
-        *tau = 0.f;
-        *beta = x1;
-        *inv = 1.f;
-        return;
-    }
-    float mu = sqrtf(x1 * x1 + sigma);
-    float v1 = (x1 <= 0.f) ? x1 - mu : -sigma / (x1 + mu);
-    float v1sq = v1 * v1;
-    *tau = 2.f * v1sq / (sigma + v1sq);
-    *inv = 1.f / v1;
-    *beta = mu;
-}
 
 // ---------------------------------------------------------------------------
 // One workgroup of NT threads per matrix (N <= 512).  Each column costs one streaming
@@ -45,20 +29,21 @@ __device__ inline void house_params(float x1, float sigma, float *beta, float *t
 //   w_{i+1} = tau (A'[i+1, :] + (1/v_1) sum_{r > i+1} c_r A'[r, :]).
 // Rows [split, M) are kept in LDS (all rows when the matrix fits), the rest is
 // streamed; every reduction runs in an order that does not depend on the split.
-// LDS: [cached rows, leading dimension N|1] cbuf[M] wbuf[N] ubuf[N] rowbuf[N] pw[N]
-//      pwr[NW][N] red[NT] (+ scalars).
+// LDS: FP64 partial sums [NW + 2], [cached rows, leading dimension N|1] cbuf[M] wbuf[N]
+//      ubuf[N] rowbuf[N] pw[N] pwr[NW][N] (+ spare).
 // ---------------------------------------------------------------------------
+// sums of squares of the Householder vectors are taken in FP64 (see house_params_d)
 template <int NT>
-__device__ inline float block_sum_t(float v, float *red)
+__device__ inline double block_sum_d(double v, double *dred)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     v = wave_sum(v);
     __syncthreads();
-    if (lane == 0) red[wv] = v;
+    if (lane == 0) dred[wv] = v;
     __syncthreads();
-    float r = 0.f;
+    double r = 0.0;
 #pragma unroll
-    for (int w = 0; w < NT / 64; ++w) r += red[w];
+    for (int w = 0; w < NT / 64; ++w) r += dred[w];
     return r;
 }
 
@@ -74,9 +59,10 @@ struct RowStore {
 
 constexpr int WG_MAXN = 512; // 8 chunks of 64 columns per lane
 
+// (the first 2 (nt / 64 + 2) floats hold FP64 partial sums)
 __host__ __device__ inline size_t bidiag_vec_floats(int M, int N, int nt)
 {
-    return (size_t)M + 5 * (size_t)N + (size_t)(nt / 64) * N + nt + 8;
+    return (size_t)M + 5 * (size_t)N + (size_t)(nt / 64) * N + nt + 8 + 2 * (size_t)(nt / 64 + 2);
 }
 
 // rows of A that fit in lds_bytes of dynamic LDS next to the vectors
@@ -93,13 +79,14 @@ __host__ __device__ inline int bidiag_lds_rows(int M, int N, int nt, size_t lds_
 // then, for column k = i + 1: row k -> rowbuf; rows r > k: cbuf[r] = A'[r, k],
 // sumsq partial, pw partial (columns > k).  Wave partials of pw go to pwr[wave].
 template <int NT>
-__device__ inline float fused_pass(const RowStore &S, int M, int N, int i, float tau, float inv, float tau2,
-                                   float *cbuf, const float *wbuf, const float *ubuf, float *rowbuf, float *pwr)
+__device__ inline double fused_pass(const RowStore &S, int M, int N, int i, float tau, float inv, float tau2,
+                                    float *cbuf, const float *wbuf, const float *ubuf, float *rowbuf, float *pwr)
 {
     constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, k = i + 1;
     const bool upd = i >= 0;
-    float pwp[8], ss = 0.f;
+    float pwp[8];
+    double ss = 0.0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) pwp[c] = 0.f;
     for (int r = k + 4 * wv; r < M; r += 4 * NW) {
@@ -160,7 +147,7 @@ __device__ inline float fused_pass(const RowStore &S, int M, int N, int i, float
             } else {
                 if (lane == 0) {
                     cbuf[rq] = cv;
-                    ss += cv * cv;
+                    ss += (double)cv * cv;
                 }
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
@@ -185,7 +172,9 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
     const int pid = order ? order[blockIdx.x] : blockIdx.x;
     const ProbDesc P = pl.get(pid);
     const int M = P.M, N = P.N, ldg = P.ld ? P.ld : P.N;
-    extern __shared__ float sm[];
+    extern __shared__ float smem[];
+    double *dred = reinterpret_cast<double *>(smem); // [NW] partials, [NW] sigma
+    float *sm = smem + 2 * (NW + 2);
     const int nl = bidiag_lds_rows(M, N, NT, lds_bytes);
     RowStore S;
     S.G = P.a;
@@ -194,17 +183,17 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
     S.ldl = N | 1;
     S.split = M - nl;
     float *cbuf = sm + (size_t)nl * S.ldl, *wbuf = cbuf + M, *ubuf = wbuf + N, *rowbuf = ubuf + N, *pw = rowbuf + N;
-    float *pwr = pw + N, *red = pwr + (size_t)NW * N, *scal = red + NT;
+    float *pwr = pw + N;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int r = S.split + wv; r < M; r += NW)
         for (int c = lane; c < N; c += 64) S.L[(size_t)(r - S.split) * S.ldl + c] = P.a[(size_t)r * ldg + c];
     __syncthreads();
 
     // initial read-only pass: column 0 data
-    float ss = fused_pass<NT>(S, M, N, -1, 0.f, 1.f, 0.f, cbuf, wbuf, ubuf, rowbuf, pwr);
+    double ss = fused_pass<NT>(S, M, N, -1, 0.f, 1.f, 0.f, cbuf, wbuf, ubuf, rowbuf, pwr);
     for (int i = 0; i < N; ++i) {
         // ---- reduce the pass partials: sum c_r^2 and pw
-        if (lane == 0) red[wv] = ss;
+        if (lane == 0) dred[wv] = ss;
         __syncthreads();
         for (int j = i + 1 + tid; j < N; j += NT) {
             float a = 0.f;
@@ -213,15 +202,15 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
             pw[j] = a;
         }
         if (tid == 0) {
-            float sg = 0.f;
+            double sg = 0.0;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) sg += red[w];
-            scal[0] = sg;
+            for (int w = 0; w < NW; ++w) sg += dred[w];
+            dred[NW] = sg;
         }
         __syncthreads();
         // ---- column reflector of column i
         float tau, beta, inv;
-        house_params(rowbuf[i], scal[0], &beta, &tau, &inv);
+        house_params_d(rowbuf[i], dred[NW], &beta, &tau, &inv);
         float *rowi = S.row(i);
         if (tid == 0) {
             rowi[i] = beta;
@@ -233,17 +222,17 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
             break;
         }
         // ---- w and the updated row i; row reflector
-        float part = 0.f;
+        double part = 0.0;
         for (int j = i + 1 + tid; j < N; j += NT) {
             const float w = (tau != 0.f) ? tau * (rowbuf[j] + inv * pw[j]) : 0.f;
             wbuf[j] = w;
             const float y = rowbuf[j] - w;
             ubuf[j] = y;
-            if (j > i + 1) part += y * y;
+            if (j > i + 1) part += (double)y * y;
         }
-        const float sigma2 = block_sum_t<NT>(part, red);
+        const double sigma2 = block_sum_d<NT>(part, dred);
         float tau2, beta2, inv2;
-        house_params(ubuf[i + 1], sigma2, &beta2, &tau2, &inv2);
+        house_params_d(ubuf[i + 1], sigma2, &beta2, &tau2, &inv2);
         __syncthreads();
         for (int j = i + 1 + tid; j < N; j += NT) {
             if (j == i + 1) {
@@ -267,6 +256,188 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
     __syncthreads();
     for (int r = S.split + wv; r < M; r += NW)
         for (int c = lane; c < N; c += 64) P.a[(size_t)r * ldg + c] = S.L[(size_t)(r - S.split) * S.ldl + c];
+}
+
+// ---------------------------------------------------------------------------
+// One wavefront per matrix, the whole matrix in LDS (batched path, matrices whose rows
+// and one column vector fit the LDS of one workgroup).  With a single wavefront no
+// workgroup barrier is needed: row k, the reflector vectors w and u and the partial
+// sums of the next left GEMV live in registers (a lane holds the columns lane + 64 c,
+// c < 8), every reduction is a DPP wave sum, and the rows of the trailing matrix are
+// visited four at a time so that four reductions are in flight.  Per column k:
+//   (x1, sum c_r^2)          -> column reflector (beta, tau, 1/v1)
+//   w = tau (row_k + (1/v1) sum_r c_r A'[r, :]),  y = row_k - w -> row reflector u
+//   rows r > k: t = A'[r, :] - v_r w, A''[r, :] = t - tau2 (t.u) u, A''[r, k] = v_r,
+//   and column k + 1 gathered on the way (c_r, sum c_r^2, sum c_r A''[r, :]).
+// LDS: rows [M][N|1], cbuf[M].
+// ---------------------------------------------------------------------------
+constexpr int WV_R = 4;
+
+__host__ __device__ inline size_t bidiag_wave_lds_bytes(int M, int N)
+{
+    return sizeof(float) * ((size_t)M * (size_t)(N | 1) + (size_t)M);
+}
+
+// value of the lane that holds column j (uniform j) of a row held as x[c]
+template <int C>
+__device__ inline float col_of(const float (&x)[C], int j)
+{
+    float v = 0.f;
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        if (c == (j >> 6)) v = x[c];
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j & 63));
+}
+
+// C = ceil(N / 64) chunks of 64 columns per row
+template <int C>
+__device__ inline void bidiag_wave_body(const ProbDesc &P, float *sm)
+{
+    const int M = uni(P.M), N = uni(P.N), ldg = uni(P.ld ? P.ld : P.N);
+    const int lane = threadIdx.x & 63, ldl = N | 1;
+    float *A = P.a;
+    float *Ls = sm, *cbuf = sm + (size_t)M * ldl;
+    for (int r = 0; r < M; ++r)
+        for (int j = lane; j < N; j += 64) Ls[(size_t)r * ldl + j] = A[(size_t)r * ldg + j];
+    __builtin_amdgcn_wave_barrier();
+
+    // lane columns: only the last chunk can reach past column N - 1
+    const bool last_ok = lane + 64 * (C - 1) < N;
+    float rowk[C], pwp[C], w[C], u[C];
+    double ss = 0.0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        rowk[c] = 0.f;
+        pwp[c] = 0.f;
+        w[c] = 0.f;
+        u[c] = 0.f;
+    }
+    float tau = 0.f, inv = 1.f, tau2 = 0.f;
+    // k = -1: gathering pass of column 0; k >= 0: rows > k take the reflectors of
+    // column k and column k + 1 is gathered
+    for (int k = -1;;) {
+        const int kn = k + 1;
+        ss = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) pwp[c] = 0.f;
+        const int c0 = k < 0 ? 0 : (k >> 6); // chunks left of c0 are final
+        for (int r0 = kn; r0 < M; r0 += WV_R) {
+            float x[WV_R][C];
+#pragma unroll
+            for (int q = 0; q < WV_R; ++q) {
+                const int r = r0 + q < M ? r0 + q : M - 1; // a row past the end re-reads row M - 1
+                const float *row = Ls + (size_t)r * ldl;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    float v = 0.f;
+                    if (c >= c0 && (c < C - 1 || last_ok)) v = row[lane + 64 * c];
+                    x[q][c] = v;
+                }
+            }
+            if (k >= 0) {
+                float acc[WV_R], colv[WV_R];
+#pragma unroll
+                for (int q = 0; q < WV_R; ++q) {
+                    const float cr = cbuf[r0 + q < M ? r0 + q : M - 1];
+                    const float vr = (tau != 0.f) ? cr * inv : 0.f;
+                    colv[q] = (tau != 0.f) ? vr : cr;
+                    float a = 0.f;
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        x[q][c] -= vr * w[c];
+                        a += x[q][c] * u[c];
+                    }
+                    acc[q] = a;
+                }
+#pragma unroll
+                for (int q = 0; q < WV_R; ++q) acc[q] = wave_sum_dpp(acc[q]) * tau2;
+                const bool lk = lane == (k & 63);
+#pragma unroll
+                for (int q = 0; q < WV_R; ++q)
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        x[q][c] -= acc[q] * u[c];
+                        if (c == (k >> 6) && lk) x[q][c] = colv[q];
+                    }
+#pragma unroll
+                for (int q = 0; q < WV_R; ++q) {
+                    if (r0 + q >= M) break;
+                    float *row = Ls + (size_t)(r0 + q) * ldl;
+#pragma unroll
+                    for (int c = 0; c < C; ++c)
+                        if (c >= c0 && (c < C - 1 || last_ok)) row[lane + 64 * c] = x[q][c];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < WV_R; ++q) {
+                const int r = r0 + q;
+                if (r >= M) break;
+                const float cv = col_of<C>(x[q], kn);
+                if (r == kn) {
+#pragma unroll
+                    for (int c = 0; c < C; ++c) rowk[c] = x[q][c];
+                } else {
+                    if (lane == 0) cbuf[r] = cv;
+                    ss += (double)cv * cv;
+#pragma unroll
+                    for (int c = 0; c < C; ++c) pwp[c] += (lane + 64 * c > kn) ? cv * x[q][c] : 0.f;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        k = kn;
+        // ---- column reflector of column k (row k in rowk, column entries in cbuf)
+        float beta;
+        house_params_d(col_of<C>(rowk, k), ss, &beta, &tau, &inv);
+        if (lane == 0) {
+            P.d[k] = beta;
+            P.tauq[k] = tau;
+        }
+        if (k == N - 1) {
+            if (lane == 0) Ls[(size_t)k * ldl + k] = beta;
+            for (int r = k + 1 + lane; r < M; r += 64)
+                Ls[(size_t)r * ldl + k] = (tau != 0.f) ? cbuf[r] * inv : cbuf[r];
+            break;
+        }
+        // ---- w, the updated row k and its row reflector
+        double part = 0.0;
+        float y[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int j = lane + 64 * c;
+            const bool on = j > k && (c < C - 1 || last_ok);
+            w[c] = (on && tau != 0.f) ? tau * (rowk[c] + inv * pwp[c]) : 0.f;
+            y[c] = on ? rowk[c] - w[c] : 0.f;
+            part += (j > k + 1) ? (double)y[c] * y[c] : 0.0;
+        }
+        const double sigma2 = wave_sum(part);
+        float beta2, inv2;
+        house_params_d(col_of<C>(y, k + 1), sigma2, &beta2, &tau2, &inv2);
+        float *rowkp = Ls + (size_t)k * ldl;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int j = lane + 64 * c;
+            const bool in = c < C - 1 || last_ok;
+            u[c] = j == k + 1 ? 1.f : (j > k + 1 && in) ? ((tau2 != 0.f) ? y[c] * inv2 : y[c]) : 0.f;
+            if (j >= k && in) rowkp[j] = j == k ? beta : j == k + 1 ? beta2 : u[c];
+        }
+        if (lane == 0) {
+            P.e[k] = beta2;
+            P.taup[k] = tau2;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int r = 0; r < M; ++r)
+        for (int j = lane; j < N; j += 64) A[(size_t)r * ldg + j] = Ls[(size_t)r * ldl + j];
+}
+
+template <int C>
+__global__ __launch_bounds__(64) void bidiag_wave_kernel(ProbList pl, const int *order)
+{
+    const int pid = order ? order[blockIdx.x] : blockIdx.x;
+    const ProbDesc P = pl.get(pid);
+    extern __shared__ float sm[];
+    bidiag_wave_body<C>(P, sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -499,6 +670,37 @@ hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds
         hipLaunchKernelGGL(bidiag_wg_kernel<512>, dim3(count), dim3(512), lds, s, pl, order, lb);
     else
         hipLaunchKernelGGL(bidiag_wg_kernel<256>, dim3(count), dim3(256), lds, s, pl, order, lb);
+    return hipGetLastError();
+}
+
+size_t bidiag_wave_lds(int M, int N) { return bidiag_wave_lds_bytes(M, N); }
+using WaveKernel = void (*)(ProbList, const int *);
+// a matrix fits the LDS of one workgroup only if N <= 200 (N <= M): C <= 4
+static WaveKernel wave_kernel_of(int C)
+{
+    switch (C) {
+    case 1: return bidiag_wave_kernel<1>;
+    case 2: return bidiag_wave_kernel<2>;
+    case 3: return bidiag_wave_kernel<3>;
+    default: return bidiag_wave_kernel<4>;
+    }
+}
+size_t bidiag_wave_lds_limit()
+{
+    static size_t lim = 0;
+    if (!lim) {
+        lim = (size_t)-1;
+        for (int c = 1; c <= BIDIAG_WAVE_MAXC; ++c)
+            lim = std::min(lim, allow_dynamic_lds((const void *)wave_kernel_of(c)));
+    }
+    return lim;
+}
+// matrices of one launch share N's chunk count C = ceil(N / 64)
+hipError_t launch_bidiag_wave(ProbList pl, const int *order, int count, int C, size_t lds, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    if (C < 1 || C > BIDIAG_WAVE_MAXC || lds > bidiag_wave_lds_limit()) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(wave_kernel_of(C), dim3(count), dim3(64), lds, s, pl, order);
     return hipGetLastError();
 }
 

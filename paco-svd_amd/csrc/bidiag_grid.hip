@@ -26,7 +26,8 @@ constexpr int GR = 16, GC = 16;
 constexpr int SWITCH = 384;
 
 struct GridWs {
-    float *X, *Y, *wcol, *wrow, *g1p, *g2p, *ssp, *rsp, *t1p, *t2p, *t3p, *t4p;
+    float *X, *Y, *wcol, *wrow, *g1p, *g2p, *t1p, *t2p, *t3p, *t4p;
+    double *ssp, *rsp; // FP64 partial sums of squares (see house_params_d)
 };
 
 static size_t carve(int M, int N, char *base, GridWs *W)
@@ -44,8 +45,8 @@ static size_t carve(int M, int N, char *base, GridWs *W)
     w.wrow = take(N);
     w.g1p = take((size_t)GR * N);
     w.g2p = take((size_t)GC * M);
-    w.ssp = take(GR);
-    w.rsp = take(GC);
+    w.ssp = reinterpret_cast<double *>(take(2 * GR));
+    w.rsp = reinterpret_cast<double *>(take(2 * GC));
     w.t1p = take(GR * NB);
     w.t2p = take(GR * NB);
     w.t3p = take(GC * NB);
@@ -75,30 +76,14 @@ __device__ inline float Uel(const float *A, int N, int p, int t, int col)
     return col > u ? A[(size_t)(p + t) * N + col] : (col == u ? 1.f : 0.f);
 }
 
-__device__ inline void hparams(float x1, float sigma, float &beta, float &tau, float &inv)
-{
-    if (sigma == 0.f) {
-        tau = 0.f;
-        beta = x1;
-        inv = 1.f;
-        return;
-    }
-    const float mu = sqrtf(x1 * x1 + sigma);
-    const float v1 = (x1 <= 0.f) ? x1 - mu : -sigma / (x1 + mu);
-    const float v1sq = v1 * v1;
-    tau = 2.f * v1sq / (sigma + v1sq);
-    inv = 1.f / v1;
-    beta = mu;
-}
-
-__device__ inline float gb_block_sum(float v, float *red)
+__device__ inline double gb_block_sum(double v, double *red)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     v = wave_sum(v);
     __syncthreads();
     if (lane == 0) red[wv] = v;
     __syncthreads();
-    float r = 0.f;
+    double r = 0.0;
 #pragma unroll
     for (int w = 0; w < GBT / 64; ++w) r += red[w];
     return r;
@@ -116,7 +101,8 @@ __global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
     const int c0 = p + cb * a.tw, c1 = min(N, c0 + a.tw);
     extern __shared__ float sm[];
     float *xs = sm, *wc = sm + a.th;
-    __shared__ float t3s[NB], t4s[NB], ys[NB], us[NB], red[GBT / 64];
+    __shared__ float t3s[NB], t4s[NB], ys[NB], us[NB];
+    __shared__ double red[GBT / 64];
     __shared__ float s_tau2, s_inv2, s_beta2;
     const int tid = threadIdx.x;
 
@@ -124,10 +110,10 @@ __global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
         // ---- finish the row reflector and x of column ip = i - 1
         const int ip = i - 1, jp = j - 1;
         if (tid == 0) {
-            float sigma2 = 0.f;
+            double sigma2 = 0.0;
             for (int q = 0; q < GC; ++q) sigma2 += W.rsp[q];
             float beta2, tau2, inv2;
-            hparams(W.wrow[ip + 1], sigma2, beta2, tau2, inv2);
+            house_params_d(W.wrow[ip + 1], sigma2, &beta2, &tau2, &inv2);
             s_tau2 = tau2;
             s_inv2 = inv2;
             s_beta2 = beta2;
@@ -180,7 +166,7 @@ __global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
         us[tid] = (tid < j) ? Uel(A, N, p, tid, i) : 0.f;
     }
     __syncthreads();
-    float part = 0.f;
+    double part = 0.0;
     for (int row = max(r0, i) + tid; row < r1; row += GBT) {
         float val = A[(size_t)row * N + i];
 #pragma unroll 8
@@ -190,9 +176,9 @@ __global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
         }
         wc[row - r0] = val;
         if (cb == 0) W.wcol[row] = val;
-        if (row > i) part += val * val;
+        if (row > i) part += (double)val * val;
     }
-    const float ss = gb_block_sum(part, red);
+    const double ss = gb_block_sum(part, red);
     if (cb == 0 && tid == 0) W.ssp[rb] = ss;
     // ---- GEMV1 partials: g1p[rb][col] = sum_{row > i} wc[row] A[row][col], col > i
     const int rs = max(r0, i + 1);
@@ -298,16 +284,17 @@ __global__ __launch_bounds__(GBT) void gb_row_kernel(PanelArgs a)
     const int c0 = p + cb * a.tw, c1 = min(N, c0 + a.tw);
     extern __shared__ float sm[];
     float *yl = sm, *wl = sm + a.tw;
-    __shared__ float t1s[NB], t2s[NB], xi[NB], vi[NB], red[GBT / 64];
+    __shared__ float t1s[NB], t2s[NB], xi[NB], vi[NB];
+    __shared__ double red[GBT / 64];
     __shared__ float s_tau, s_inv, s_beta;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
     // ---- column reflector of column i
     if (tid == 0) {
-        float sigma = 0.f;
+        double sigma = 0.0;
         for (int q = 0; q < GR; ++q) sigma += W.ssp[q];
         float beta, tau, inv;
-        hparams(W.wcol[i], sigma, beta, tau, inv);
+        house_params_d(W.wcol[i], sigma, &beta, &tau, &inv);
         s_tau = tau;
         s_inv = inv;
         s_beta = beta;
@@ -339,7 +326,7 @@ __global__ __launch_bounds__(GBT) void gb_row_kernel(PanelArgs a)
     }
     __syncthreads();
     // ---- y for own columns, row-i update
-    float part = 0.f;
+    double part = 0.0;
     for (int col = max(c0, i + 1) + tid; col < c1; col += GBT) {
         float g1 = 0.f;
         for (int q = 0; q < GR; ++q) g1 += W.g1p[(size_t)q * N + col];
@@ -354,9 +341,9 @@ __global__ __launch_bounds__(GBT) void gb_row_kernel(PanelArgs a)
         for (int t = 0; t < j; ++t) w -= W.Y[(size_t)col * NB + t] * vi[t] + Uel(A, N, p, t, col) * xi[t];
         wl[col - c0] = w;
         if (rb == 0) W.wrow[col] = w;
-        if (col > i + 1) part += w * w;
+        if (col > i + 1) part += (double)w * w;
     }
-    const float ss = gb_block_sum(part, red);
+    const double ss = gb_block_sum(part, red);
     if (rb == 0 && tid == 0) W.rsp[cb] = ss;
     // ---- GEMV2 partials: g2p[cb][row] = sum_{col > i+1} A[row][col] wl[col], row > i
     const int cs = max(c0, i + 2);

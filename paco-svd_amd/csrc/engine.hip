@@ -339,6 +339,12 @@ struct BatchPlan {
     // streamed bidiagonalization: one launch per workgroup size (256, 512, 1024)
     size_t o_order_nt[3] = {0, 0, 0}, lds_nt[3] = {0, 0, 0};
     int n_nt[3] = {0, 0, 0};
+    // one wavefront per LDS-resident matrix: one launch per LDS class (the dynamic LDS
+    // of a launch is its largest member's, which sets the occupancy of all)
+    // and per chunk count C = ceil(N / 64) (the kernel is specialised on it)
+    static constexpr int kWaveClasses = 4, kWaveGroups = kWaveClasses * BIDIAG_WAVE_MAXC;
+    size_t o_order_wave[kWaveGroups] = {}, lds_wave[kWaveGroups] = {};
+    int n_wave[kWaveGroups] = {};
     int n_res = 0;
     size_t lds_res = 0;
     size_t o_rowin = 0, o_idq = 0, o_idp = 0, o_bq = 0, o_bp = 0, o_sq = 0, o_sp = 0, o_tp = 0;
@@ -407,9 +413,21 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         return svd_cost(jobs[x].M, jobs[x].N) > svd_cost(jobs[y].M, jobs[y].N);
     });
     // a matrix's launch shape (and so its arithmetic) depends on its shape only
-    std::vector<int> og[3], orr;
+    std::vector<int> og[3], orr, ow[BatchPlan::kWaveGroups];
     bp.lds_res = 0;
+    const char *wenv = getenv("PACO_BIDIAG_WAVE");
+    const bool wave_on = !(wenv && wenv[0] == '0');
+    const size_t wave_lim = bidiag_wave_lds_limit();
     for (int i : order) {
+        const size_t wl = bidiag_wave_lds(jobs[i].M, jobs[i].N);
+        if (wave_on && jobs[i].N <= 64 * BIDIAG_WAVE_MAXC && wl <= wave_lim) {
+            int cl = 0;
+            while (cl < BatchPlan::kWaveClasses - 1 && wl > (wave_lim >> (BatchPlan::kWaveClasses - 1 - cl))) ++cl;
+            const int g = cl * BIDIAG_WAVE_MAXC + (cdiv(jobs[i].N, 64) - 1);
+            ow[g].push_back(i);
+            bp.lds_wave[g] = std::max(bp.lds_wave[g], wl);
+            continue;
+        }
         const BidiagChoice c = bidiag_choice(jobs[i].M, jobs[i].N);
         if (c.resident) {
             orr.push_back(i);
@@ -426,6 +444,10 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     }
     bp.n_res = (int)orr.size();
     bp.o_order_res = blob_put(b, orr);
+    for (int g = 0; g < BatchPlan::kWaveGroups; ++g) {
+        bp.n_wave[g] = (int)ow[g].size();
+        bp.o_order_wave[g] = blob_put(b, ow[g]);
+    }
     bp.o_rowin = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nrowin));
     bp.o_idq = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nidq));
     bp.o_idp = blob_put(b, prefix(count, [&](int i) { return jobs[i].N; }, bp.nidp));
@@ -574,10 +596,17 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     hipLaunchKernelGGL(copy_in_kernel, dim3(bp.nrowin), dim3(256), 0, s, pl,
                        reinterpret_cast<const float *const *>(dev + bp.o_src), I(bp.o_rowin));
     tmark(s, 1, tm);
-    // the two bidiagonalization variants run concurrently: resident on a forked stream
-    if (bp.n_res > 0) {
+    // the LDS-resident matrices (one wavefront each, or one workgroup) run on a forked
+    // stream, concurrently with the streamed ones
+    int n_side = bp.n_res;
+    for (int g = 0; g < BatchPlan::kWaveGroups; ++g) n_side += bp.n_wave[g];
+    if (n_side > 0) {
         if ((e = hipEventRecord(res.evf, s)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(res.side, res.evf, 0)) != hipSuccess) return e;
+        for (int g = BatchPlan::kWaveGroups - 1; g >= 0; --g)
+            if ((e = launch_bidiag_wave(pl, I(bp.o_order_wave[g]), bp.n_wave[g], g % BIDIAG_WAVE_MAXC + 1,
+                                        bp.lds_wave[g], res.side)) != hipSuccess)
+                return e;
         if ((e = launch_bidiag_wg(pl, I(bp.o_order_res), bp.n_res, bp.lds_res, res.side, 256)) != hipSuccess)
             return e;
         if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
@@ -585,7 +614,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     for (int g = 2; g >= 0; --g)
         if ((e = launch_bidiag_wg(pl, I(bp.o_order_nt[g]), bp.n_nt[g], bp.lds_nt[g], s, 256 << g)) != hipSuccess)
             return e;
-    if (bp.n_res > 0 && (e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
+    if (n_side > 0 && (e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
     tmark(s, 2, tm);
     if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
                        I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)

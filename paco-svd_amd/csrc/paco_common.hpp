@@ -167,6 +167,33 @@ __device__ inline double wave_max(double v)
     return v;
 }
 
+// Householder parameters of a vector [x1; c] with sigma = |c|^2, in the reference's sign
+// convention (upper_bidiagonalization.cpp:182-218): beta = +mu, v1 = x1 - mu (x1 <= 0) or
+// -sigma / (x1 + mu), tau = 2 v1^2 / (sigma + v1^2), inv = 1 / v1; sigma == 0 gives
+// tau = 0, beta = x1.  sigma is summed and the parameters are formed in FP64: squares of
+// FP32 entries below ~1e-19 underflow FP32 (denormal sigma, a few significant bits), which
+// would leave tau inconsistent with the stored vector c / v1 (H = I - tau v v^T no longer
+// orthogonal).  When tau rounds to 0 in FP32 or 1/v1 overflows it, the reflector is the
+// identity (tau = 0, beta = x1), so no stored vector entry can be infinite.
+__device__ inline void house_params_d(double x1, double sigma, float *beta, float *tau, float *inv)
+{
+    if (sigma > 0.0) {
+        const double mu = sqrt(x1 * x1 + sigma);
+        const double v1 = (x1 <= 0.0) ? x1 - mu : -sigma / (x1 + mu);
+        const double v1sq = v1 * v1;
+        const float t = (float)(2.0 * v1sq / (sigma + v1sq)), iv = (float)(1.0 / v1);
+        if (t != 0.f && fabsf(iv) <= 3.0e38f) {
+            *tau = t;
+            *inv = iv;
+            *beta = (float)mu;
+            return;
+        }
+    }
+    *tau = 0.f;
+    *beta = (float)x1;
+    *inv = 1.f;
+}
+
 // Binary search: index i of the group with offs[i] <= x < offs[i+1] (offs sorted).
 // A single group needs no offsets array (offs may be null).
 __host__ __device__ inline int find_task(const int *offs, int ntask, int x)

@@ -57,6 +57,11 @@ size_t bidiag_wg_lds_resident_bytes(int M, int N, int nt = 256); // whole matrix
 size_t bidiag_wg_lds_limit();
 constexpr int BIDIAG_WG_MAXN = 512; // widest matrix of the one-workgroup kernel
 hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s, int nt = 256);
+// one wavefront per matrix, whole matrix in LDS (batched path), N <= 64 * BIDIAG_WAVE_MAXC
+constexpr int BIDIAG_WAVE_MAXC = 4;
+size_t bidiag_wave_lds(int M, int N);
+size_t bidiag_wave_lds_limit();
+hipError_t launch_bidiag_wave(ProbList pl, const int *order, int count, int C, size_t lds, hipStream_t s);
 hipError_t launch_tfactor(ProbList pl, const int *blk_off, int nblocks, int side, hipStream_t s);
 // strips of sw columns (64 or 16); strip_off counts strips of that width
 hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int side, hipStream_t s, int sw = 64);
