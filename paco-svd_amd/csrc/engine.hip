@@ -48,8 +48,11 @@ __global__ __launch_bounds__(256) void copy_in_kernel(ProbList pl, const float *
 // single-problem workspace
 // ---------------------------------------------------------------------------
 struct WsLayout {
-    size_t vmt, tq, tp, info, states, grid, total;
+    size_t vmt, tq, tp, info, states, grid, qp, total;
 };
+
+// matrices that take the chip-wide bidiagonalization and the GEMM formation of Q, P
+static bool large_matrix(int M, int N) { return N > BIDIAG_WG_MAXN || (size_t)M * N >= (size_t)1024 * 1024; }
 
 static int total_merges(int N) { return leaf_count(N) - 1; }
 
@@ -69,6 +72,8 @@ static WsLayout ws_layout(int M, int N)
     o = align_up(o + sizeof(MergeState) * (size_t)std::max(total_merges(N), 1), 256);
     L.grid = o;
     o = align_up(o + bidiag_grid_ws_bytes(M, N), 256);
+    L.qp = o;
+    if (large_matrix(M, N)) o = align_up(o + qp_gemm_ws_bytes(M, N), 256);
     L.total = o;
     return L;
 }
@@ -97,7 +102,7 @@ static bool use_grid_bidiag(int M, int N)
     const char *env = getenv("PACO_BIDIAG");
     if (env && !strcmp(env, "wg") && N <= BIDIAG_WG_MAXN) return false;
     if (env && !strcmp(env, "grid")) return true;
-    return N > BIDIAG_WG_MAXN || (size_t)M * N >= (size_t)1024 * 1024;
+    return large_matrix(M, N);
 }
 
 static ProbDesc bidiag_desc(int M, int N, float *a, float *u, float *vt, float *d, float *e, float *tauq,
@@ -205,6 +210,18 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     if (err != hipSuccess) return (int)err;
     tmark(s, 2, tm);
     const int nt = cdiv(N, 32);
+    if (large_matrix(M, N)) {
+        // blocks of 32 reflectors applied to the whole trailing part of Q, P as GEMMs
+        if ((err = launch_identity(pl, nullptr, M, 0, s)) != hipSuccess) return (int)err;
+        if ((err = launch_identity(pl, nullptr, N, 1, s)) != hipSuccess) return (int)err;
+        if ((err = launch_tfactor(pl, nullptr, cdiv(N, 32), 0, s)) != hipSuccess) return (int)err;
+        if ((err = launch_tfactor(pl, nullptr, cdiv(std::max(N - 1, 0), 32), 1, s)) != hipSuccess) return (int)err;
+        if ((err = launch_qp_gemm(pl.single, 0, ws + L.qp, s)) != hipSuccess) return (int)err;
+        if ((err = launch_qp_gemm(pl.single, 1, ws + L.qp, s)) != hipSuccess) return (int)err;
+        err = launch_transpose_vt(pl, nullptr, nt * (nt + 1) / 2, s);
+        tmark(s, 3, tm);
+        return (int)err;
+    }
     // a single matrix: 16-column strips unless 64-column ones already give every CU a strip
     const int sw = cdiv(M, 64) >= 512 ? 64 : 16;
     err = form_q_pt(pl, nullptr, M, nullptr, N, nullptr, cdiv(N, 32), nullptr, cdiv(std::max(N - 1, 0), 32), nullptr,

@@ -81,6 +81,12 @@ hipError_t launch_dc_backtransform(ProbList pl, const int *tile_off_u, int ntile
                                    int ntiles_v, const int *row_off_u, int nrows_u, const int *row_off_v,
                                    int nrows_v, hipStream_t s);
 
+// qp_gemm.hip: Q, P of one large matrix by blocked GEMMs (X already the identity, T factors
+// formed)
+constexpr int QP_MAX_SPLITS = 16;
+size_t qp_gemm_ws_bytes(int M, int N);
+hipError_t launch_qp_gemm(const ProbDesc &P, int side, char *ws, hipStream_t st);
+
 // gemm.hip
 hipError_t launch_gemm_grouped(const GemmDesc *descs, int ndesc, int ntiles, hipStream_t s);
 hipError_t launch_copy2d(const CopyDesc *descs, int ndesc, int nrows, hipStream_t s);

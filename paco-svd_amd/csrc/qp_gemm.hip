@@ -1,0 +1,170 @@
+// Q and P of one large matrix from its reflectors on the matrix cores (GetUVt,
+// upper_bidiagonalization.cpp:774-893).  X = I, then for the blocks of 32 reflectors in
+// descending order (rows and columns R0.. of X, Lr = L - R0 of each):
+//   V, VT = V T           (qp_extract_kernel: V with its unit diagonal explicit)
+//   W_s = V^T X[R0:, R0:] (split over the row range: S partial products in parallel)
+//   W = sum_s W_s          (qp_sum_kernel)
+//   X[R0:, R0:] -= VT W    (one GEMM, K = 32)
+// The columns of X left of R0 are identity columns that the block does not touch.  The
+// GEMMs are the 64 x 64 tiles of gemm_tile.hpp (precision setting honoured); their
+// descriptors travel as kernel arguments, so nothing is uploaded.
+#include <cstring>
+
+#include "gemm_tile.hpp"
+#include "paco_kernels.hpp"
+
+namespace paco {
+
+static inline int cdiv_host(int a, int b) { return (a + b - 1) / b; }
+
+// reflector i of side 0 (Q: unit at row i, tail A[r][i]) or 1 (P: unit at i+1, tail A[i][r])
+__device__ inline float qp_refl(const ProbDesc &P, int side, int i, int r)
+{
+    const int unit = i + side;
+    if (r < unit) return 0.f;
+    if (r == unit) return 1.f;
+    return side == 0 ? P.a[(size_t)r * P.N + i] : P.a[(size_t)i * P.N + r];
+}
+
+// V (Lr x 32) and VT = V T (Lr x 32) of block b; eight rows per workgroup
+__global__ __launch_bounds__(256) void qp_extract_kernel(ProbDesc P, int side, int b, const float *T, float *V,
+                                                         float *VT)
+{
+    __shared__ float Ts[32][33];
+    __shared__ float Vr[8][33];
+    const int L = side == 0 ? P.M : P.N, nref = side == 0 ? P.N : P.N - 1;
+    const int i0 = 32 * b, nb = min(32, nref - i0), R0 = i0 + side;
+    const int tid = threadIdx.x, t = tid & 31, rb = tid >> 5;
+    for (int q = tid; q < 1024; q += 256) Ts[q >> 5][q & 31] = T[(size_t)b * 1024 + q];
+    const int rr = blockIdx.x * 8 + rb, r = R0 + rr;
+    const bool ok = r < L;
+    const float v = (ok && t < nb) ? qp_refl(P, side, i0 + t, r) : 0.f;
+    Vr[rb][t] = v;
+    __syncthreads();
+    float acc = 0.f;
+    for (int s = 0; s <= t; ++s) acc += Vr[rb][s] * Ts[s][t];
+    if (ok) {
+        V[(size_t)rr * 32 + t] = v;
+        VT[(size_t)rr * 32 + t] = acc;
+    }
+}
+
+// one GEMM; blockIdx.y = split s of the K range (A, B advance by s * kchunk rows of K,
+// C by s * cstride)
+template <int MODE>
+__global__ __launch_bounds__(GT) void qp_gemm_kernel(GemmDesc G, int kchunk, size_t cstride)
+{
+    TileCtx t;
+    const int s = blockIdx.y, k0 = s * kchunk;
+    t.A = G.A + (G.transA ? (size_t)k0 * G.lda : (size_t)k0);
+    t.B = G.B + (G.transB ? (size_t)k0 : (size_t)k0 * G.ldb);
+    t.C = G.C + (size_t)s * cstride;
+    t.brow = nullptr;
+    t.crow = nullptr;
+    t.lda = G.lda;
+    t.ldb = G.ldb;
+    t.ldc = G.ldc;
+    t.m = G.m;
+    t.n = G.n;
+    t.K = min(kchunk, G.k - k0);
+    t.transA = G.transA;
+    t.transB = G.transB;
+    t.alpha = G.alpha;
+    t.beta = G.beta;
+    t.m0 = (blockIdx.x / G.tiles_n) * 64;
+    t.n0 = (blockIdx.x % G.tiles_n) * 64;
+    if (t.m0 >= t.m || t.n0 >= t.n || t.K <= 0) return;
+    tile_gemm<MODE>(t);
+}
+
+// W[q] = sum_s Wp[s][q], q < count
+__global__ __launch_bounds__(256) void qp_sum_kernel(const float *Wp, int S, size_t stride, float *W, int count)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= count) return;
+    float a = 0.f;
+    for (int s = 0; s < S; ++s) a += Wp[(size_t)s * stride + q];
+    W[q] = a;
+}
+
+static hipError_t qp_gemm(const GemmDesc &G, int splits, int kchunk, size_t cstride, hipStream_t st)
+{
+    const int tiles = cdiv_host(G.m, 64) * G.tiles_n;
+    const dim3 grid(tiles, splits);
+    const int mode = gemm_get_mode();
+    if (mode == 1)
+        hipLaunchKernelGGL(qp_gemm_kernel<1>, grid, dim3(GT), 0, st, G, kchunk, cstride);
+    else if (mode == 2)
+        hipLaunchKernelGGL(qp_gemm_kernel<2>, grid, dim3(GT), 0, st, G, kchunk, cstride);
+    else
+        hipLaunchKernelGGL(qp_gemm_kernel<0>, grid, dim3(GT), 0, st, G, kchunk, cstride);
+    return hipGetLastError();
+}
+
+size_t qp_gemm_ws_bytes(int M, int N)
+{
+    const size_t L = (size_t)(M > N ? M : N);
+    return sizeof(float) * (2 * 32 * L + (size_t)(QP_MAX_SPLITS + 1) * 32 * L) + 256;
+}
+
+// X (side 0: u = Q, M x M; side 1: vt = P, N x N) for one matrix whose X is already the
+// identity and whose T factors are in P.tq / P.tp.
+hipError_t launch_qp_gemm(const ProbDesc &P, int side, char *ws, hipStream_t st)
+{
+    const int L = side == 0 ? P.M : P.N, nref = side == 0 ? P.N : P.N - 1;
+    if (nref <= 0) return hipSuccess;
+    float *X = side == 0 ? P.u : P.vt;
+    const float *T = side == 0 ? P.tq : P.tp;
+    const size_t Lmax = (size_t)(P.M > P.N ? P.M : P.N);
+    float *V = reinterpret_cast<float *>(ws), *VT = V + 32 * Lmax, *W = VT + 32 * Lmax, *Wp = W + 32 * Lmax;
+    hipError_t e;
+    for (int b = (nref + 31) / 32 - 1; b >= 0; --b) {
+        const int R0 = 32 * b + side, Lr = L - R0;
+        hipLaunchKernelGGL(qp_extract_kernel, dim3(cdiv_host(Lr, 8)), dim3(256), 0, st, P, side, b, T, V, VT);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        // W_s = V^T X[R0:, R0:]: 32 x Lr, K = Lr split so that about 512 workgroups run
+        GemmDesc g;
+        std::memset(&g, 0, sizeof(g));
+        g.A = V;
+        g.lda = 32;
+        g.transA = 1;
+        g.B = X + (size_t)R0 * L + R0;
+        g.ldb = L;
+        g.C = Wp;
+        g.ldc = Lr;
+        g.m = 32;
+        g.n = Lr;
+        g.k = Lr;
+        g.alpha = 1.f;
+        g.beta = 0.f;
+        g.tiles_n = cdiv_host(Lr, 64);
+        int S = 512 / g.tiles_n;
+        S = S < 1 ? 1 : S > QP_MAX_SPLITS ? QP_MAX_SPLITS : S;
+        const int kchunk = ((cdiv_host(Lr, S) + 15) / 16) * 16;
+        S = cdiv_host(Lr, kchunk);
+        if ((e = qp_gemm(g, S, kchunk, (size_t)32 * Lr, st)) != hipSuccess) return e;
+        hipLaunchKernelGGL(qp_sum_kernel, dim3(cdiv_host(32 * Lr, 256)), dim3(256), 0, st, Wp, S, (size_t)32 * Lr, W,
+                           32 * Lr);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        // X[R0:, R0:] -= VT W
+        GemmDesc h;
+        std::memset(&h, 0, sizeof(h));
+        h.A = VT;
+        h.lda = 32;
+        h.transA = 0;
+        h.B = W;
+        h.ldb = Lr;
+        h.C = X + (size_t)R0 * L + R0;
+        h.ldc = L;
+        h.m = Lr;
+        h.n = Lr;
+        h.k = 32;
+        h.alpha = -1.f;
+        h.beta = 1.f;
+        h.tiles_n = cdiv_host(Lr, 64);
+        if ((e = qp_gemm(h, 1, 32, 0, st)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+} // namespace paco
