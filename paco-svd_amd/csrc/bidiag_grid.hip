@@ -22,7 +22,9 @@ namespace paco {
 
 #define GBT 256
 constexpr int NB = 32;
-constexpr int GR = 16, GC = 16;
+// workgroup grid over the trailing matrix of a panel: gr x gc tiles (runtime, at most
+// GMAX x GMAX; the workspace holds GMAX partial sums per row / column)
+constexpr int GMAX = 32;
 constexpr int SWITCH = 384;
 
 struct GridWs {
@@ -43,14 +45,14 @@ static size_t carve(int M, int N, char *base, GridWs *W)
     w.Y = take((size_t)N * NB);
     w.wcol = take(M);
     w.wrow = take(N);
-    w.g1p = take((size_t)GR * N);
-    w.g2p = take((size_t)GC * M);
-    w.ssp = reinterpret_cast<double *>(take(2 * GR));
-    w.rsp = reinterpret_cast<double *>(take(2 * GC));
-    w.t1p = take(GR * NB);
-    w.t2p = take(GR * NB);
-    w.t3p = take(GC * NB);
-    w.t4p = take(GC * NB);
+    w.g1p = take((size_t)GMAX * N);
+    w.g2p = take((size_t)GMAX * M);
+    w.ssp = reinterpret_cast<double *>(take(2 * GMAX));
+    w.rsp = reinterpret_cast<double *>(take(2 * GMAX));
+    w.t1p = take(GMAX * NB);
+    w.t2p = take(GMAX * NB);
+    w.t3p = take(GMAX * NB);
+    w.t4p = take(GMAX * NB);
     if (W) *W = w;
     return o;
 }
@@ -61,6 +63,7 @@ struct PanelArgs {
     ProbDesc P;
     GridWs W;
     int p, j, th, tw, finish_only;
+    int gr, gc; // tile grid
 };
 
 // column reflector t of the panel (unit at row p+t), entry at row r
@@ -92,6 +95,7 @@ __device__ inline double gb_block_sum(double v, double *red)
 // dynamic LDS: xs[th] (x of the previous column), wc[th] (updated column i)
 __global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
 {
+    const int GR = a.gr, GC = a.gc;
     const int rb = blockIdx.x / GC, cb = blockIdx.x % GC;
     const ProbDesc &P = a.P;
     const GridWs &W = a.W;
@@ -275,6 +279,7 @@ __global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
 // dynamic LDS: yl[tw] (y of column i for own columns), wl[tw] (updated row i)
 __global__ __launch_bounds__(GBT) void gb_row_kernel(PanelArgs a)
 {
+    const int GR = a.gr, GC = a.gc;
     const int rb = blockIdx.x / GC, cb = blockIdx.x % GC;
     const ProbDesc &P = a.P;
     const GridWs &W = a.W;
@@ -454,12 +459,19 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s)
     const int M = P.M, N = P.N;
     GridWs W;
     carve(M, N, static_cast<char *>(ws), &W);
+    // tile grid: PACO_GRID overrides (8 .. GMAX)
+    const char *genv = getenv("PACO_GRID");
+    int G = genv ? atoi(genv) : 16;
+    G = G < 8 ? 8 : G > GMAX ? GMAX : G;
+    const int GR = G, GC = G;
     int p = 0;
     while (N - p > SWITCH) {
         PanelArgs a;
         a.P = P;
         a.W = W;
         a.p = p;
+        a.gr = GR;
+        a.gc = GC;
         a.th = (M - p + GR - 1) / GR;
         a.tw = (N - p + GC - 1) / GC;
         const size_t lds_col = sizeof(float) * 2 * (size_t)a.th;
