@@ -83,19 +83,22 @@ __device__ inline double fused_pass(const RowStore &S, int M, int N, int i, floa
                                     float *cbuf, const float *wbuf, const float *ubuf, float *rowbuf, float *pwr)
 {
     constexpr int NW = NT / 64;
+    // rows per wavefront step: four (four reductions in flight), two at 1024 threads
+    // (128 VGPRs per lane there: four rows of eight chunks would spill)
+    constexpr int RQ = NT >= 1024 ? 2 : 4;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, k = i + 1;
     const bool upd = i >= 0;
     float pwp[8];
     double ss = 0.0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) pwp[c] = 0.f;
-    for (int r = k + 4 * wv; r < M; r += 4 * NW) {
-        float x[4][8], acc[4];
-        float *rp[4];
+    for (int r = k + RQ * wv; r < M; r += RQ * NW) {
+        float x[RQ][8], acc[RQ];
+        float *rp[RQ];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) rp[q] = (r + q < M) ? S.row(r + q) : nullptr;
+        for (int q = 0; q < RQ; ++q) rp[q] = (r + q < M) ? S.row(r + q) : nullptr;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < RQ; ++q)
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const int j = k + lane + 64 * c;
@@ -103,7 +106,7 @@ __device__ inline double fused_pass(const RowStore &S, int M, int N, int i, floa
             }
         if (upd) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < RQ; ++q) {
                 const float vr = (rp[q] && tau != 0.f) ? cbuf[r + q] * inv : 0.f;
                 acc[q] = 0.f;
 #pragma unroll
@@ -117,9 +120,9 @@ __device__ inline double fused_pass(const RowStore &S, int M, int N, int i, floa
                 if (rp[q] && lane == 0) rp[q][i] = (tau != 0.f) ? vr : cbuf[r + q];
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = wave_sum(acc[q]) * tau2;
+            for (int q = 0; q < RQ; ++q) acc[q] = wave_sum(acc[q]) * tau2;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < RQ; ++q) {
                 if (rp[q]) {
 #pragma unroll
                     for (int c = 0; c < 8; ++c) {
@@ -134,7 +137,7 @@ __device__ inline double fused_pass(const RowStore &S, int M, int N, int i, floa
         }
         // next column's data
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < RQ; ++q) {
             const int rq = r + q;
             if (rq >= M) break;
             const float cv = __shfl(x[q][0], 0, 64);

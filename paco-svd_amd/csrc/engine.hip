@@ -617,21 +617,27 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     // stream, concurrently with the streamed ones
     int n_side = bp.n_res;
     for (int g = 0; g < BatchPlan::kWaveGroups; ++g) n_side += bp.n_wave[g];
+    // PACO_BIDIAG_SERIAL=1: every bidiagonalization launch on the caller's stream (schedule
+    // comparisons)
+    const char *serial_env = getenv("PACO_BIDIAG_SERIAL");
+    const bool serial = serial_env && serial_env[0] == '1';
+    hipStream_t side = serial ? s : res.side;
     if (n_side > 0) {
-        if ((e = hipEventRecord(res.evf, s)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(res.side, res.evf, 0)) != hipSuccess) return e;
+        if (!serial) {
+            if ((e = hipEventRecord(res.evf, s)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(res.side, res.evf, 0)) != hipSuccess) return e;
+        }
         for (int g = BatchPlan::kWaveGroups - 1; g >= 0; --g)
             if ((e = launch_bidiag_wave(pl, I(bp.o_order_wave[g]), bp.n_wave[g], g % BIDIAG_WAVE_MAXC + 1,
-                                        bp.lds_wave[g], res.side)) != hipSuccess)
+                                        bp.lds_wave[g], side)) != hipSuccess)
                 return e;
-        if ((e = launch_bidiag_wg(pl, I(bp.o_order_res), bp.n_res, bp.lds_res, res.side, 256)) != hipSuccess)
-            return e;
-        if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
+        if ((e = launch_bidiag_wg(pl, I(bp.o_order_res), bp.n_res, bp.lds_res, side, 256)) != hipSuccess) return e;
+        if (!serial && (e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
     }
     for (int g = 2; g >= 0; --g)
         if ((e = launch_bidiag_wg(pl, I(bp.o_order_nt[g]), bp.n_nt[g], bp.lds_nt[g], s, 256 << g)) != hipSuccess)
             return e;
-    if (n_side > 0 && (e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
+    if (n_side > 0 && !serial && (e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
     tmark(s, 2, tm);
     if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
                        I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)
