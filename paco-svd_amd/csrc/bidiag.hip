@@ -120,7 +120,7 @@ __device__ inline double fused_pass(const RowStore &S, int M, int N, int i, floa
                 if (rp[q] && lane == 0) rp[q][i] = (tau != 0.f) ? vr : cbuf[r + q];
             }
 #pragma unroll
-            for (int q = 0; q < RQ; ++q) acc[q] = wave_sum(acc[q]) * tau2;
+            for (int q = 0; q < RQ; ++q) acc[q] = wave_sum_dpp(acc[q]) * tau2;
 #pragma unroll
             for (int q = 0; q < RQ; ++q) {
                 if (rp[q]) {
