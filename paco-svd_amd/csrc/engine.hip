@@ -319,7 +319,7 @@ static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, floa
 // batched path
 // ---------------------------------------------------------------------------
 struct BatchScratch {
-    size_t a, e, tauq, taup, qt, wt, idx, vmt, tq, tp, states, total;
+    size_t a, e, tauq, taup, qt, wt, idx, vmt, tq, tp, states, qp, total;
 };
 
 static BatchScratch scratch_layout(int M, int N)
@@ -342,6 +342,7 @@ static BatchScratch scratch_layout(int M, int N)
     B.tq = take(sizeof(float) * 1024 * (size_t)cdiv(N, 32));
     B.tp = take(sizeof(float) * 1024 * (size_t)cdiv(std::max(N - 1, 1), 32));
     B.states = take(sizeof(MergeState) * (size_t)std::max(total_merges(N), 1));
+    B.qp = take(sizeof(float) * qpb_scratch_floats(M, N));
     B.total = o;
     return B;
 }
@@ -371,6 +372,11 @@ struct BatchPlan {
     int ntu = 0, ntv = 0, nru = 0, nrv = 0;
     std::vector<int> ntask, maxn;
     std::vector<size_t> state_off; // per round, in MergeState units from the states base
+    // Q, P by rounds of grouped GEMMs: per side and round, offsets of the extraction
+    // blocks, W tiles and X tiles over the problems; per-problem scratch pointers at o_qp
+    size_t o_qp = 0;
+    std::vector<size_t> o_qext[2], o_qg1[2], o_qg3[2];
+    std::vector<int> n_qext[2], n_qg1[2], n_qg3[2];
     bool fits = true;
 };
 
@@ -468,6 +474,33 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     bp.o_rowin = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nrowin));
     bp.o_idq = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nidq));
     bp.o_idp = blob_put(b, prefix(count, [&](int i) { return jobs[i].N; }, bp.nidp));
+    {
+        std::vector<float *> qpp(count, nullptr); // device addresses patched at upload
+        bp.o_qp = blob_put(b, qpp);
+        for (int side = 0; side < 2; ++side) {
+            int rounds = 0;
+            for (int i = 0; i < count; ++i)
+                rounds = std::max(rounds, cdiv(side == 0 ? jobs[i].N : std::max(jobs[i].N - 1, 0), 32));
+            bp.o_qext[side].assign(rounds, 0);
+            bp.o_qg1[side].assign(rounds, 0);
+            bp.o_qg3[side].assign(rounds, 0);
+            bp.n_qext[side].assign(rounds, 0);
+            bp.n_qg1[side].assign(rounds, 0);
+            bp.n_qg3[side].assign(rounds, 0);
+            for (int r = 0; r < rounds; ++r) {
+                auto lr = [&](int i) {
+                    const int L = side == 0 ? jobs[i].M : jobs[i].N;
+                    const int nb = cdiv(side == 0 ? jobs[i].N : std::max(jobs[i].N - 1, 0), 32);
+                    const int b = nb - 1 - r;
+                    return b < 0 ? 0 : L - (32 * b + side);
+                };
+                bp.o_qext[side][r] = blob_put(b, prefix(count, [&](int i) { return cdiv(lr(i), 8); }, bp.n_qext[side][r]));
+                bp.o_qg1[side][r] = blob_put(b, prefix(count, [&](int i) { return cdiv(lr(i), 64); }, bp.n_qg1[side][r]));
+                bp.o_qg3[side][r] =
+                    blob_put(b, prefix(count, [&](int i) { return cdiv(lr(i), 64) * cdiv(lr(i), 64); }, bp.n_qg3[side][r]));
+            }
+        }
+    }
     bp.o_bq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, 32); }, bp.nbq));
     bp.o_bp = blob_put(b, prefix(count, [&](int i) { return cdiv(std::max(jobs[i].N - 1, 0), 32); }, bp.nbp));
     bp.o_sq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, 64); }, bp.nsq));
@@ -588,6 +621,7 @@ static hipError_t prepare_batch(const paco_svd_job *jobs, int count, BatchPlan &
         P.tq = reinterpret_cast<float *>(base + L.tq);
         P.tp = reinterpret_cast<float *>(base + L.tp);
         P.info = pr.info + i;
+        reinterpret_cast<float **>(bp.blob.data() + bp.o_qp)[i] = reinterpret_cast<float *>(base + L.qp);
     }
     memcpy(res.pinned, bp.blob.data(), bp.blob.size());
     return hipSuccess;
@@ -639,9 +673,25 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
             return e;
     if (n_side > 0 && !serial && (e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
     tmark(s, 2, tm);
-    if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
-                       I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)
-        return e;
+    const char *qenv = getenv("PACO_QP_BATCH");
+    if (qenv && qenv[0] == '0') { // strips of 64 columns, FMA
+        if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
+                           I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)
+            return e;
+    } else { // rounds of grouped GEMMs on the matrix cores
+        if ((e = launch_identity(pl, I(bp.o_idq), bp.nidq, 0, s)) != hipSuccess) return e;
+        if ((e = launch_identity(pl, I(bp.o_idp), bp.nidp, 1, s)) != hipSuccess) return e;
+        if ((e = launch_tfactor(pl, I(bp.o_bq), bp.nbq, 0, s)) != hipSuccess) return e;
+        if ((e = launch_tfactor(pl, I(bp.o_bp), bp.nbp, 1, s)) != hipSuccess) return e;
+        float *const *qp = reinterpret_cast<float *const *>(dev + bp.o_qp);
+        for (int side = 0; side < 2; ++side)
+            for (size_t r = 0; r < bp.o_qext[side].size(); ++r)
+                if ((e = launch_qpb_round(pl, qp, side, (int)r, I(bp.o_qext[side][r]), bp.n_qext[side][r],
+                                          I(bp.o_qg1[side][r]), bp.n_qg1[side][r], I(bp.o_qg3[side][r]),
+                                          bp.n_qg3[side][r], s)) != hipSuccess)
+                    return e;
+        if ((e = launch_transpose_vt(pl, I(bp.o_tp), bp.ntp, s)) != hipSuccess) return e;
+    }
     tmark(s, 3, tm);
     DcOffsets o;
     o.leaf_off = I(bp.o_leaf);

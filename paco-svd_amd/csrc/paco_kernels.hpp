@@ -86,6 +86,11 @@ hipError_t launch_dc_backtransform(ProbList pl, const int *tile_off_u, int ntile
 constexpr int QP_MAX_SPLITS = 16;
 size_t qp_gemm_ws_bytes(int M, int N);
 hipError_t launch_qp_gemm(const ProbDesc &P, int side, char *ws, hipStream_t st);
+// batched: per-problem scratch qp[pid] of qpb_scratch_floats(M, N) floats; round r applies
+// block nblk - 1 - r of every problem that has it
+size_t qpb_scratch_floats(int M, int N);
+hipError_t launch_qpb_round(ProbList pl, float *const *qp, int side, int round, const int *ext_off, int next,
+                            const int *g1_off, int ng1, const int *g3_off, int ng3, hipStream_t st);
 
 // gemm.hip
 hipError_t launch_gemm_grouped(const GemmDesc *descs, int ndesc, int ntiles, hipStream_t s);

@@ -167,4 +167,139 @@ hipError_t launch_qp_gemm(const ProbDesc &P, int side, char *ws, hipStream_t st)
     return hipSuccess;
 }
 
+// ---------------------------------------------------------------------------
+// Batched: round r applies, for every problem with more than r blocks, its block
+// nblk - 1 - r; the work of a round is dealt through per-problem offsets (off) and each
+// problem has a scratch area qp[pid] of 96 max(M, N) floats (V, VT, W).
+// ---------------------------------------------------------------------------
+struct QpbRound {
+    int R0, Lr, L, b;
+    float *X, *V, *VT, *W;
+    const float *T;
+};
+__device__ inline QpbRound qpb_round(const ProbDesc &P, float *qp, int side, int round)
+{
+    QpbRound q;
+    q.L = side == 0 ? P.M : P.N;
+    const int nref = side == 0 ? P.N : P.N - 1;
+    q.b = (nref + 31) / 32 - 1 - round;
+    q.R0 = 32 * q.b + side;
+    q.Lr = q.L - q.R0;
+    q.X = side == 0 ? P.u : P.vt;
+    const size_t Lmax = (size_t)(P.M > P.N ? P.M : P.N);
+    q.V = qp;
+    q.VT = qp + 32 * Lmax;
+    q.W = qp + 64 * Lmax;
+    q.T = side == 0 ? P.tq : P.tp;
+    return q;
+}
+
+__global__ __launch_bounds__(256) void qpb_extract_kernel(ProbList pl, float *const *qp, const int *off, int side,
+                                                          int round)
+{
+    __shared__ float Ts[32][33];
+    __shared__ float Vr[8][33];
+    const int pid = find_task(off, pl.count, blockIdx.x);
+    const int blk = blockIdx.x - off_at(off, pid);
+    const ProbDesc P = pl.get(pid);
+    const QpbRound q = qpb_round(P, qp[pid], side, round);
+    const int nref = side == 0 ? P.N : P.N - 1, i0 = 32 * q.b, nb = min(32, nref - i0);
+    const int tid = threadIdx.x, t = tid & 31, rb = tid >> 5;
+    for (int e = tid; e < 1024; e += 256) Ts[e >> 5][e & 31] = q.T[(size_t)q.b * 1024 + e];
+    const int rr = blk * 8 + rb, r = q.R0 + rr;
+    const bool ok = rr < q.Lr;
+    const float v = (ok && t < nb) ? qp_refl(P, side, i0 + t, r) : 0.f;
+    Vr[rb][t] = v;
+    __syncthreads();
+    float acc = 0.f;
+    for (int s2 = 0; s2 <= t; ++s2) acc += Vr[rb][s2] * Ts[s2][t];
+    if (ok) {
+        q.V[(size_t)rr * 32 + t] = v;
+        q.VT[(size_t)rr * 32 + t] = acc;
+    }
+}
+
+// W (32 x Lr) = V^T X[R0:, R0:]; one 64-column tile of W per workgroup
+template <int MODE>
+__global__ __launch_bounds__(GT) void qpb_gemm1_kernel(ProbList pl, float *const *qp, const int *off, int side,
+                                                       int round)
+{
+    const int pid = find_task(off, pl.count, blockIdx.x);
+    const int tile = blockIdx.x - off_at(off, pid);
+    const ProbDesc P = pl.get(pid);
+    const QpbRound q = qpb_round(P, qp[pid], side, round);
+    TileCtx t;
+    t.A = q.V;
+    t.lda = 32;
+    t.transA = 1;
+    t.B = q.X + (size_t)q.R0 * q.L + q.R0;
+    t.ldb = q.L;
+    t.transB = 0;
+    t.C = q.W;
+    t.ldc = q.Lr;
+    t.brow = t.crow = nullptr;
+    t.m = 32;
+    t.n = q.Lr;
+    t.K = q.Lr;
+    t.alpha = 1.f;
+    t.beta = 0.f;
+    t.m0 = 0;
+    t.n0 = tile * 64;
+    if (t.n0 >= t.n) return;
+    tile_gemm<MODE>(t);
+}
+
+// X[R0:, R0:] -= VT W; one 64 x 64 tile per workgroup
+template <int MODE>
+__global__ __launch_bounds__(GT) void qpb_gemm3_kernel(ProbList pl, float *const *qp, const int *off, int side,
+                                                       int round)
+{
+    const int pid = find_task(off, pl.count, blockIdx.x);
+    const int tile = blockIdx.x - off_at(off, pid);
+    const ProbDesc P = pl.get(pid);
+    const QpbRound q = qpb_round(P, qp[pid], side, round);
+    const int tn = (q.Lr + 63) / 64;
+    TileCtx t;
+    t.A = q.VT;
+    t.lda = 32;
+    t.transA = 0;
+    t.B = q.W;
+    t.ldb = q.Lr;
+    t.transB = 0;
+    t.C = q.X + (size_t)q.R0 * q.L + q.R0;
+    t.ldc = q.L;
+    t.brow = t.crow = nullptr;
+    t.m = q.Lr;
+    t.n = q.Lr;
+    t.K = 32;
+    t.alpha = -1.f;
+    t.beta = 1.f;
+    t.m0 = (tile / tn) * 64;
+    t.n0 = (tile % tn) * 64;
+    if (t.m0 >= t.m || t.n0 >= t.n) return;
+    tile_gemm<MODE>(t);
+}
+
+size_t qpb_scratch_floats(int M, int N) { return 96 * (size_t)(M > N ? M : N); }
+
+// one round of one side: extraction blocks, W tiles and X tiles through their offsets
+hipError_t launch_qpb_round(ProbList pl, float *const *qp, int side, int round, const int *ext_off, int next,
+                            const int *g1_off, int ng1, const int *g3_off, int ng3, hipStream_t st)
+{
+    if (next <= 0) return hipSuccess;
+    hipLaunchKernelGGL(qpb_extract_kernel, dim3(next), dim3(256), 0, st, pl, qp, ext_off, side, round);
+    const int mode = gemm_get_mode();
+    if (mode == 1) {
+        hipLaunchKernelGGL(qpb_gemm1_kernel<1>, dim3(ng1), dim3(GT), 0, st, pl, qp, g1_off, side, round);
+        hipLaunchKernelGGL(qpb_gemm3_kernel<1>, dim3(ng3), dim3(GT), 0, st, pl, qp, g3_off, side, round);
+    } else if (mode == 2) {
+        hipLaunchKernelGGL(qpb_gemm1_kernel<2>, dim3(ng1), dim3(GT), 0, st, pl, qp, g1_off, side, round);
+        hipLaunchKernelGGL(qpb_gemm3_kernel<2>, dim3(ng3), dim3(GT), 0, st, pl, qp, g3_off, side, round);
+    } else {
+        hipLaunchKernelGGL(qpb_gemm1_kernel<0>, dim3(ng1), dim3(GT), 0, st, pl, qp, g1_off, side, round);
+        hipLaunchKernelGGL(qpb_gemm3_kernel<0>, dim3(ng3), dim3(GT), 0, st, pl, qp, g3_off, side, round);
+    }
+    return hipGetLastError();
+}
+
 } // namespace paco
