@@ -684,12 +684,19 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
         if ((e = launch_tfactor(pl, I(bp.o_bq), bp.nbq, 0, s)) != hipSuccess) return e;
         if ((e = launch_tfactor(pl, I(bp.o_bp), bp.nbp, 1, s)) != hipSuccess) return e;
         float *const *qp = reinterpret_cast<float *const *>(dev + bp.o_qp);
-        for (int side = 0; side < 2; ++side)
+        // P's rounds on the forked stream, concurrently with Q's
+        if ((e = hipEventRecord(res.evf, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(res.side, res.evf, 0)) != hipSuccess) return e;
+        for (int side = 1; side >= 0; --side) {
+            hipStream_t st = side == 1 ? res.side : s;
             for (size_t r = 0; r < bp.o_qext[side].size(); ++r)
                 if ((e = launch_qpb_round(pl, qp, side, (int)r, I(bp.o_qext[side][r]), bp.n_qext[side][r],
                                           I(bp.o_qg1[side][r]), bp.n_qg1[side][r], I(bp.o_qg3[side][r]),
-                                          bp.n_qg3[side][r], s)) != hipSuccess)
+                                          bp.n_qg3[side][r], st)) != hipSuccess)
                     return e;
+        }
+        if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
         if ((e = launch_transpose_vt(pl, I(bp.o_tp), bp.ntp, s)) != hipSuccess) return e;
     }
     tmark(s, 3, tm);

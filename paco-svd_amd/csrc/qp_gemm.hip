@@ -170,7 +170,7 @@ hipError_t launch_qp_gemm(const ProbDesc &P, int side, char *ws, hipStream_t st)
 // ---------------------------------------------------------------------------
 // Batched: round r applies, for every problem with more than r blocks, its block
 // nblk - 1 - r; the work of a round is dealt through per-problem offsets (off) and each
-// problem has a scratch area qp[pid] of 96 max(M, N) floats (V, VT, W).
+// problem has a scratch area qp[pid] of 2 x 96 max(M, N) floats (V, VT, W per side).
 // ---------------------------------------------------------------------------
 struct QpbRound {
     int R0, Lr, L, b;
@@ -187,9 +187,10 @@ __device__ inline QpbRound qpb_round(const ProbDesc &P, float *qp, int side, int
     q.Lr = q.L - q.R0;
     q.X = side == 0 ? P.u : P.vt;
     const size_t Lmax = (size_t)(P.M > P.N ? P.M : P.N);
-    q.V = qp;
-    q.VT = qp + 32 * Lmax;
-    q.W = qp + 64 * Lmax;
+    // each side has its own half of the area: the two sides' rounds may run concurrently
+    q.V = qp + (size_t)side * 96 * Lmax;
+    q.VT = q.V + 32 * Lmax;
+    q.W = q.V + 64 * Lmax;
     q.T = side == 0 ? P.tq : P.tp;
     return q;
 }
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(GT) void qpb_gemm3_kernel(ProbList pl, float *const
     tile_gemm<MODE>(t);
 }
 
-size_t qpb_scratch_floats(int M, int N) { return 96 * (size_t)(M > N ? M : N); }
+size_t qpb_scratch_floats(int M, int N) { return 2 * 96 * (size_t)(M > N ? M : N); }
 
 // one round of one side: extraction blocks, W tiles and X tiles through their offsets
 hipError_t launch_qpb_round(ProbList pl, float *const *qp, int side, int round, const int *ext_off, int next,
