@@ -47,17 +47,19 @@ __device__ inline double block_sum_d(double v, double *dred)
     return r;
 }
 
-struct RowStore {
-    float *G; // global matrix
-    float *L; // LDS rows split..M-1
-    int ldg, ldl, split;
-    __device__ inline float *row(int r) const
-    {
-        return r >= split ? L + (size_t)(r - split) * ldl : G + (size_t)r * ldg;
-    }
-};
+// LDS- and global-typed pointers: accesses through them compile to LDS (ds_*) and
+// global (global_*) instructions, never to FLAT ones that must wait on both counters.
+typedef __attribute__((address_space(3))) float lds_f32;
+typedef __attribute__((address_space(1))) float glb_f32;
 
 constexpr int WG_MAXN = 512; // 8 chunks of 64 columns per lane
+// the first LDS-cached row is a multiple of this (every group size of the pass divides it)
+constexpr int WG_SPLIT_ALIGN = 8;
+#ifndef RQ1024_2
+#define RQ1024_2 4
+#define RQ1024_4 2
+#define RQ1024_8 1
+#endif
 
 // (the first 2 (nt / 64 + 2) floats hold FP64 partial sums)
 __host__ __device__ inline size_t bidiag_vec_floats(int M, int N, int nt)
@@ -74,98 +76,193 @@ __host__ __device__ inline int bidiag_lds_rows(int M, int N, int nt, size_t lds_
     return r > (size_t)M ? M : (int)r;
 }
 
-// Fused pass over rows r > i (i = -1: read-only initial pass).  Row update
-//   t = A[r, j] - v_r w_j, x_r = tau2 t.u, A'[r, j] = t_j - x_r u_j   (j > i)
-// then, for column k = i + 1: row k -> rowbuf; rows r > k: cbuf[r] = A'[r, k],
-// sumsq partial, pw partial (columns > k).  Wave partials of pw go to pwr[wave].
-template <int NT>
-__device__ inline double fused_pass(const RowStore &S, int M, int N, int i, float tau, float inv, float tau2,
-                                    float *cbuf, const float *wbuf, const float *ubuf, float *rowbuf, float *pwr)
+// first LDS-cached row: M - (rows that fit), rounded up to WG_SPLIT_ALIGN (M: none)
+__host__ __device__ inline int bidiag_split_row(int M, int N, int nt, size_t lds_bytes)
 {
-    constexpr int NW = NT / 64;
-    // rows per wavefront step: four (four reductions in flight), two at 1024 threads
-    // (128 VGPRs per lane there: four rows of eight chunks would spill)
-    constexpr int RQ = NT >= 1024 ? 2 : 4;
+    const int s = M - bidiag_lds_rows(M, N, nt, lds_bytes);
+    if (s == 0) return 0;
+    const int a = (s + WG_SPLIT_ALIGN - 1) / WG_SPLIT_ALIGN * WG_SPLIT_ALIGN;
+    return a < M ? a : M;
+}
+
+// rows per group of the fused pass (one DPP reduction in flight per row; the next
+// group is loaded while this one is processed), chosen so that the two groups stay in
+// registers: at 1024 threads a lane has 128 VGPRs
+template <int NT, int C>
+struct PassRows {
+    static constexpr int RQ = NT >= 1024 ? (C <= 2 ? RQ1024_2 : C <= 4 ? RQ1024_4 : RQ1024_8)
+                                         : (C <= 2 ? 8 : C <= 4 ? 4 : 2);
+};
+
+// Chunk c of a lane holds column k + lane + 64 c; with C = ceil((N - k) / 64) only the
+// last chunk can reach past column N - 1, so every other chunk is read and written
+// without a test, at immediate offsets from one address per row.
+//
+// Loads the group of RQ rows starting at row r0 into x.  base points at row rbase; rows
+// outside [k, lim) are replaced by the nearest row inside (read, never written back).
+template <int C, int RQ, typename T>
+__device__ inline void pass_load(const T *base, int ld, int rbase, int r0, int k, int lim, int N, int lane,
+                                 float (&x)[RQ][C])
+{
+    const bool tail = k + lane + 64 * (C - 1) < N;
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+        int rr = r0 + q;
+        rr = rr < k ? k : rr;
+        rr = rr >= lim ? lim - 1 : rr;
+        const T *row = base + (size_t)(rr - rbase) * ld + k + lane;
+#pragma unroll
+        for (int c = 0; c < C - 1; ++c) x[q][c] = row[64 * c];
+        x[q][C - 1] = tail ? row[64 * (C - 1)] : 0.f;
+    }
+}
+
+// One group of the fused pass over rows r > i (i = -1: read-only initial pass).  Row update
+//   t = A[r, j] - v_r w_j, x_r = tau2 t.u, A'[r, j] = t_j - x_r u_j   (j > i), A'[r, i] = v_r
+// then, for column k = i + 1: row k -> rowbuf; rows r > k: cbuf[r] = A'[r, k], sum of
+// squares (FP64, lane 0) and the pw partials of this lane (the entry of column k itself
+// is never read).  Columns past N - 1 hold zeros (w, u are zero there).
+template <int C, int RQ, typename T>
+__device__ inline void pass_group(T *base, int ld, int rbase, int r0, int k, int M, int N, int lane, bool upd,
+                                  float tau, float inv, float tau2, const float (&wr)[C], const float (&ur)[C],
+                                  float (&x)[RQ][C], lds_f32 *cbuf, lds_f32 *rowbuf, float (&pwp)[C], double &ss)
+{
+    const int i = k - 1;
+    const bool tail = k + lane + 64 * (C - 1) < N;
+    if (upd) {
+        float acc[RQ];
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int rr = r0 + q;
+            const bool on = rr >= k && rr < M;
+            const float cr = cbuf[rr < k ? k : (rr >= M ? M - 1 : rr)];
+            const float vr = (on && tau != 0.f) ? cr * inv : 0.f;
+            float a = 0.f;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                x[q][c] -= vr * wr[c];
+                a += x[q][c] * ur[c];
+            }
+            acc[q] = a;
+            if (on && lane == 0) base[(size_t)(rr - rbase) * ld + i] = (tau != 0.f) ? vr : cr;
+        }
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) acc[q] = wave_sum_dpp(acc[q]) * tau2;
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int rr = r0 + q;
+#pragma unroll
+            for (int c = 0; c < C; ++c) x[q][c] -= acc[q] * ur[c];
+            if (rr >= k && rr < M) {
+                T *row = base + (size_t)(rr - rbase) * ld + k + lane;
+#pragma unroll
+                for (int c = 0; c < C - 1; ++c) row[64 * c] = x[q][c];
+                if (tail) row[64 * (C - 1)] = x[q][C - 1];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+        const int rr = r0 + q;
+        if (rr < k || rr >= M) continue;
+        const float cv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x[q][0]), 0));
+        if (rr == k) {
+            lds_f32 *rb = rowbuf + k + lane;
+#pragma unroll
+            for (int c = 0; c < C - 1; ++c) rb[64 * c] = x[q][c];
+            if (tail) rb[64 * (C - 1)] = x[q][C - 1];
+        } else {
+            if (lane == 0) {
+                cbuf[rr] = cv;
+                ss += (double)cv * cv;
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) pwp[c] += cv * x[q][c];
+        }
+    }
+}
+
+// Fused pass over rows r > i with C = ceil((N - i - 1) / 64) chunks per row.  Rows are
+// taken in groups of RQ at multiples of RQ, group g by wavefront g % NW, so that every
+// wavefront visits its rows in increasing order whatever the LDS split (the reductions
+// do not depend on it) and no group straddles the split (a multiple of RQ below M).
+// Wave partials of pw go to pwr[wave]; the return value is lane 0's partial sum of squares.
+template <int NT, int C>
+__device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int split, int M, int N, int i,
+                                      float tau, float inv, float tau2, lds_f32 *cbuf, const lds_f32 *wbuf,
+                                      const lds_f32 *ubuf, lds_f32 *rowbuf, lds_f32 *pwr)
+{
+    constexpr int NW = NT / 64, RQ = PassRows<NT, C>::RQ;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, k = i + 1;
     const bool upd = i >= 0;
-    float pwp[8];
+    float wr[C], ur[C], pwp[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int j = k + lane + 64 * c;
+        wr[c] = (upd && j < N) ? wbuf[j] : 0.f;
+        ur[c] = (upd && j < N) ? ubuf[j] : 0.f;
+        pwp[c] = 0.f;
+    }
     double ss = 0.0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) pwp[c] = 0.f;
-    for (int r = k + RQ * wv; r < M; r += RQ * NW) {
-        float x[RQ][8], acc[RQ];
-        float *rp[RQ];
-#pragma unroll
-        for (int q = 0; q < RQ; ++q) rp[q] = (r + q < M) ? S.row(r + q) : nullptr;
+    const int g0 = k / RQ;
+    int g = g0 + ((wv - g0 % NW) % NW + NW) % NW; // first group of this wavefront
+    float xa[RQ][C], xb[RQ][C];
+    // rows in global memory: [k, split)
+    if (RQ * g < split) pass_load<C, RQ>(G, ldg, 0, RQ * g, k, split, N, lane, xa);
+    for (; RQ * g < split; g += NW) {
+        if (RQ * (g + NW) < split) pass_load<C, RQ>(G, ldg, 0, RQ * (g + NW), k, split, N, lane, xb);
+        pass_group<C, RQ>(G, ldg, 0, RQ * g, k, M, N, lane, upd, tau, inv, tau2, wr, ur, xa, cbuf, rowbuf, pwp, ss);
 #pragma unroll
         for (int q = 0; q < RQ; ++q)
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int j = k + lane + 64 * c;
-                x[q][c] = (rp[q] && j < N) ? rp[q][j] : 0.f;
-            }
-        if (upd) {
+            for (int c = 0; c < C; ++c) xa[q][c] = xb[q][c];
+    }
+    // rows in LDS: [max(k, split), M)
+    if (RQ * g < M) pass_load<C, RQ>(L, ldl, split, RQ * g, k, M, N, lane, xa);
+    for (; RQ * g < M; g += NW) {
+        if (RQ * (g + NW) < M) pass_load<C, RQ>(L, ldl, split, RQ * (g + NW), k, M, N, lane, xb);
+        pass_group<C, RQ>(L, ldl, split, RQ * g, k, M, N, lane, upd, tau, inv, tau2, wr, ur, xa, cbuf, rowbuf, pwp,
+                          ss);
 #pragma unroll
-            for (int q = 0; q < RQ; ++q) {
-                const float vr = (rp[q] && tau != 0.f) ? cbuf[r + q] * inv : 0.f;
-                acc[q] = 0.f;
+        for (int q = 0; q < RQ; ++q)
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const int j = k + lane + 64 * c;
-                    if (j < N) {
-                        x[q][c] -= vr * wbuf[j];
-                        acc[q] += x[q][c] * ubuf[j];
-                    }
-                }
-                if (rp[q] && lane == 0) rp[q][i] = (tau != 0.f) ? vr : cbuf[r + q];
-            }
-#pragma unroll
-            for (int q = 0; q < RQ; ++q) acc[q] = wave_sum_dpp(acc[q]) * tau2;
-#pragma unroll
-            for (int q = 0; q < RQ; ++q) {
-                if (rp[q]) {
-#pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        const int j = k + lane + 64 * c;
-                        if (j < N) {
-                            x[q][c] -= acc[q] * ubuf[j];
-                            rp[q][j] = x[q][c];
-                        }
-                    }
-                }
-            }
-        }
-        // next column's data
-#pragma unroll
-        for (int q = 0; q < RQ; ++q) {
-            const int rq = r + q;
-            if (rq >= M) break;
-            const float cv = __shfl(x[q][0], 0, 64);
-            if (rq == k) {
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const int j = k + lane + 64 * c;
-                    if (j < N) rowbuf[j] = x[q][c];
-                }
-            } else {
-                if (lane == 0) {
-                    cbuf[rq] = cv;
-                    ss += (double)cv * cv;
-                }
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const int j = k + lane + 64 * c;
-                    if (j > k && j < N) pwp[c] += cv * x[q][c];
-                }
-            }
-        }
+            for (int c = 0; c < C; ++c) xa[q][c] = xb[q][c];
     }
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < C; ++c) {
         const int j = k + lane + 64 * c;
-        if (j < N) pwr[(size_t)wv * N + j] = pwp[c];
+        if (c < C - 1 || j < N) pwr[(size_t)wv * N + j] = pwp[c];
     }
-    return ss; // lane 0 of each wave holds its partial
+    return ss;
+}
+
+template <int NT>
+__device__ __attribute__((always_inline)) inline double fused_pass(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int split, int M, int N, int i,
+                                    float tau, float inv, float tau2, lds_f32 *cbuf, const lds_f32 *wbuf,
+                                    const lds_f32 *ubuf, lds_f32 *rowbuf, lds_f32 *pwr)
+{
+#define PACO_PASS(CC)                                                                                              \
+    return fused_pass_c<NT, CC>(G, ldg, L, ldl, split, M, N, i, tau, inv, tau2, cbuf, wbuf, ubuf, rowbuf, pwr)
+    switch ((N - i - 1 + 63) >> 6) {
+    case 1: PACO_PASS(1);
+    case 2: PACO_PASS(2);
+    case 3: PACO_PASS(3);
+    case 4: PACO_PASS(4);
+    case 5: PACO_PASS(5);
+    case 6: PACO_PASS(6);
+    case 7: PACO_PASS(7);
+    default: PACO_PASS(8);
+    }
+#undef PACO_PASS
+}
+
+// element (r, j) of the matrix whose rows [split, M) are in LDS
+__device__ inline void store_elem(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int split, int r, int j, float v)
+{
+    if (r < split)
+        G[(size_t)r * ldg + j] = v;
+    else
+        L[(size_t)(r - split) * ldl + j] = v;
 }
 
 template <int NT>
@@ -176,26 +273,27 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
     const ProbDesc P = pl.get(pid);
     const int M = P.M, N = P.N, ldg = P.ld ? P.ld : P.N;
     extern __shared__ float smem[];
+    lds_f32 *lsm = (lds_f32 *)smem;
     double *dred = reinterpret_cast<double *>(smem); // [NW] partials, [NW] sigma
-    float *sm = smem + 2 * (NW + 2);
-    const int nl = bidiag_lds_rows(M, N, NT, lds_bytes);
-    RowStore S;
-    S.G = P.a;
-    S.L = sm;
-    S.ldg = ldg;
-    S.ldl = N | 1;
-    S.split = M - nl;
-    float *cbuf = sm + (size_t)nl * S.ldl, *wbuf = cbuf + M, *ubuf = wbuf + N, *rowbuf = ubuf + N, *pw = rowbuf + N;
-    float *pwr = pw + N;
+    lds_f32 *sm = lsm + 2 * (NW + 2);
+    const int split = bidiag_split_row(M, N, NT, lds_bytes), nl = M - split, ldl = N | 1;
+    glb_f32 *G = (glb_f32 *)P.a;
+    lds_f32 *L = sm;
+    lds_f32 *cbuf = sm + (size_t)nl * ldl, *wbuf = cbuf + M, *ubuf = wbuf + N, *rowbuf = ubuf + N, *pw = rowbuf + N;
+    lds_f32 *pwr = pw + N;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int r = S.split + wv; r < M; r += NW)
-        for (int c = lane; c < N; c += 64) S.L[(size_t)(r - S.split) * S.ldl + c] = P.a[(size_t)r * ldg + c];
+    for (int r = split + wv; r < M; r += NW)
+        for (int c = lane; c < N; c += 64) L[(size_t)(r - split) * ldl + c] = G[(size_t)r * ldg + c];
     __syncthreads();
 
-    // initial read-only pass: column 0 data
-    double ss = fused_pass<NT>(S, M, N, -1, 0.f, 1.f, 0.f, cbuf, wbuf, ubuf, rowbuf, pwr);
-    for (int i = 0; i < N; ++i) {
-        // ---- reduce the pass partials: sum c_r^2 and pw
+    // pass i = -1 is the read-only gathering pass of column 0; pass i >= 0 applies the
+    // reflectors of column i to rows > i and gathers column i + 1
+    float tau = 0.f, inv = 1.f, tau2 = 0.f;
+    for (int i = -1;;) {
+        const double ss = fused_pass<NT>(G, ldg, L, ldl, split, M, N, i, tau, inv, tau2, cbuf, wbuf, ubuf, rowbuf, pwr);
+        ++i;
+        // ---- reduce the pass partials: sum c_r^2 and pw (no reader of dred is left
+        // since the barrier before the pass)
         if (lane == 0) dred[wv] = ss;
         __syncthreads();
         for (int j = i + 1 + tid; j < N; j += NT) {
@@ -212,16 +310,16 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
         }
         __syncthreads();
         // ---- column reflector of column i
-        float tau, beta, inv;
+        float beta;
         house_params_d(rowbuf[i], dred[NW], &beta, &tau, &inv);
-        float *rowi = S.row(i);
         if (tid == 0) {
-            rowi[i] = beta;
+            store_elem(G, ldg, L, ldl, split, i, i, beta);
             P.d[i] = beta;
             P.tauq[i] = tau;
         }
         if (i == N - 1) {
-            for (int r = i + 1 + tid; r < M; r += NT) S.row(r)[i] = (tau != 0.f) ? cbuf[r] * inv : cbuf[r];
+            for (int r = i + 1 + tid; r < M; r += NT)
+                store_elem(G, ldg, L, ldl, split, r, i, (tau != 0.f) ? cbuf[r] * inv : cbuf[r]);
             break;
         }
         // ---- w and the updated row i; row reflector
@@ -234,31 +332,29 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
             if (j > i + 1) part += (double)y * y;
         }
         const double sigma2 = block_sum_d<NT>(part, dred);
-        float tau2, beta2, inv2;
+        float beta2, inv2;
         house_params_d(ubuf[i + 1], sigma2, &beta2, &tau2, &inv2);
         __syncthreads();
         for (int j = i + 1 + tid; j < N; j += NT) {
+            float v;
             if (j == i + 1) {
-                rowi[j] = beta2;
+                v = beta2;
                 ubuf[j] = 1.f;
             } else {
-                const float uv = (tau2 != 0.f) ? ubuf[j] * inv2 : ubuf[j];
-                ubuf[j] = uv;
-                rowi[j] = uv;
+                v = (tau2 != 0.f) ? ubuf[j] * inv2 : ubuf[j];
+                ubuf[j] = v;
             }
+            store_elem(G, ldg, L, ldl, split, i, j, v);
         }
         if (tid == 0) {
             P.e[i] = beta2;
             P.taup[i] = tau2;
         }
         __syncthreads();
-        // ---- fused pass: rows > i updated, column i+1 data gathered
-        ss = fused_pass<NT>(S, M, N, i, tau, inv, tau2, cbuf, wbuf, ubuf, rowbuf, pwr);
-        __syncthreads();
     }
     __syncthreads();
-    for (int r = S.split + wv; r < M; r += NW)
-        for (int c = lane; c < N; c += 64) P.a[(size_t)r * ldg + c] = S.L[(size_t)(r - S.split) * S.ldl + c];
+    for (int r = split + wv; r < M; r += NW)
+        for (int c = lane; c < N; c += 64) G[(size_t)r * ldg + c] = L[(size_t)(r - split) * ldl + c];
 }
 
 // ---------------------------------------------------------------------------

@@ -347,13 +347,17 @@ static BatchScratch scratch_layout(int M, int N)
     return B;
 }
 
+// LDS classes (0..3: <= 1/8, 1/4, 1/2, all of the LDS) whose resident matrices take the
+// one-wavefront kernel by default
+static constexpr int kWaveMaxClassDefault = -1;
+
 // Host-side plan of a batch: a byte blob (uploaded once) + launch sizes.
 struct BatchPlan {
     std::vector<uint8_t> blob;
     size_t scratch_bytes = 0;
     int count = 0;
     // offsets into blob (bytes)
-    size_t o_probs = 0, o_src = 0, o_order_res = 0, o_info = 0;
+    size_t o_probs = 0, o_src = 0, o_info = 0;
     // streamed bidiagonalization: one launch per workgroup size (256, 512, 1024)
     size_t o_order_nt[3] = {0, 0, 0}, lds_nt[3] = {0, 0, 0};
     int n_nt[3] = {0, 0, 0};
@@ -363,8 +367,11 @@ struct BatchPlan {
     static constexpr int kWaveClasses = 4, kWaveGroups = kWaveClasses * BIDIAG_WAVE_MAXC;
     size_t o_order_wave[kWaveGroups] = {}, lds_wave[kWaveGroups] = {};
     int n_wave[kWaveGroups] = {};
-    int n_res = 0;
-    size_t lds_res = 0;
+    // one workgroup per LDS-resident matrix not taken by the wave kernel: one launch per
+    // LDS class
+    size_t o_order_res[kWaveClasses] = {}, lds_res[kWaveClasses] = {};
+    int n_res[kWaveClasses] = {};
+    int res_threads = 512;
     size_t o_rowin = 0, o_idq = 0, o_idp = 0, o_bq = 0, o_bp = 0, o_sq = 0, o_sp = 0, o_tp = 0;
     size_t o_leaf = 0, o_tu = 0, o_tv = 0, o_ru = 0, o_rv = 0;
     std::vector<size_t> o_task;
@@ -436,25 +443,44 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         return svd_cost(jobs[x].M, jobs[x].N) > svd_cost(jobs[y].M, jobs[y].N);
     });
     // a matrix's launch shape (and so its arithmetic) depends on its shape only
-    std::vector<int> og[3], orr, ow[BatchPlan::kWaveGroups];
-    bp.lds_res = 0;
+    std::vector<int> og[3], orr[BatchPlan::kWaveClasses], ow[BatchPlan::kWaveGroups];
+    // LDS class of a resident matrix: 0..3 for <= 1/8, 1/4, 1/2, all of the limit
+    auto lds_class = [](size_t bytes, size_t lim) {
+        int cl = 0;
+        while (cl < BatchPlan::kWaveClasses - 1 && bytes > (lim >> (BatchPlan::kWaveClasses - 1 - cl))) ++cl;
+        return cl;
+    };
+    // PACO_BIDIAG_WAVE=0: no one-wavefront kernel; PACO_WAVE_MAX_CLASS: the LDS classes
+    // (0..3) it takes (larger resident matrices run one workgroup each, whose several
+    // wavefronts share the CU the matrix's LDS occupies)
     const char *wenv = getenv("PACO_BIDIAG_WAVE");
-    const bool wave_on = !(wenv && wenv[0] == '0');
+    const char *cenv = getenv("PACO_WAVE_MAX_CLASS");
+    const int wave_max_class = (wenv && wenv[0] == '0') ? -1 : cenv ? atoi(cenv) : kWaveMaxClassDefault;
+    const char *tenv = getenv("PACO_RES_THREADS");
+    bp.res_threads = tenv ? atoi(tenv) : 512;
+    if (bp.res_threads != 256 && bp.res_threads != 1024) bp.res_threads = 512;
     const size_t wave_lim = bidiag_wave_lds_limit();
     for (int i : order) {
         const size_t wl = bidiag_wave_lds(jobs[i].M, jobs[i].N);
-        if (wave_on && jobs[i].N <= 64 * BIDIAG_WAVE_MAXC && wl <= wave_lim) {
-            int cl = 0;
-            while (cl < BatchPlan::kWaveClasses - 1 && wl > (wave_lim >> (BatchPlan::kWaveClasses - 1 - cl))) ++cl;
-            const int g = cl * BIDIAG_WAVE_MAXC + (cdiv(jobs[i].N, 64) - 1);
+        if (jobs[i].N <= 64 * BIDIAG_WAVE_MAXC && wl <= wave_lim && lds_class(wl, wave_lim) <= wave_max_class) {
+            const int g = lds_class(wl, wave_lim) * BIDIAG_WAVE_MAXC + (cdiv(jobs[i].N, 64) - 1);
             ow[g].push_back(i);
             bp.lds_wave[g] = std::max(bp.lds_wave[g], wl);
             continue;
         }
         const BidiagChoice c = bidiag_choice(jobs[i].M, jobs[i].N);
         if (c.resident) {
-            orr.push_back(i);
-            bp.lds_res = std::max(bp.lds_res, c.lds);
+            const size_t lr = bidiag_wg_lds_resident_bytes(jobs[i].M, jobs[i].N, bp.res_threads);
+            if (lr <= bidiag_wg_lds_limit()) {
+                const int cl = lds_class(lr, bidiag_wg_lds_limit());
+                orr[cl].push_back(i);
+                bp.lds_res[cl] = std::max(bp.lds_res[cl], lr);
+                continue;
+            }
+            // too large for LDS at the wider workgroup: the streamed launch of that size
+            og[bp.res_threads >= 1024 ? 2 : 1].push_back(i);
+            bp.lds_nt[bp.res_threads >= 1024 ? 2 : 1] = std::max(bp.lds_nt[bp.res_threads >= 1024 ? 2 : 1],
+                                                                 bidiag_wg_lds_limit());
         } else {
             const int g = c.nt >= 1024 ? 2 : c.nt >= 512 ? 1 : 0;
             og[g].push_back(i);
@@ -465,8 +491,10 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         bp.n_nt[g] = (int)og[g].size();
         bp.o_order_nt[g] = blob_put(b, og[g]);
     }
-    bp.n_res = (int)orr.size();
-    bp.o_order_res = blob_put(b, orr);
+    for (int cl = 0; cl < BatchPlan::kWaveClasses; ++cl) {
+        bp.n_res[cl] = (int)orr[cl].size();
+        bp.o_order_res[cl] = blob_put(b, orr[cl]);
+    }
     for (int g = 0; g < BatchPlan::kWaveGroups; ++g) {
         bp.n_wave[g] = (int)ow[g].size();
         bp.o_order_wave[g] = blob_put(b, ow[g]);
@@ -649,7 +677,8 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     tmark(s, 1, tm);
     // the LDS-resident matrices (one wavefront each, or one workgroup) run on a forked
     // stream, concurrently with the streamed ones
-    int n_side = bp.n_res;
+    int n_side = 0;
+    for (int cl = 0; cl < BatchPlan::kWaveClasses; ++cl) n_side += bp.n_res[cl];
     for (int g = 0; g < BatchPlan::kWaveGroups; ++g) n_side += bp.n_wave[g];
     // PACO_BIDIAG_SERIAL=1: every bidiagonalization launch on the caller's stream (schedule
     // comparisons)
@@ -665,7 +694,10 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
             if ((e = launch_bidiag_wave(pl, I(bp.o_order_wave[g]), bp.n_wave[g], g % BIDIAG_WAVE_MAXC + 1,
                                         bp.lds_wave[g], side)) != hipSuccess)
                 return e;
-        if ((e = launch_bidiag_wg(pl, I(bp.o_order_res), bp.n_res, bp.lds_res, side, 256)) != hipSuccess) return e;
+        for (int cl = BatchPlan::kWaveClasses - 1; cl >= 0; --cl)
+            if ((e = launch_bidiag_wg(pl, I(bp.o_order_res[cl]), bp.n_res[cl], bp.lds_res[cl], side,
+                                      bp.res_threads)) != hipSuccess)
+                return e;
         if (!serial && (e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
     }
     for (int g = 2; g >= 0; --g)

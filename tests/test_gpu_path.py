@@ -261,40 +261,35 @@ def test_batched_graph_replays(torch_dev):
                 assert torch.equal(x, y), (rep, i)
 
 
-def test_bidiag_variants_bitwise_equal(torch_dev):
+@pytest.mark.parametrize("threads", ["256", "512", "1024"])
+def test_bidiag_variants_bitwise_equal(torch_dev, threads):
     """The LDS-resident and the streamed one-workgroup bidiagonalization run the same
-    arithmetic in the same order: their outputs must be identical."""
+    arithmetic in the same order at one workgroup size: their outputs must be identical."""
     torch = torch_dev
     import pacosvd
     rng = np.random.default_rng(21)
     mats = mixed_batch(rng, 16, mmax=180)
-    say("variants", [m.shape for m in mats])
+    say("variants", threads, [m.shape for m in mats])
     As = [torch.from_numpy(m).cuda() for m in mats]
-    os.environ["PACO_BIDIAG_WAVE"] = "0"  # the one-workgroup kernel in both runs
-    try:
-        res = pacosvd.svd_batched(As)
-        torch.cuda.synchronize()
-    except BaseException:
-        del os.environ["PACO_BIDIAG_WAVE"]
-        raise
-    os.environ["PACO_LDS_RESIDENT_MAX"] = "0"
-    os.environ["PACO_BIDIAG_THREADS"] = "256"
-    try:
-        glb = pacosvd.svd_batched(As)
-        torch.cuda.synchronize()
-    finally:
-        del os.environ["PACO_LDS_RESIDENT_MAX"]
-        del os.environ["PACO_BIDIAG_THREADS"]
-        os.environ["PACO_BIDIAG_WAVE"] = "0"
-    del os.environ["PACO_BIDIAG_WAVE"]
+    runs = []
+    for env in ({"PACO_RES_THREADS": threads},
+                {"PACO_LDS_RESIDENT_MAX": "0", "PACO_BIDIAG_THREADS": threads}):
+        os.environ.update(env)
+        try:
+            runs.append(pacosvd.svd_batched(As))
+            torch.cuda.synchronize()
+        finally:
+            for k in env:
+                del os.environ[k]
     for i in range(len(As)):
-        for x, y in zip(res[i], glb[i]):
+        for x, y in zip(runs[0][i], runs[1][i]):
             assert torch.equal(x, y), i
 
 
 def test_bidiag_one_wavefront(torch_dev):
-    """LDS-resident matrices of a batch take the one-wavefront kernel: full SVD accuracy
-    against LAPACK, agreement with the one-workgroup kernel, edge shapes included."""
+    """LDS-resident matrices of a batch on the one-wavefront kernel (forced for every LDS
+    class): full SVD accuracy against LAPACK, agreement with the one-workgroup kernel,
+    edge shapes included."""
     torch = torch_dev
     import pacosvd
     rng = np.random.default_rng(13)
@@ -304,8 +299,12 @@ def test_bidiag_one_wavefront(torch_dev):
     mats[5][:, 1::2] = mats[5][:, 0:-1:2]  # rank-deficient: repeated columns
     say("one wavefront", shapes)
     As = [torch.from_numpy(m).cuda() for m in mats]
-    wave = pacosvd.svd_batched(As)
-    torch.cuda.synchronize()
+    os.environ["PACO_WAVE_MAX_CLASS"] = "3"
+    try:
+        wave = pacosvd.svd_batched(As)
+        torch.cuda.synchronize()
+    finally:
+        del os.environ["PACO_WAVE_MAX_CLASS"]
     os.environ["PACO_BIDIAG_WAVE"] = "0"
     try:
         wg = pacosvd.svd_batched(As)
