@@ -360,7 +360,7 @@ def main():
         traffic = tj.get("hbm_bytes_per_launch")
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel="bidiagonalization phase: bidiag_wg_kernel (main stream) + bidiag_wave_kernel (second stream)",
+                kernel="bidiagonalization phase: bidiag_wg_kernel streamed (main stream) + LDS-resident (second stream)",
                 algorithmic_bytes_per_launch=bb, launch_ms=round(ph["bidiag"], 3),
                 traffic_source="profiles/traffic_bidiag_batched_c4.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                "calibrated with the copy kernel)" if traffic else None)
