@@ -8,6 +8,7 @@
 // The columns of X left of R0 are identity columns that the block does not touch.  The
 // GEMMs are the 64 x 64 tiles of gemm_tile.hpp (precision setting honoured); their
 // descriptors travel as kernel arguments, so nothing is uploaded.
+#include <cstdlib>
 #include <cstring>
 
 #include "gemm_tile.hpp"
@@ -281,6 +282,116 @@ __global__ __launch_bounds__(GT) void qpb_gemm3_kernel(ProbList pl, float *const
     tile_gemm<MODE>(t);
 }
 
+// ---------------------------------------------------------------------------
+// Batched, FP32 matrix cores: one round of one side in a single launch.  A workgroup owns
+// a 64-column strip [c0, c0 + 64) of X[R0:, R0:] (the strips of a round touch disjoint
+// columns, so they are independent):
+//   W  = V^T X[R0:, strip]   (K = Lr in 64-row chunks; wavefront (h, kh) takes column
+//                             half h and row half kh of every chunk, the kh halves are
+//                             added through LDS)
+//   W' = T W                 (T upper triangular: H_b ... H_{b+nb-1} = I - V T V^T)
+//   X[R0:, strip] -= V W'
+// V is read from the reflectors kept in A (its unit diagonal and the zeros above made
+// explicit), staged per 64-row chunk through LDS in a coalesced order for either side.
+// ---------------------------------------------------------------------------
+__device__ inline void qpb_stage_v(float (*Vs)[33], const float *A, int N, int side, int i0, int nb, int base, int L)
+{
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int q = tid + 256 * e;
+        // Q: row r of V is a row segment of A; P: column i of V is a row segment of A
+        const int i = side == 0 ? (q & 31) : (q >> 6), rr = side == 0 ? (q >> 5) : (q & 63);
+        const int r = base + rr, g = i0 + i, unit = g + side;
+        float v = 0.f;
+        if (i < nb && r < L && r >= unit)
+            v = r == unit ? 1.f : (side == 0 ? A[(size_t)r * N + g] : A[(size_t)g * N + r]);
+        Vs[rr][i] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void qpb_strip_kernel(ProbList pl, const int *off, int side, int round)
+{
+    __shared__ float Vs[64][33];
+    __shared__ float Ts[32][33];
+    __shared__ float Wp[32][68];
+    __shared__ float W2[32][68];
+    const int pid = find_task(off, pl.count, blockIdx.x);
+    const int strip = blockIdx.x - off_at(off, pid);
+    const ProbDesc P = pl.get(pid);
+    const int N = P.N, L = side == 0 ? P.M : P.N, nref = side == 0 ? P.N : P.N - 1;
+    const int b = (nref + 31) / 32 - 1 - round;
+    if (b < 0) return;
+    const int i0 = 32 * b, nb = min(32, nref - i0), R0 = i0 + side, c0 = R0 + 64 * strip;
+    if (c0 >= L) return;
+    float *X = side == 0 ? P.u : P.vt;
+    const float *T = (side == 0 ? P.tq : P.tp) + (size_t)b * 1024;
+    const float *A = P.a;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, h = w & 1, kh = w >> 1;
+    const int l32 = lane & 31, lh = lane >> 5;
+    for (int q = tid; q < 1024; q += 256) Ts[q >> 5][q & 31] = T[q];
+    const int cx = c0 + 32 * h + l32; // this lane's column of X
+    const bool cok = cx < L;
+    floatx16 acc;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.f;
+    for (int base = R0; base < L; base += 64) {
+        __syncthreads();
+        qpb_stage_v(Vs, A, N, side, i0, nb, base, L);
+        __syncthreads();
+        float bx[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            const int r = base + 32 * kh + 2 * kk + lh;
+            bx[kk] = (cok && r < L) ? X[(size_t)r * L + cx] : 0.f;
+        }
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk)
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Vs[32 * kh + 2 * kk + lh][l32], bx[kk], acc, 0, 0, 0);
+    }
+    // W = sum of the two row halves
+    if (kh == 1) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) Wp[(j >> 2) * 8 + lh * 4 + (j & 3)][32 * h + l32] = acc[j];
+    }
+    __syncthreads();
+    if (kh == 0) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            float *p = &Wp[(j >> 2) * 8 + lh * 4 + (j & 3)][32 * h + l32];
+            *p = acc[j] + *p;
+        }
+    }
+    __syncthreads();
+    if (kh == 0) { // W' = T W, column half h
+        floatx16 a2;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) a2[j] = 0.f;
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk)
+            a2 = __builtin_amdgcn_mfma_f32_32x32x2f32(Ts[l32][2 * kk + lh], Wp[2 * kk + lh][32 * h + l32], a2, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) W2[(j >> 2) * 8 + lh * 4 + (j & 3)][32 * h + l32] = a2[j];
+    }
+    for (int base = R0; base < L; base += 64) {
+        __syncthreads();
+        qpb_stage_v(Vs, A, N, side, i0, nb, base, L);
+        __syncthreads();
+        floatx16 a3;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) a3[j] = 0.f;
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk)
+            a3 = __builtin_amdgcn_mfma_f32_32x32x2f32(Vs[32 * kh + l32][2 * kk + lh], W2[2 * kk + lh][32 * h + l32], a3, 0,
+                                                      0, 0);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int r = base + 32 * kh + (j >> 2) * 8 + lh * 4 + (j & 3);
+            if (cok && r < L) X[(size_t)r * L + cx] -= a3[j];
+        }
+    }
+}
+
 size_t qpb_scratch_floats(int M, int N) { return 2 * 96 * (size_t)(M > N ? M : N); }
 
 // one round of one side: extraction blocks, W tiles and X tiles through their offsets
@@ -288,8 +399,13 @@ hipError_t launch_qpb_round(ProbList pl, float *const *qp, int side, int round, 
                             const int *g1_off, int ng1, const int *g3_off, int ng3, hipStream_t st)
 {
     if (next <= 0) return hipSuccess;
-    hipLaunchKernelGGL(qpb_extract_kernel, dim3(next), dim3(256), 0, st, pl, qp, ext_off, side, round);
     const int mode = gemm_get_mode();
+    const char *senv = getenv("PACO_QP_STRIP");
+    if (mode == 0 && !(senv && senv[0] == '0')) { // FP32 matrix cores: one strip launch per round
+        hipLaunchKernelGGL(qpb_strip_kernel, dim3(ng1), dim3(256), 0, st, pl, g1_off, side, round);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(qpb_extract_kernel, dim3(next), dim3(256), 0, st, pl, qp, ext_off, side, round);
     if (mode == 1) {
         hipLaunchKernelGGL(qpb_gemm1_kernel<1>, dim3(ng1), dim3(GT), 0, st, pl, qp, g1_off, side, round);
         hipLaunchKernelGGL(qpb_gemm3_kernel<1>, dim3(ng3), dim3(GT), 0, st, pl, qp, g3_off, side, round);

@@ -239,6 +239,47 @@ def test_batched_matches_plain_and_alone(torch_dev):
             assert torch.equal(x, y), i
 
 
+@pytest.mark.parametrize("variant", ["strip", "rounds", "bf16x3", "fma_strips"])
+def test_batched_qp_formation_paths(torch_dev, variant):
+    """Q and P^T of a batch by every formation path (one strip launch per round on the
+    matrix cores, the extract + two tile GEMMs per round, the BF16x3 GEMMs, the 64-column
+    FMA strips): full SVD accuracy; where only the formation differs (same
+    bidiagonalization and D&C arithmetic: not BF16x3, which also changes the D&C
+    products), U and V^T within 2e-5 of the strip path."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(17)
+    mats = mixed_batch(rng, 20, mmax=330)
+    mats += [rng.standard_normal(sh).astype(np.float32) for sh in [(512, 512), (300, 33), (97, 96), (3, 1)]]
+    say("qp formation", variant, [m.shape for m in mats])
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    ref = pacosvd.svd_batched(As)
+    torch.cuda.synchronize()
+    env = {"rounds": {"PACO_QP_STRIP": "0"}, "fma_strips": {"PACO_QP_BATCH": "0"}}.get(variant, {})
+    os.environ.update(env)
+    old = pacosvd.set_precision(1) if variant == "bf16x3" else None
+    try:
+        out = pacosvd.svd_batched(As)
+        torch.cuda.synchronize()
+    finally:
+        for k in env:
+            del os.environ[k]
+        if old is not None:
+            pacosvd.set_precision(old)
+    assert pacosvd.last_info(len(As)) == [0] * len(As)
+    for i, A in enumerate(mats):
+        s, U, Vt = (np_of(x) for x in out[i])
+        m = measures(A, s, U, Vt)
+        assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL, (i, A.shape, m)
+        assert m["resid_ratio"] <= RESID_TOL, (i, A.shape, m)
+        if variant == "bf16x3":
+            continue
+        N = A.shape[1]
+        U0, V0 = np_of(ref[i][1]), np_of(ref[i][2])
+        assert np.max(np.abs(U[:, :N] - U0[:, :N])) <= 2e-5, (i, A.shape)
+        assert np.max(np.abs(Vt - V0)) <= 2e-5, (i, A.shape)
+
+
 def test_batched_graph_replays(torch_dev):
     torch = torch_dev
     import pacosvd
