@@ -692,16 +692,20 @@ __global__ __launch_bounds__(BT) void wy_apply_kernel(ProbList pl, const int *st
     }
 }
 
-// X = identity (L x L), grid over (problem, row) through offsets.
+// X = identity (L x L), grid over (problem, group of IDENTITY_ROWS rows) through offsets.
 __global__ __launch_bounds__(BT) void identity_kernel(ProbList pl, const int *row_off, int side)
 {
     const int pid = find_task(row_off, pl.count, blockIdx.x);
-    const int r = blockIdx.x - off_at(row_off, pid);
+    const int g = blockIdx.x - off_at(row_off, pid);
     const ProbDesc P = pl.get(pid);
     const int L = side == 0 ? P.M : P.N;
     float *X = side == 0 ? P.u : P.vt;
-    if (r >= L) return;
-    for (int c = threadIdx.x; c < L; c += BT) X[(size_t)r * L + c] = (c == r) ? 1.f : 0.f;
+    const int r0 = g * IDENTITY_ROWS, nr = min(IDENTITY_ROWS, L - r0);
+    if (nr <= 0) return;
+    for (int q = threadIdx.x; q < nr * L; q += BT) {
+        const int r = r0 + q / L, c = q % L;
+        X[(size_t)r * L + c] = (c == r) ? 1.f : 0.f;
+    }
 }
 
 // In-place transpose of vt (N x N) by 32x32 tile pairs; grid over (problem, tile pair).
@@ -820,10 +824,10 @@ hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int s
     return hipGetLastError();
 }
 
-hipError_t launch_identity(ProbList pl, const int *row_off, int nrows, int side, hipStream_t s)
+hipError_t launch_identity(ProbList pl, const int *row_off, int ngroups, int side, hipStream_t s)
 {
-    if (nrows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(identity_kernel, dim3(nrows), dim3(BT), 0, s, pl, row_off, side);
+    if (ngroups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(identity_kernel, dim3(ngroups), dim3(BT), 0, s, pl, row_off, side);
     return hipGetLastError();
 }
 

@@ -212,8 +212,8 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     const int nt = cdiv(N, 32);
     if (large_matrix(M, N)) {
         // blocks of 32 reflectors applied to the whole trailing part of Q, P as GEMMs
-        if ((err = launch_identity(pl, nullptr, M, 0, s)) != hipSuccess) return (int)err;
-        if ((err = launch_identity(pl, nullptr, N, 1, s)) != hipSuccess) return (int)err;
+        if ((err = launch_identity(pl, nullptr, cdiv(M, IDENTITY_ROWS), 0, s)) != hipSuccess) return (int)err;
+        if ((err = launch_identity(pl, nullptr, cdiv(N, IDENTITY_ROWS), 1, s)) != hipSuccess) return (int)err;
         if ((err = launch_tfactor(pl, nullptr, cdiv(N, 32), 0, s)) != hipSuccess) return (int)err;
         if ((err = launch_tfactor(pl, nullptr, cdiv(std::max(N - 1, 0), 32), 1, s)) != hipSuccess) return (int)err;
         if ((err = launch_qp_gemm(pl.single, 0, ws + L.qp, s)) != hipSuccess) return (int)err;
@@ -224,7 +224,7 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     }
     // a single matrix: 16-column strips unless 64-column ones already give every CU a strip
     const int sw = cdiv(M, 64) >= 512 ? 64 : 16;
-    err = form_q_pt(pl, nullptr, M, nullptr, N, nullptr, cdiv(N, 32), nullptr, cdiv(std::max(N - 1, 0), 32), nullptr,
+    err = form_q_pt(pl, nullptr, cdiv(M, IDENTITY_ROWS), nullptr, cdiv(N, IDENTITY_ROWS), nullptr, cdiv(N, 32), nullptr, cdiv(std::max(N - 1, 0), 32), nullptr,
                     cdiv(M, sw), nullptr, cdiv(N, sw), nullptr, nt * (nt + 1) / 2, s, sw);
     tmark(s, 3, tm);
     return (int)err;
@@ -500,8 +500,8 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         bp.o_order_wave[g] = blob_put(b, ow[g]);
     }
     bp.o_rowin = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nrowin));
-    bp.o_idq = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nidq));
-    bp.o_idp = blob_put(b, prefix(count, [&](int i) { return jobs[i].N; }, bp.nidp));
+    bp.o_idq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, IDENTITY_ROWS); }, bp.nidq));
+    bp.o_idp = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, IDENTITY_ROWS); }, bp.nidp));
     {
         std::vector<float *> qpp(count, nullptr); // device addresses patched at upload
         bp.o_qp = blob_put(b, qpp);

@@ -65,7 +65,9 @@ hipError_t launch_bidiag_wave(ProbList pl, const int *order, int count, int C, s
 hipError_t launch_tfactor(ProbList pl, const int *blk_off, int nblocks, int side, hipStream_t s);
 // strips of sw columns (64 or 16); strip_off counts strips of that width
 hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int side, hipStream_t s, int sw = 64);
-hipError_t launch_identity(ProbList pl, const int *row_off, int nrows, int side, hipStream_t s);
+// X = I; row_off counts groups of IDENTITY_ROWS rows per problem
+constexpr int IDENTITY_ROWS = 8;
+hipError_t launch_identity(ProbList pl, const int *row_off, int ngroups, int side, hipStream_t s);
 hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hipStream_t s);
 
 // bidiag_grid.hip (large single matrices)

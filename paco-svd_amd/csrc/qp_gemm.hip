@@ -294,19 +294,41 @@ __global__ __launch_bounds__(GT) void qpb_gemm3_kernel(ProbList pl, float *const
 // V is read from the reflectors kept in A (its unit diagonal and the zeros above made
 // explicit), staged per 64-row chunk through LDS in a coalesced order for either side.
 // ---------------------------------------------------------------------------
-__device__ inline void qpb_stage_v(float (*Vs)[33], const float *A, int N, int side, int i0, int nb, int base, int L)
+// This thread's eight elements of the 64-row chunk of V at row base (Q: a row of V is a
+// row segment of A; P: a column of V is a row segment of A; both read coalesced), and
+// their place in the LDS tile.
+__device__ inline void qpb_load_v(float (&v)[8], const float *A, int N, int side, int i0, int nb, int base, int L)
 {
     const int tid = threadIdx.x;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int q = tid + 256 * e;
-        // Q: row r of V is a row segment of A; P: column i of V is a row segment of A
         const int i = side == 0 ? (q & 31) : (q >> 6), rr = side == 0 ? (q >> 5) : (q & 63);
         const int r = base + rr, g = i0 + i, unit = g + side;
-        float v = 0.f;
+        float x = 0.f;
         if (i < nb && r < L && r >= unit)
-            v = r == unit ? 1.f : (side == 0 ? A[(size_t)r * N + g] : A[(size_t)g * N + r]);
-        Vs[rr][i] = v;
+            x = r == unit ? 1.f : (side == 0 ? A[(size_t)r * N + g] : A[(size_t)g * N + r]);
+        v[e] = x;
+    }
+}
+__device__ inline void qpb_store_v(float (*Vs)[33], const float (&v)[8], int side)
+{
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int q = tid + 256 * e;
+        const int i = side == 0 ? (q & 31) : (q >> 6), rr = side == 0 ? (q >> 5) : (q & 63);
+        Vs[rr][i] = v[e];
+    }
+}
+
+// the lane's 16 entries of column cx of X in the chunk at row base (rows base + r0 + 2 kk)
+__device__ inline void qpb_load_x(float (&x)[16], const float *X, int L, int cx, bool cok, int base, int r0)
+{
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+        const int r = base + r0 + 2 * kk;
+        x[kk] = (cok && r < L) ? X[(size_t)r * L + cx] : 0.f;
     }
 }
 
@@ -329,26 +351,34 @@ __global__ __launch_bounds__(256) void qpb_strip_kernel(ProbList pl, const int *
     const float *A = P.a;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, h = w & 1, kh = w >> 1;
     const int l32 = lane & 31, lh = lane >> 5;
-    for (int q = tid; q < 1024; q += 256) Ts[q >> 5][q & 31] = T[q];
     const int cx = c0 + 32 * h + l32; // this lane's column of X
     const bool cok = cx < L;
+    // the chunk's loads are issued one chunk ahead: V (staged through LDS) and X
+    float vn[8], xn[16];
+    qpb_load_v(vn, A, N, side, i0, nb, R0, L);
+    qpb_load_x(xn, X, L, cx, cok, R0, 32 * kh + lh);
+    for (int q = tid; q < 1024; q += 256) Ts[q >> 5][q & 31] = T[q];
     floatx16 acc;
 #pragma unroll
     for (int j = 0; j < 16; ++j) acc[j] = 0.f;
     for (int base = R0; base < L; base += 64) {
         __syncthreads();
-        qpb_stage_v(Vs, A, N, side, i0, nb, base, L);
-        __syncthreads();
+        qpb_store_v(Vs, vn, side);
         float bx[16];
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            const int r = base + 32 * kh + 2 * kk + lh;
-            bx[kk] = (cok && r < L) ? X[(size_t)r * L + cx] : 0.f;
+        for (int kk = 0; kk < 16; ++kk) bx[kk] = xn[kk];
+        if (base + 64 < L) {
+            qpb_load_v(vn, A, N, side, i0, nb, base + 64, L);
+            qpb_load_x(xn, X, L, cx, cok, base + 64, 32 * kh + lh);
         }
+        __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk)
             acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Vs[32 * kh + 2 * kk + lh][l32], bx[kk], acc, 0, 0, 0);
     }
+    // first chunk of the update, loaded while W is reduced; the update's X entries of a
+    // lane are rows base + 32 kh + 8 (j / 4) + 4 lh + j % 4 (the accumulator layout)
+    qpb_load_v(vn, A, N, side, i0, nb, R0, L);
     // W = sum of the two row halves
     if (kh == 1) {
 #pragma unroll
@@ -375,7 +405,14 @@ __global__ __launch_bounds__(256) void qpb_strip_kernel(ProbList pl, const int *
     }
     for (int base = R0; base < L; base += 64) {
         __syncthreads();
-        qpb_stage_v(Vs, A, N, side, i0, nb, base, L);
+        qpb_store_v(Vs, vn, side);
+        if (base + 64 < L) qpb_load_v(vn, A, N, side, i0, nb, base + 64, L);
+        float xo[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int r = base + 32 * kh + (j >> 2) * 8 + lh * 4 + (j & 3);
+            xo[j] = (cok && r < L) ? X[(size_t)r * L + cx] : 0.f;
+        }
         __syncthreads();
         floatx16 a3;
 #pragma unroll
@@ -387,7 +424,7 @@ __global__ __launch_bounds__(256) void qpb_strip_kernel(ProbList pl, const int *
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const int r = base + 32 * kh + (j >> 2) * 8 + lh * 4 + (j & 3);
-            if (cok && r < L) X[(size_t)r * L + cx] -= a3[j];
+            if (cok && r < L) X[(size_t)r * L + cx] = xo[j] - a3[j];
         }
     }
 }
