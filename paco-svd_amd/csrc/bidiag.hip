@@ -692,7 +692,7 @@ __global__ __launch_bounds__(BT) void wy_apply_kernel(ProbList pl, const int *st
     }
 }
 
-// X = identity (L x L), grid over (problem, group of IDENTITY_ROWS rows) through offsets.
+// X = identity (L x L), grid over (problem, group of ROW_GROUP rows) through offsets.
 __global__ __launch_bounds__(BT) void identity_kernel(ProbList pl, const int *row_off, int side)
 {
     const int pid = find_task(row_off, pl.count, blockIdx.x);
@@ -700,7 +700,7 @@ __global__ __launch_bounds__(BT) void identity_kernel(ProbList pl, const int *ro
     const ProbDesc P = pl.get(pid);
     const int L = side == 0 ? P.M : P.N;
     float *X = side == 0 ? P.u : P.vt;
-    const int r0 = g * IDENTITY_ROWS, nr = min(IDENTITY_ROWS, L - r0);
+    const int r0 = g * ROW_GROUP, nr = min(ROW_GROUP, L - r0);
     if (nr <= 0) return;
     for (int q = threadIdx.x; q < nr * L; q += BT) {
         const int r = r0 + q / L, c = q % L;

@@ -730,7 +730,7 @@ __global__ __launch_bounds__(DT) void dc_finalize_kernel(ProbList pl, LevelArgs 
 // side 0: coef = left secular vectors (in a), basis = gathered left bases, out qt;
 // side 1: coef = right secular vectors (vmt), basis = gathered right bases, out wt.
 // ---------------------------------------------------------------------------
-template <int MODE>
+template <int MODE, int TKV>
 __global__ __launch_bounds__(GT) void dc_merge_gemm_kernel(ProbList pl, LevelArgs la, int side)
 {
     const int tpt = (la.stride + 63) / 64, tiles = tpt * tpt;
@@ -757,7 +757,7 @@ __global__ __launch_bounds__(GT) void dc_merge_gemm_kernel(ProbList pl, LevelArg
     t.m0 = (tile / tpt) * 64;
     t.n0 = (tile % tpt) * 64;
     if (t.m0 >= t.m || t.n0 >= t.n) return;
-    tile_gemm<MODE>(t);
+    tile_gemm<MODE, TKV>(t);
 }
 
 // back-transforms (updateUVt, svd.cpp:1911-1981):
@@ -802,19 +802,25 @@ __global__ __launch_bounds__(GT) void dc_bt_gemm_kernel(ProbList pl, const int *
 }
 
 // copy of the back-transform results into u[:, :N] (side 0) and vt (side 1)
+// rows [ROW_GROUP g, ROW_GROUP (g + 1)) of the back-transformed vectors into U (its first
+// N columns) or Vt
 __global__ __launch_bounds__(DT) void dc_bt_copy_kernel(ProbList pl, const int *row_off, int side)
 {
     const int pid = find_task(row_off, pl.count, blockIdx.x);
-    const int r = blockIdx.x - off_at(row_off, pid);
+    const int g = blockIdx.x - off_at(row_off, pid);
     const ProbDesc P = pl.get(pid);
-    const int M = P.M, N = P.N;
+    const int M = P.M, N = P.N, r0 = g * ROW_GROUP;
+    const int nr = min(ROW_GROUP, (side == 0 ? M : N) - r0);
+    if (nr <= 0) return;
     if (side == 0) {
-        if (r >= M) return;
-        for (int c = threadIdx.x; c < N; c += DT) P.u[(size_t)r * M + c] = P.a[(size_t)r * N + c];
+        for (int q = threadIdx.x; q < nr * N; q += DT) {
+            const int r = r0 + q / N, c = q % N;
+            P.u[(size_t)r * M + c] = P.a[(size_t)r * N + c];
+        }
     } else {
-        if (r >= N) return;
-        const float *src = P.qt + (size_t)N * N;
-        for (int c = threadIdx.x; c < N; c += DT) P.vt[(size_t)r * N + c] = src[(size_t)r * N + c];
+        const float *src = P.qt + (size_t)N * N + (size_t)r0 * N;
+        float *dst = P.vt + (size_t)r0 * N;
+        for (int q = threadIdx.x; q < nr * N; q += DT) dst[q] = src[q];
     }
 }
 
@@ -833,8 +839,14 @@ static void launch_merge_gemms(ProbList pl, LevelArgs la, hipStream_t s)
 {
     const int tpt = (la.stride + 63) / 64;
     const int grid = la.ntask * tpt * tpt;
-    hipLaunchKernelGGL(dc_merge_gemm_kernel<MODE>, dim3(grid), dim3(GT), 0, s, pl, la, 0);
-    hipLaunchKernelGGL(dc_merge_gemm_kernel<MODE>, dim3(grid), dim3(GT), 0, s, pl, la, 1);
+    // K <= stride: steps of 16 for the short products of the first rounds
+    if (la.stride <= 48) {
+        hipLaunchKernelGGL((dc_merge_gemm_kernel<MODE, 16>), dim3(grid), dim3(GT), 0, s, pl, la, 0);
+        hipLaunchKernelGGL((dc_merge_gemm_kernel<MODE, 16>), dim3(grid), dim3(GT), 0, s, pl, la, 1);
+    } else {
+        hipLaunchKernelGGL((dc_merge_gemm_kernel<MODE, 32>), dim3(grid), dim3(GT), 0, s, pl, la, 0);
+        hipLaunchKernelGGL((dc_merge_gemm_kernel<MODE, 32>), dim3(grid), dim3(GT), 0, s, pl, la, 1);
+    }
 }
 
 hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s)

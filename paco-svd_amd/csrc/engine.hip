@@ -33,15 +33,18 @@ __global__ void zero_ints_kernel(int *p, int n)
 }
 
 // a <- A (row-major M x N) for every problem; grid over (problem, row).
+// rows [ROW_GROUP g, ROW_GROUP (g + 1)) of every input into its working copy
 __global__ __launch_bounds__(256) void copy_in_kernel(ProbList pl, const float *const *src, const int *row_off)
 {
     const int pid = find_task(row_off, pl.count, blockIdx.x);
-    const int r = blockIdx.x - off_at(row_off, pid);
+    const int g = blockIdx.x - off_at(row_off, pid);
     const ProbDesc P = pl.get(pid);
-    if (r >= P.M) return;
-    const float *s = src[pid] + (size_t)r * P.N;
-    float *d = P.a + (size_t)r * P.N;
-    for (int c = threadIdx.x; c < P.N; c += 256) d[c] = s[c];
+    const int r0 = g * ROW_GROUP, nr = min(ROW_GROUP, P.M - r0);
+    if (nr <= 0) return;
+    // the rows of a group are contiguous in both (leading dimension N)
+    const float *s = src[pid] + (size_t)r0 * P.N;
+    float *d = P.a + (size_t)r0 * P.N;
+    for (int q = threadIdx.x; q < nr * P.N; q += 256) d[q] = s[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -212,8 +215,8 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     const int nt = cdiv(N, 32);
     if (large_matrix(M, N)) {
         // blocks of 32 reflectors applied to the whole trailing part of Q, P as GEMMs
-        if ((err = launch_identity(pl, nullptr, cdiv(M, IDENTITY_ROWS), 0, s)) != hipSuccess) return (int)err;
-        if ((err = launch_identity(pl, nullptr, cdiv(N, IDENTITY_ROWS), 1, s)) != hipSuccess) return (int)err;
+        if ((err = launch_identity(pl, nullptr, cdiv(M, ROW_GROUP), 0, s)) != hipSuccess) return (int)err;
+        if ((err = launch_identity(pl, nullptr, cdiv(N, ROW_GROUP), 1, s)) != hipSuccess) return (int)err;
         if ((err = launch_tfactor(pl, nullptr, cdiv(N, 32), 0, s)) != hipSuccess) return (int)err;
         if ((err = launch_tfactor(pl, nullptr, cdiv(std::max(N - 1, 0), 32), 1, s)) != hipSuccess) return (int)err;
         if ((err = launch_qp_gemm(pl.single, 0, ws + L.qp, s)) != hipSuccess) return (int)err;
@@ -224,7 +227,7 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     }
     // a single matrix: 16-column strips unless 64-column ones already give every CU a strip
     const int sw = cdiv(M, 64) >= 512 ? 64 : 16;
-    err = form_q_pt(pl, nullptr, cdiv(M, IDENTITY_ROWS), nullptr, cdiv(N, IDENTITY_ROWS), nullptr, cdiv(N, 32), nullptr, cdiv(std::max(N - 1, 0), 32), nullptr,
+    err = form_q_pt(pl, nullptr, cdiv(M, ROW_GROUP), nullptr, cdiv(N, ROW_GROUP), nullptr, cdiv(N, 32), nullptr, cdiv(std::max(N - 1, 0), 32), nullptr,
                     cdiv(M, sw), nullptr, cdiv(N, sw), nullptr, nt * (nt + 1) / 2, s, sw);
     tmark(s, 3, tm);
     return (int)err;
@@ -298,8 +301,8 @@ static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, floa
     }
     o.ntu = cdiv(M, 64) * cdiv(N, 64);
     o.ntv = cdiv(N, 64) * cdiv(N, 64);
-    o.nru = M;
-    o.nrv = N;
+    o.nru = cdiv(M, ROW_GROUP);
+    o.nrv = cdiv(N, ROW_GROUP);
     const uint32_t *blob = nullptr;
     if (tiling_dev) blob = reinterpret_cast<const uint32_t *>(static_cast<const char *>(tiling_dev) + 64);
     const bool tm = g_timing;
@@ -499,9 +502,9 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         bp.n_wave[g] = (int)ow[g].size();
         bp.o_order_wave[g] = blob_put(b, ow[g]);
     }
-    bp.o_rowin = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nrowin));
-    bp.o_idq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, IDENTITY_ROWS); }, bp.nidq));
-    bp.o_idp = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, IDENTITY_ROWS); }, bp.nidp));
+    bp.o_rowin = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, ROW_GROUP); }, bp.nrowin));
+    bp.o_idq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, ROW_GROUP); }, bp.nidq));
+    bp.o_idp = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, ROW_GROUP); }, bp.nidp));
     {
         std::vector<float *> qpp(count, nullptr); // device addresses patched at upload
         bp.o_qp = blob_put(b, qpp);
@@ -558,8 +561,8 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     }
     bp.o_tu = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, 64) * cdiv(jobs[i].N, 64); }, bp.ntu));
     bp.o_tv = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, 64) * cdiv(jobs[i].N, 64); }, bp.ntv));
-    bp.o_ru = blob_put(b, prefix(count, [&](int i) { return jobs[i].M; }, bp.nru));
-    bp.o_rv = blob_put(b, prefix(count, [&](int i) { return jobs[i].N; }, bp.nrv));
+    bp.o_ru = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, ROW_GROUP); }, bp.nru));
+    bp.o_rv = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, ROW_GROUP); }, bp.nrv));
     std::vector<MergeState> states(std::max<size_t>(sacc, 1));
     bp.o_info = blob_put(b, std::vector<int>(count, 0));
     // merge states live after the blob in device memory (not uploaded)

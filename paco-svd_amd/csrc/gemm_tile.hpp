@@ -3,7 +3,9 @@
 // back-transforms (dc.hip).
 //
 // Workgroup of 256 threads = four wavefronts, each owning a 32x32 sub-tile.
-// K advances in steps of 16 through LDS-staged 64x16 (A) and 16x64 (B) tiles.
+// K advances in steps of TK (32, or 16 for short K) through LDS-staged 64xTK (A) and
+// TKx64 (B) tiles; the operands of the next step are loaded into registers while the
+// current step's products run.
 //  mode 0: v_mfma_f32_32x32x2_f32 (FP32 matrix cores)
 //  mode 1: BF16x3 compensated: operands split x = h + m + l in bf16, six products
 //          with v_mfma_f32_32x32x8_bf16 accumulated in FP32 (smallest terms first)
@@ -17,7 +19,6 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef short shortx4 __attribute__((ext_vector_type(4)));
 
 #define GT 256
-#define TK 16
 
 struct TileCtx {
     const float *A, *B;
@@ -28,11 +29,20 @@ struct TileCtx {
     int m0, n0;
 };
 
-__device__ inline void tile_load(const TileCtx &t, int k0, float (*As)[64 + 4], float (*Bs)[64 + 4])
+// elements of each operand per thread and K step
+template <int TK>
+struct TileEl {
+    static constexpr int n = (64 * TK) / GT;
+};
+
+// operands of the K step at k0 into registers (zero outside the matrices)
+template <int TK>
+__device__ inline void tile_fetch(const TileCtx &t, int k0, float (&ra)[TileEl<TK>::n], float (&rb)[TileEl<TK>::n])
 {
+    constexpr int TILE_EL = TileEl<TK>::n;
     const int tid = threadIdx.x;
 #pragma unroll
-    for (int e = 0; e < (64 * TK) / GT; ++e) {
+    for (int e = 0; e < TILE_EL; ++e) {
         const int q = tid + e * GT;
         int r, kk;
         if (t.transA) {
@@ -45,10 +55,10 @@ __device__ inline void tile_load(const TileCtx &t, int k0, float (*As)[64 + 4], 
         const int gr = t.m0 + r, gk = k0 + kk;
         float v = 0.f;
         if (gr < t.m && gk < t.K) v = t.transA ? t.A[(size_t)gk * t.lda + gr] : t.A[(size_t)gr * t.lda + gk];
-        As[kk][r] = v;
+        ra[e] = v;
     }
 #pragma unroll
-    for (int e = 0; e < (64 * TK) / GT; ++e) {
+    for (int e = 0; e < TILE_EL; ++e) {
         const int q = tid + e * GT;
         int c, kk;
         if (t.transB) {
@@ -64,7 +74,28 @@ __device__ inline void tile_load(const TileCtx &t, int k0, float (*As)[64 + 4], 
             const int krow = t.brow ? t.brow[gk] : gk;
             v = t.transB ? t.B[(size_t)gc * t.ldb + krow] : t.B[(size_t)krow * t.ldb + gc];
         }
-        Bs[kk][c] = v;
+        rb[e] = v;
+    }
+}
+
+// registers of tile_fetch into the LDS tiles As[k][row], Bs[k][col]
+template <int TK>
+__device__ inline void tile_put(const TileCtx &t, const float (&ra)[TileEl<TK>::n], const float (&rb)[TileEl<TK>::n],
+                                float (*As)[64 + 4], float (*Bs)[64 + 4])
+{
+    constexpr int TILE_EL = TileEl<TK>::n;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < TILE_EL; ++e) {
+        const int q = tid + e * GT;
+        if (t.transA)
+            As[q / 64][q % 64] = ra[e];
+        else
+            As[q % TK][q / TK] = ra[e];
+        if (t.transB)
+            Bs[q % TK][q / TK] = rb[e];
+        else
+            Bs[q / 64][q % 64] = rb[e];
     }
 }
 
@@ -98,18 +129,21 @@ __device__ inline void split3(float x, unsigned short &h, unsigned short &m, uns
 }
 
 // Whole tile: loads, products, stores.  Every thread of the workgroup must call it.
-template <int MODE>
+template <int MODE, int TK = 32>
 __device__ inline void tile_gemm(const TileCtx &t)
 {
     __shared__ float As[TK][64 + 4];
     __shared__ float Bs[TK][64 + 4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
+    float ra[TileEl<TK>::n], rb[TileEl<TK>::n];
+    tile_fetch<TK>(t, 0, ra, rb);
     if (MODE == 2) {
         const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
         float acc[4][4] = {};
         for (int k0 = 0; k0 < t.K; k0 += TK) {
-            tile_load(t, k0, As, Bs);
+            tile_put<TK>(t, ra, rb, As, Bs);
             __syncthreads();
+            if (k0 + TK < t.K) tile_fetch<TK>(t, k0 + TK, ra, rb);
 #pragma unroll
             for (int kk = 0; kk < TK; ++kk)
 #pragma unroll
@@ -128,8 +162,9 @@ __device__ inline void tile_gemm(const TileCtx &t)
 #pragma unroll
     for (int j = 0; j < 16; ++j) acc[j] = 0.f;
     for (int k0 = 0; k0 < t.K; k0 += TK) {
-        tile_load(t, k0, As, Bs);
+        tile_put<TK>(t, ra, rb, As, Bs);
         __syncthreads();
+        if (k0 + TK < t.K) tile_fetch<TK>(t, k0 + TK, ra, rb);
         if (MODE == 0) {
 #pragma unroll
             for (int kk = 0; kk < TK; kk += 2) {

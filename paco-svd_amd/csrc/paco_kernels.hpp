@@ -65,8 +65,10 @@ hipError_t launch_bidiag_wave(ProbList pl, const int *order, int count, int C, s
 hipError_t launch_tfactor(ProbList pl, const int *blk_off, int nblocks, int side, hipStream_t s);
 // strips of sw columns (64 or 16); strip_off counts strips of that width
 hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int side, hipStream_t s, int sw = 64);
-// X = I; row_off counts groups of IDENTITY_ROWS rows per problem
-constexpr int IDENTITY_ROWS = 8;
+// Row-wise kernels (identity, copies) take ROW_GROUP rows per workgroup: one workgroup
+// per row made the launch bound by workgroup dispatch.  Their offsets count groups.
+constexpr int ROW_GROUP = 8;
+// X = I; row_off counts groups of ROW_GROUP rows per problem
 hipError_t launch_identity(ProbList pl, const int *row_off, int ngroups, int side, hipStream_t s);
 hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hipStream_t s);
 
