@@ -29,24 +29,10 @@ namespace paco {
 //   w_{i+1} = tau (A'[i+1, :] + (1/v_1) sum_{r > i+1} c_r A'[r, :]).
 // Rows [split, M) are kept in LDS (all rows when the matrix fits), the rest is
 // streamed; every reduction runs in an order that does not depend on the split.
-// LDS: FP64 partial sums [NW + 2], [cached rows, leading dimension N|1] cbuf[M] wbuf[N]
-//      ubuf[N] rowbuf[N] pw[N] pwr[NW][N] (+ spare).
+// Sums of squares of the Householder vectors are taken in FP64 (see house_params_d).
+// LDS: FP64 partial sums [2 NW + 2], [cached rows, leading dimension N|1] cbuf[M] wbuf[N]
+//      ubuf[N] rowbuf[N] (spare N) pwr[NW][N] (+ spare).
 // ---------------------------------------------------------------------------
-// sums of squares of the Householder vectors are taken in FP64 (see house_params_d)
-template <int NT>
-__device__ inline double block_sum_d(double v, double *dred)
-{
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    v = wave_sum(v);
-    __syncthreads();
-    if (lane == 0) dred[wv] = v;
-    __syncthreads();
-    double r = 0.0;
-#pragma unroll
-    for (int w = 0; w < NT / 64; ++w) r += dred[w];
-    return r;
-}
-
 // LDS- and global-typed pointers: accesses through them compile to LDS (ds_*) and
 // global (global_*) instructions, never to FLAT ones that must wait on both counters.
 typedef __attribute__((address_space(3))) float lds_f32;
@@ -61,10 +47,10 @@ constexpr int WG_SPLIT_ALIGN = 8;
 #define RQ1024_8 1
 #endif
 
-// (the first 2 (nt / 64 + 2) floats hold FP64 partial sums)
+// (the first 2 (2 nt / 64 + 2) floats hold FP64 partial sums and a broadcast slot)
 __host__ __device__ inline size_t bidiag_vec_floats(int M, int N, int nt)
 {
-    return (size_t)M + 5 * (size_t)N + (size_t)(nt / 64) * N + nt + 8 + 2 * (size_t)(nt / 64 + 2);
+    return (size_t)M + 5 * (size_t)N + (size_t)(nt / 64) * N + nt + 8 + 2 * (size_t)(2 * (nt / 64) + 2);
 }
 
 // rows of A that fit in lds_bytes of dynamic LDS next to the vectors
@@ -87,11 +73,10 @@ __host__ __device__ inline int bidiag_split_row(int M, int N, int nt, size_t lds
 
 // rows per group of the fused pass (one DPP reduction in flight per row; the next
 // group is loaded while this one is processed), chosen so that the two groups stay in
-// registers: at 1024 threads a lane has 128 VGPRs
+// registers: every workgroup size runs at 4 wavefronts per SIMD, 128 VGPRs per lane
 template <int NT, int C>
 struct PassRows {
-    static constexpr int RQ = NT >= 1024 ? (C <= 2 ? RQ1024_2 : C <= 4 ? RQ1024_4 : RQ1024_8)
-                                         : (C <= 2 ? 8 : C <= 4 ? 4 : 2);
+    static constexpr int RQ = C <= 2 ? RQ1024_2 : C <= 4 ? RQ1024_4 : RQ1024_8;
 };
 
 // Chunk c of a lane holds column k + lane + 64 c; with C = ceil((N - k) / 64) only the
@@ -265,8 +250,12 @@ __device__ inline void store_elem(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int 
         L[(size_t)(r - split) * ldl + j] = v;
 }
 
+// 4 wavefronts per SIMD whatever NT: up to 4 workgroups of 256 threads or 2 of 512 share
+// a CU when their LDS allows it
 template <int NT>
-__global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *order, unsigned lds_bytes)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bidiag_wg_kernel(ProbList pl,
+                                                                                               const int *order,
+                                                                                               unsigned lds_bytes)
 {
     constexpr int NW = NT / 64;
     const int pid = order ? order[blockIdx.x] : blockIdx.x;
@@ -274,44 +263,38 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
     const int M = P.M, N = P.N, ldg = P.ld ? P.ld : P.N;
     extern __shared__ float smem[];
     lds_f32 *lsm = (lds_f32 *)smem;
-    double *dred = reinterpret_cast<double *>(smem); // [NW] partials, [NW] sigma
-    lds_f32 *sm = lsm + 2 * (NW + 2);
+    // FP64: [NW] sum-of-squares partials of the pass, [NW] of the row reflector, [1] x1 of
+    // the row reflector
+    double *dred = reinterpret_cast<double *>(smem);
+    lds_f32 *sm = lsm + 2 * (2 * NW + 2);
     const int split = bidiag_split_row(M, N, NT, lds_bytes), nl = M - split, ldl = N | 1;
     glb_f32 *G = (glb_f32 *)P.a;
     lds_f32 *L = sm;
-    lds_f32 *cbuf = sm + (size_t)nl * ldl, *wbuf = cbuf + M, *ubuf = wbuf + N, *rowbuf = ubuf + N, *pw = rowbuf + N;
-    lds_f32 *pwr = pw + N;
+    lds_f32 *cbuf = sm + (size_t)nl * ldl, *wbuf = cbuf + M, *ubuf = wbuf + N, *rowbuf = ubuf + N;
+    lds_f32 *pwr = rowbuf + 2 * N;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int r = split + wv; r < M; r += NW)
         for (int c = lane; c < N; c += 64) L[(size_t)(r - split) * ldl + c] = G[(size_t)r * ldg + c];
     __syncthreads();
 
     // pass i = -1 is the read-only gathering pass of column 0; pass i >= 0 applies the
-    // reflectors of column i to rows > i and gathers column i + 1
+    // reflectors of column i to rows > i and gathers column i + 1.  Per column three
+    // barriers: after the pass; after the row reflector's partial sums; after its vector.
+    // Every thread forms both reflectors' parameters from the same sums in the same order.
+    constexpr int JPT = (WG_MAXN + NT - 1) / NT; // columns of row i per thread
     float tau = 0.f, inv = 1.f, tau2 = 0.f;
     for (int i = -1;;) {
         const double ss = fused_pass<NT>(G, ldg, L, ldl, split, M, N, i, tau, inv, tau2, cbuf, wbuf, ubuf, rowbuf, pwr);
         ++i;
-        // ---- reduce the pass partials: sum c_r^2 and pw (no reader of dred is left
-        // since the barrier before the pass)
+        // the previous column's readers of dred finished before the barrier ahead of the pass
         if (lane == 0) dred[wv] = ss;
         __syncthreads();
-        for (int j = i + 1 + tid; j < N; j += NT) {
-            float a = 0.f;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) a += pwr[(size_t)w * N + j];
-            pw[j] = a;
-        }
-        if (tid == 0) {
-            double sg = 0.0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) sg += dred[w];
-            dred[NW] = sg;
-        }
-        __syncthreads();
         // ---- column reflector of column i
+        double sg = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) sg += dred[w];
         float beta;
-        house_params_d(rowbuf[i], dred[NW], &beta, &tau, &inv);
+        house_params_d(rowbuf[i], sg, &beta, &tau, &inv);
         if (tid == 0) {
             store_elem(G, ldg, L, ldl, split, i, i, beta);
             P.d[i] = beta;
@@ -322,29 +305,42 @@ __global__ __launch_bounds__(NT) void bidiag_wg_kernel(ProbList pl, const int *o
                 store_elem(G, ldg, L, ldl, split, r, i, (tau != 0.f) ? cbuf[r] * inv : cbuf[r]);
             break;
         }
-        // ---- w and the updated row i; row reflector
+        // ---- w = tau (row i + (1/v1) pw) and y = row i - w on this thread's columns
+        float yv[JPT];
         double part = 0.0;
-        for (int j = i + 1 + tid; j < N; j += NT) {
-            const float w = (tau != 0.f) ? tau * (rowbuf[j] + inv * pw[j]) : 0.f;
-            wbuf[j] = w;
-            const float y = rowbuf[j] - w;
-            ubuf[j] = y;
-            if (j > i + 1) part += (double)y * y;
-        }
-        const double sigma2 = block_sum_d<NT>(part, dred);
-        float beta2, inv2;
-        house_params_d(ubuf[i + 1], sigma2, &beta2, &tau2, &inv2);
-        __syncthreads();
-        for (int j = i + 1 + tid; j < N; j += NT) {
-            float v;
-            if (j == i + 1) {
-                v = beta2;
-                ubuf[j] = 1.f;
-            } else {
-                v = (tau2 != 0.f) ? ubuf[j] * inv2 : ubuf[j];
-                ubuf[j] = v;
+#pragma unroll
+        for (int t = 0; t < JPT; ++t) {
+            const int j = i + 1 + tid + t * NT;
+            float y = 0.f;
+            if (j < N) {
+                float a = 0.f;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) a += pwr[(size_t)w * N + j];
+                const float wj = (tau != 0.f) ? tau * (rowbuf[j] + inv * a) : 0.f;
+                wbuf[j] = wj;
+                y = rowbuf[j] - wj;
+                if (j > i + 1) part += (double)y * y;
+                if (j == i + 1) dred[2 * NW] = (double)y;
             }
-            store_elem(G, ldg, L, ldl, split, i, j, v);
+            yv[t] = y;
+        }
+        part = wave_sum_dpp_d(part);
+        if (lane == 0) dred[NW + wv] = part;
+        __syncthreads();
+        // ---- row reflector of row i
+        double sigma2 = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) sigma2 += dred[NW + w];
+        float beta2, inv2;
+        house_params_d(dred[2 * NW], sigma2, &beta2, &tau2, &inv2);
+#pragma unroll
+        for (int t = 0; t < JPT; ++t) {
+            const int j = i + 1 + tid + t * NT;
+            if (j < N) {
+                const float v = j == i + 1 ? beta2 : (tau2 != 0.f) ? yv[t] * inv2 : yv[t];
+                ubuf[j] = j == i + 1 ? 1.f : v;
+                store_elem(G, ldg, L, ldl, split, i, j, v);
+            }
         }
         if (tid == 0) {
             P.e[i] = beta2;

@@ -353,6 +353,8 @@ static BatchScratch scratch_layout(int M, int N)
 // LDS classes (0..3: <= 1/8, 1/4, 1/2, all of the LDS) whose resident matrices take the
 // one-wavefront kernel by default
 static constexpr int kWaveMaxClassDefault = -1;
+// workgroup size of the LDS-resident launches per LDS class
+static constexpr int kResThreadsDefault[4] = {512, 512, 512, 512};
 
 // Host-side plan of a batch: a byte blob (uploaded once) + launch sizes.
 struct BatchPlan {
@@ -374,7 +376,7 @@ struct BatchPlan {
     // LDS class
     size_t o_order_res[kWaveClasses] = {}, lds_res[kWaveClasses] = {};
     int n_res[kWaveClasses] = {};
-    int res_threads = 512;
+    int res_threads[kWaveClasses] = {512, 512, 512, 512};
     size_t o_rowin = 0, o_idq = 0, o_idp = 0, o_bq = 0, o_bp = 0, o_sq = 0, o_sp = 0, o_tp = 0;
     size_t o_leaf = 0, o_tu = 0, o_tv = 0, o_ru = 0, o_rv = 0;
     std::vector<size_t> o_task;
@@ -459,9 +461,18 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     const char *wenv = getenv("PACO_BIDIAG_WAVE");
     const char *cenv = getenv("PACO_WAVE_MAX_CLASS");
     const int wave_max_class = (wenv && wenv[0] == '0') ? -1 : cenv ? atoi(cenv) : kWaveMaxClassDefault;
-    const char *tenv = getenv("PACO_RES_THREADS");
-    bp.res_threads = tenv ? atoi(tenv) : 512;
-    if (bp.res_threads != 256 && bp.res_threads != 1024) bp.res_threads = 512;
+    // PACO_RES_THREADS: workgroup size of the resident launches, one value for all LDS
+    // classes or four comma-separated ones (classes <= 1/8, 1/4, 1/2, all of the LDS)
+    for (int cl = 0; cl < BatchPlan::kWaveClasses; ++cl) bp.res_threads[cl] = kResThreadsDefault[cl];
+    if (const char *tenv = getenv("PACO_RES_THREADS")) {
+        const char *p = tenv;
+        for (int cl = 0; cl < BatchPlan::kWaveClasses; ++cl) {
+            const int v = atoi(p);
+            bp.res_threads[cl] = (v == 256 || v == 1024) ? v : 512;
+            const char *comma = strchr(p, ',');
+            if (comma) p = comma + 1; // a single value applies to the remaining classes
+        }
+    }
     const size_t wave_lim = bidiag_wave_lds_limit();
     for (int i : order) {
         const size_t wl = bidiag_wave_lds(jobs[i].M, jobs[i].N);
@@ -473,17 +484,18 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         }
         const BidiagChoice c = bidiag_choice(jobs[i].M, jobs[i].N);
         if (c.resident) {
-            const size_t lr = bidiag_wg_lds_resident_bytes(jobs[i].M, jobs[i].N, bp.res_threads);
+            // class from the 256-thread footprint, then that class's workgroup size
+            const int cl = lds_class(c.lds, bidiag_wg_lds_limit()), nt = bp.res_threads[cl];
+            const size_t lr = bidiag_wg_lds_resident_bytes(jobs[i].M, jobs[i].N, nt);
             if (lr <= bidiag_wg_lds_limit()) {
-                const int cl = lds_class(lr, bidiag_wg_lds_limit());
                 orr[cl].push_back(i);
                 bp.lds_res[cl] = std::max(bp.lds_res[cl], lr);
                 continue;
             }
             // too large for LDS at the wider workgroup: the streamed launch of that size
-            og[bp.res_threads >= 1024 ? 2 : 1].push_back(i);
-            bp.lds_nt[bp.res_threads >= 1024 ? 2 : 1] = std::max(bp.lds_nt[bp.res_threads >= 1024 ? 2 : 1],
-                                                                 bidiag_wg_lds_limit());
+            const int g = nt >= 1024 ? 2 : nt >= 512 ? 1 : 0;
+            og[g].push_back(i);
+            bp.lds_nt[g] = std::max(bp.lds_nt[g], bidiag_wg_lds_limit());
         } else {
             const int g = c.nt >= 1024 ? 2 : c.nt >= 512 ? 1 : 0;
             og[g].push_back(i);
@@ -699,7 +711,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
                 return e;
         for (int cl = BatchPlan::kWaveClasses - 1; cl >= 0; --cl)
             if ((e = launch_bidiag_wg(pl, I(bp.o_order_res[cl]), bp.n_res[cl], bp.lds_res[cl], side,
-                                      bp.res_threads)) != hipSuccess)
+                                      bp.res_threads[cl])) != hipSuccess)
                 return e;
         if (!serial && (e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
     }

@@ -153,6 +153,24 @@ __device__ inline float wave_sum_dpp(float v)
     const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
     return (r0 + r1) + (r2 + r3);
 }
+// FP64 form of wave_sum_dpp: the same lane moves applied to both 32-bit halves
+#define PACO_DPP_D(v, ctrl)                                                                                      \
+    __hiloint2double(__builtin_amdgcn_update_dpp(0, __double2hiint(v), ctrl, 0xF, 0xF, false),                  \
+                     __builtin_amdgcn_update_dpp(0, __double2loint(v), ctrl, 0xF, 0xF, false))
+__device__ inline double wave_sum_dpp_d(double v)
+{
+    v += PACO_DPP_D(v, 0xB1);  // quad_perm [1,0,3,2]
+    v += PACO_DPP_D(v, 0x4E);  // quad_perm [2,3,0,1]
+    v += PACO_DPP_D(v, 0x141); // row_half_mirror
+    v += PACO_DPP_D(v, 0x140); // row_mirror
+    const int hi = __double2hiint(v), lo = __double2loint(v);
+    const double r0 = __hiloint2double(__builtin_amdgcn_readlane(hi, 0), __builtin_amdgcn_readlane(lo, 0));
+    const double r1 = __hiloint2double(__builtin_amdgcn_readlane(hi, 16), __builtin_amdgcn_readlane(lo, 16));
+    const double r2 = __hiloint2double(__builtin_amdgcn_readlane(hi, 32), __builtin_amdgcn_readlane(lo, 32));
+    const double r3 = __hiloint2double(__builtin_amdgcn_readlane(hi, 48), __builtin_amdgcn_readlane(lo, 48));
+    return (r0 + r1) + (r2 + r3);
+}
+#undef PACO_DPP_D
 #undef PACO_DPP
 __device__ inline double wave_sum(double v)
 {
