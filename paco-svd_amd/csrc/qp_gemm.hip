@@ -439,7 +439,7 @@ hipError_t launch_qpb_round(ProbList pl, float *const *qp, int side, int round, 
     const int mode = gemm_get_mode();
     const char *senv = getenv("PACO_QP_STRIP");
     if (mode == 0 && !(senv && senv[0] == '0')) { // FP32 matrix cores: one strip launch per round
-        hipLaunchKernelGGL(qpb_strip_kernel, dim3(ng1), dim3(256), 0, st, pl, g1_off, side, round);
+        if (ng1 > 0) hipLaunchKernelGGL(qpb_strip_kernel, dim3(ng1), dim3(256), 0, st, pl, g1_off, side, round);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(qpb_extract_kernel, dim3(next), dim3(256), 0, st, pl, qp, ext_off, side, round);

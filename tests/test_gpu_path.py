@@ -250,7 +250,8 @@ def test_batched_qp_formation_paths(torch_dev, variant):
     import pacosvd
     rng = np.random.default_rng(17)
     mats = mixed_batch(rng, 20, mmax=330)
-    mats += [rng.standard_normal(sh).astype(np.float32) for sh in [(512, 512), (300, 33), (97, 96), (3, 1)]]
+    mats += [rng.standard_normal(sh).astype(np.float32)
+             for sh in [(512, 512), (300, 33), (97, 96), (3, 1), (600, 200), (513, 64)]]
     say("qp formation", variant, [m.shape for m in mats])
     As = [torch.from_numpy(m).cuda() for m in mats]
     ref = pacosvd.svd_batched(As)
