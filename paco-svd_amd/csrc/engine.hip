@@ -254,8 +254,10 @@ static hipError_t run_dc_levels(ProbList pl, const DcOffsets &o, const uint32_t 
         la.states = o.states[r - 1];
         la.ntask = o.ntask[r - 1];
         la.r = (int)r;
-        la.stride = 3 << r;
+        // every merge of the round has at most min(3 2^r, N) rows and columns: the
+        // launches give each merge that many threads / tiles
         la.maxn = o.maxn[r - 1];
+        la.stride = std::max(1, std::min(3 << r, la.maxn));
         if ((e = launch_dc_level(pl, la, s)) != hipSuccess) return e;
     }
     return hipSuccess;

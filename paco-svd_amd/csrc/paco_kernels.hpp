@@ -16,7 +16,7 @@ struct ProbList {
 // One D&C level (merge round r) over every problem of the launch.  Merge t belongs to
 // problem p with task_off[p] <= t < task_off[p+1] (task_off may be null for one
 // problem); its columns follow from (N, r, t - task_off[p]).  stride bounds the rows
-// and columns of every merge of the round (3 * 2^r).
+// and columns of every merge of the round (min(3 * 2^r, largest N of the launch)).
 struct LevelArgs {
     const int *task_off;
     MergeState *states; // [ntask]
