@@ -643,12 +643,23 @@ __global__ __launch_bounds__(DT) void dc_lowner_kernel(ProbList pl, LevelArgs la
 }
 
 // ---------------------------------------------------------------------------
-// secular vectors (svd.cpp:1440-1496): one wavefront per root i; FP64, stored FP32.
-// UMt (left, in a) and VMt (right, in vmt), row i = coefficients of root i.
+// secular vectors (svd.cpp:1440-1496): a group of G lanes per root i (G = 64, a
+// wavefront, for the large merges; 16, 8 or 4 for small ones, several roots per
+// wavefront); FP64, stored FP32.  UMt (left, in a) and VMt (right, in vmt), row i =
+// coefficients of root i.
 // ---------------------------------------------------------------------------
+template <int G>
+__device__ inline double group_sum(double v)
+{
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int G>
 __global__ __launch_bounds__(DT) void dc_vectors_kernel(ProbList pl, LevelArgs la)
 {
-    const int gw = (blockIdx.x * DT + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    const int gw = (blockIdx.x * DT + threadIdx.x) / G, lane = threadIdx.x % G;
     if (gw >= la.ntask * la.stride) return;
     const int i = gw % la.stride;
     const MergeCtx T = merge_ctx(pl, la, gw / la.stride);
@@ -671,16 +682,16 @@ __global__ __launch_bounds__(DT) void dc_vectors_kernel(ProbList pl, LevelArgs l
     const int K = Ar.Kx[i];
     const double dK = d[K], mu = Ar.mu[i];
     double su = 0, sv = 0;
-    for (int j = lane; j < k; j += 64) {
+    for (int j = lane; j < k; j += G) {
         const double df = (d[j] - dK) * (d[j] + dK) - mu;
         const double v = z[j] / df, u = (j == 0) ? -1.0 : d[j] * z[j] / df;
         su += u * u;
         sv += v * v;
     }
-    su = wave_sum(su);
-    sv = wave_sum(sv);
+    su = group_sum<G>(su);
+    sv = group_sum<G>(sv);
     const double nu = 1.0 / sqrt(su), nv = 1.0 / sqrt(sv);
-    for (int j = lane; j < k; j += 64) {
+    for (int j = lane; j < k; j += G) {
         const double df = (d[j] - dK) * (d[j] + dK) - mu;
         const double v = z[j] / df, u = (j == 0) ? -1.0 : d[j] * z[j] / df;
         um[j] = (float)(u * nu);
@@ -758,6 +769,29 @@ __global__ __launch_bounds__(GT) void dc_merge_gemm_kernel(ProbList pl, LevelArg
     t.n0 = (tile % tpt) * 64;
     if (t.m0 >= t.m || t.n0 >= t.n) return;
     tile_gemm<MODE, TKV>(t);
+}
+
+// Merge products of the first rounds (merges of at most 24 columns): one thread per
+// output entry (i, c), FP32 FMA over the k terms, same gather and scatter as the tiles.
+__global__ __launch_bounds__(DT) void dc_merge_small_kernel(ProbList pl, LevelArgs la, int side)
+{
+    const size_t g = (size_t)blockIdx.x * DT + threadIdx.x;
+    const int per = la.stride * la.stride;
+    if (g >= (size_t)la.ntask * per) return;
+    const int t = (int)(g / per), e = (int)(g % per), i = e / la.stride, c = e % la.stride;
+    const MergeCtx T = merge_ctx(pl, la, t);
+    const MergeState &S = *T.S;
+    const int k = S.k, ncol = side == 0 ? S.n : S.m;
+    if (i >= k || c >= ncol) return;
+    const ProbDesc P = pl.get(T.prob);
+    const int N = P.N, c0 = T.c0;
+    const Arena Ar = make_arena(P.idx, c0, S.n);
+    const size_t base = (size_t)c0 * N + c0;
+    const float *A = (side == 0 ? P.a : P.vmt) + base + (size_t)i * N;
+    const float *B = (side == 0 ? P.qt : P.wt) + (size_t)N * N + base + c;
+    float acc = 0.f;
+    for (int kk = 0; kk < k; ++kk) acc += A[kk] * B[(size_t)Ar.order[kk] * N];
+    ((side == 0 ? P.qt : P.wt) + base)[(size_t)Ar.pos[i] * N + c] = acc;
 }
 
 // back-transforms (updateUVt, svd.cpp:1911-1981):
@@ -839,6 +873,12 @@ static void launch_merge_gemms(ProbList pl, LevelArgs la, hipStream_t s)
 {
     const int tpt = (la.stride + 63) / 64;
     const int grid = la.ntask * tpt * tpt;
+    if (la.stride <= 24) { // one thread per output entry
+        const int blocks = (int)(((size_t)la.ntask * la.stride * la.stride + DT - 1) / DT);
+        hipLaunchKernelGGL(dc_merge_small_kernel, dim3(blocks), dim3(DT), 0, s, pl, la, 0);
+        hipLaunchKernelGGL(dc_merge_small_kernel, dim3(blocks), dim3(DT), 0, s, pl, la, 1);
+        return;
+    }
     // K <= stride: steps of 16 for the short products of the first rounds
     if (la.stride <= 48) {
         hipLaunchKernelGGL((dc_merge_gemm_kernel<MODE, 16>), dim3(grid), dim3(GT), 0, s, pl, la, 0);
@@ -865,7 +905,17 @@ hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s)
     else
         hipLaunchKernelGGL(dc_secular_kernel, dim3(wave_blocks), dim3(DT), 0, s, pl, la);
     hipLaunchKernelGGL(dc_lowner_kernel, dim3((rows + DT - 1) / DT), dim3(DT), 0, s, pl, la);
-    hipLaunchKernelGGL(dc_vectors_kernel, dim3(wave_blocks), dim3(DT), 0, s, pl, la);
+    // lanes per root: a wavefront for merges above 192 columns, fewer below
+    const int vg = la.stride <= 12 ? 4 : la.stride <= 48 ? 8 : la.stride <= 192 ? 16 : 64;
+    const int vblocks = (int)(((size_t)rows * vg + DT - 1) / DT);
+    if (vg == 4)
+        hipLaunchKernelGGL(dc_vectors_kernel<4>, dim3(vblocks), dim3(DT), 0, s, pl, la);
+    else if (vg == 8)
+        hipLaunchKernelGGL(dc_vectors_kernel<8>, dim3(vblocks), dim3(DT), 0, s, pl, la);
+    else if (vg == 16)
+        hipLaunchKernelGGL(dc_vectors_kernel<16>, dim3(vblocks), dim3(DT), 0, s, pl, la);
+    else
+        hipLaunchKernelGGL(dc_vectors_kernel<64>, dim3(vblocks), dim3(DT), 0, s, pl, la);
     const int mode = gemm_get_mode();
     if (mode == 1)
         launch_merge_gemms<1>(pl, la, s);
