@@ -771,29 +771,6 @@ __global__ __launch_bounds__(GT) void dc_merge_gemm_kernel(ProbList pl, LevelArg
     tile_gemm<MODE, TKV>(t);
 }
 
-// Merge products of the first rounds (merges of at most 24 columns): one thread per
-// output entry (i, c), FP32 FMA over the k terms, same gather and scatter as the tiles.
-__global__ __launch_bounds__(DT) void dc_merge_small_kernel(ProbList pl, LevelArgs la, int side)
-{
-    const size_t g = (size_t)blockIdx.x * DT + threadIdx.x;
-    const int per = la.stride * la.stride;
-    if (g >= (size_t)la.ntask * per) return;
-    const int t = (int)(g / per), e = (int)(g % per), i = e / la.stride, c = e % la.stride;
-    const MergeCtx T = merge_ctx(pl, la, t);
-    const MergeState &S = *T.S;
-    const int k = S.k, ncol = side == 0 ? S.n : S.m;
-    if (i >= k || c >= ncol) return;
-    const ProbDesc P = pl.get(T.prob);
-    const int N = P.N, c0 = T.c0;
-    const Arena Ar = make_arena(P.idx, c0, S.n);
-    const size_t base = (size_t)c0 * N + c0;
-    const float *A = (side == 0 ? P.a : P.vmt) + base + (size_t)i * N;
-    const float *B = (side == 0 ? P.qt : P.wt) + (size_t)N * N + base + c;
-    float acc = 0.f;
-    for (int kk = 0; kk < k; ++kk) acc += A[kk] * B[(size_t)Ar.order[kk] * N];
-    ((side == 0 ? P.qt : P.wt) + base)[(size_t)Ar.pos[i] * N + c] = acc;
-}
-
 // back-transforms (updateUVt, svd.cpp:1911-1981):
 //   side 0: a (M x N) <- u[:, :N] * qt^T        side 1: qt+N*N (N x N) <- wt * vt
 template <int MODE>
@@ -873,12 +850,6 @@ static void launch_merge_gemms(ProbList pl, LevelArgs la, hipStream_t s)
 {
     const int tpt = (la.stride + 63) / 64;
     const int grid = la.ntask * tpt * tpt;
-    if (la.stride <= 24) { // one thread per output entry
-        const int blocks = (int)(((size_t)la.ntask * la.stride * la.stride + DT - 1) / DT);
-        hipLaunchKernelGGL(dc_merge_small_kernel, dim3(blocks), dim3(DT), 0, s, pl, la, 0);
-        hipLaunchKernelGGL(dc_merge_small_kernel, dim3(blocks), dim3(DT), 0, s, pl, la, 1);
-        return;
-    }
     // K <= stride: steps of 16 for the short products of the first rounds
     if (la.stride <= 48) {
         hipLaunchKernelGGL((dc_merge_gemm_kernel<MODE, 16>), dim3(grid), dim3(GT), 0, s, pl, la, 0);
