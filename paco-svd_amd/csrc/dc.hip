@@ -365,9 +365,19 @@ __global__ __launch_bounds__(DT) void dc_gather_kernel(ProbList pl, LevelArgs la
 // svd.cpp:977-1336): FP64, origin chosen by the midpoint test, two-pole rational
 // model, bracket with bisection fallback, convergence flag into info.
 // ---------------------------------------------------------------------------
+// sum over the G lanes of an aligned lane group (G = 64: the wavefront)
+template <int G>
+__device__ inline double group_sum(double v)
+{
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int G>
 __global__ __launch_bounds__(DT) void dc_secular_kernel(ProbList pl, LevelArgs la)
 {
-    const int gw = (blockIdx.x * DT + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    const int gw = (blockIdx.x * DT + threadIdx.x) / G, lane = threadIdx.x % G;
     if (gw >= la.ntask * la.stride) return;
     const int i = gw % la.stride;
     const MergeCtx T = merge_ctx(pl, la, gw / la.stride);
@@ -389,16 +399,16 @@ __global__ __launch_bounds__(DT) void dc_secular_kernel(ProbList pl, LevelArgs l
     double lo, hi;
     if (i == k - 1) {
         double zz = 0;
-        for (int j = lane; j < k; j += 64) zz += z[j] * z[j];
-        zz = wave_sum(zz);
+        for (int j = lane; j < k; j += G) zz += z[j] * z[j];
+        zz = group_sum<G>(zz);
         K = k - 1;
         lo = 0.0;
         hi = zz;
     } else {
         const double dl = d[i], dr = d[i + 1], om = 0.5 * (dl + dr);
         double f = 0;
-        for (int j = lane; j < k; j += 64) f += z[j] * z[j] / ((d[j] - om) * (d[j] + om));
-        f = 1.0 + wave_sum(f);
+        for (int j = lane; j < k; j += G) f += z[j] * z[j] / ((d[j] - om) * (d[j] + om));
+        f = 1.0 + group_sum<G>(f);
         if (f > 0) {
             K = i;
             lo = 0.0;
@@ -414,7 +424,7 @@ __global__ __launch_bounds__(DT) void dc_secular_kernel(ProbList pl, LevelArgs l
     int it;
     for (it = 1; it <= 200; ++it) {
         double psi = 0, dpsi = 0, phi = 0, dphi = 0, erretm = 0;
-        for (int j = lane; j < k; j += 64) {
+        for (int j = lane; j < k; j += G) {
             const double t = (d[j] - dK) * (d[j] + dK) - mu;
             const double q = z[j] / t, term = z[j] * q;
             if (j <= i) {
@@ -426,11 +436,11 @@ __global__ __launch_bounds__(DT) void dc_secular_kernel(ProbList pl, LevelArgs l
             }
             erretm += fabs(term);
         }
-        psi = wave_sum(psi);
-        dpsi = wave_sum(dpsi);
-        phi = wave_sum(phi);
-        dphi = wave_sum(dphi);
-        erretm = wave_sum(erretm);
+        psi = group_sum<G>(psi);
+        dpsi = group_sum<G>(dpsi);
+        phi = group_sum<G>(phi);
+        dphi = group_sum<G>(dphi);
+        erretm = group_sum<G>(erretm);
         const double w = 1.0 + psi + phi;
         if (w == 0.0 || fabs(w) <= 8.0 * eps * (1.0 + erretm)) break;
         if (w < 0)
@@ -649,14 +659,6 @@ __global__ __launch_bounds__(DT) void dc_lowner_kernel(ProbList pl, LevelArgs la
 // coefficients of root i.
 // ---------------------------------------------------------------------------
 template <int G>
-__device__ inline double group_sum(double v)
-{
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-template <int G>
 __global__ __launch_bounds__(DT) void dc_vectors_kernel(ProbList pl, LevelArgs la)
 {
     const int gw = (blockIdx.x * DT + threadIdx.x) / G, lane = threadIdx.x % G;
@@ -873,8 +875,12 @@ hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s)
     hipLaunchKernelGGL(dc_gather_kernel, dim3(la.ntask * spt), dim3(DT), 0, s, pl, la);
     if (la.stride <= 96)
         hipLaunchKernelGGL(dc_secular_thread_kernel, dim3((rows + DT - 1) / DT), dim3(DT), 0, s, pl, la);
+    else if (la.stride <= 192) // lanes per root: 16, 32, then a wavefront
+        hipLaunchKernelGGL(dc_secular_kernel<16>, dim3((int)(((size_t)rows * 16 + DT - 1) / DT)), dim3(DT), 0, s, pl, la);
+    else if (la.stride <= 384)
+        hipLaunchKernelGGL(dc_secular_kernel<32>, dim3((int)(((size_t)rows * 32 + DT - 1) / DT)), dim3(DT), 0, s, pl, la);
     else
-        hipLaunchKernelGGL(dc_secular_kernel, dim3(wave_blocks), dim3(DT), 0, s, pl, la);
+        hipLaunchKernelGGL(dc_secular_kernel<64>, dim3(wave_blocks), dim3(DT), 0, s, pl, la);
     hipLaunchKernelGGL(dc_lowner_kernel, dim3((rows + DT - 1) / DT), dim3(DT), 0, s, pl, la);
     // lanes per root: a wavefront for merges above 192 columns, fewer below
     const int vg = la.stride <= 12 ? 4 : la.stride <= 48 ? 8 : la.stride <= 192 ? 16 : 64;
