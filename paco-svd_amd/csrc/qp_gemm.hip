@@ -332,20 +332,15 @@ __device__ inline void qpb_load_x(float (&x)[16], const float *X, int L, int cx,
     }
 }
 
-__global__ __launch_bounds__(256) void qpb_strip_kernel(ProbList pl, const int *off, int side, int round)
+// Block b of one side applied to the 64 columns [c0, c0 + 64) of X[R0:, :] (columns
+// left of R0 are identity columns the block leaves unchanged; their W columns are zero).
+// Every thread of the workgroup calls it; it starts and ends with the workgroup's
+// accesses to X ordered by a barrier.
+__device__ inline void qpb_strip_block(const ProbDesc &P, int side, int b, int c0, float (*Vs)[33], float (*Ts)[33],
+                                       float (*Wp)[68], float (*W2)[68])
 {
-    __shared__ float Vs[64][33];
-    __shared__ float Ts[32][33];
-    __shared__ float Wp[32][68];
-    __shared__ float W2[32][68];
-    const int pid = find_task(off, pl.count, blockIdx.x);
-    const int strip = blockIdx.x - off_at(off, pid);
-    const ProbDesc P = pl.get(pid);
     const int N = P.N, L = side == 0 ? P.M : P.N, nref = side == 0 ? P.N : P.N - 1;
-    const int b = (nref + 31) / 32 - 1 - round;
-    if (b < 0) return;
-    const int i0 = 32 * b, nb = min(32, nref - i0), R0 = i0 + side, c0 = R0 + 64 * strip;
-    if (c0 >= L) return;
+    const int i0 = 32 * b, nb = min(32, nref - i0), R0 = i0 + side;
     float *X = side == 0 ? P.u : P.vt;
     const float *T = (side == 0 ? P.tq : P.tp) + (size_t)b * 1024;
     const float *A = P.a;
@@ -427,6 +422,24 @@ __global__ __launch_bounds__(256) void qpb_strip_kernel(ProbList pl, const int *
             if (cok && r < L) X[(size_t)r * L + cx] = xo[j] - a3[j];
         }
     }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void qpb_strip_kernel(ProbList pl, const int *off, int side, int round)
+{
+    __shared__ float Vs[64][33];
+    __shared__ float Ts[32][33];
+    __shared__ float Wp[32][68];
+    __shared__ float W2[32][68];
+    const int pid = find_task(off, pl.count, blockIdx.x);
+    const int strip = blockIdx.x - off_at(off, pid);
+    const ProbDesc P = pl.get(pid);
+    const int L = side == 0 ? P.M : P.N, nref = side == 0 ? P.N : P.N - 1;
+    const int b = (nref + 31) / 32 - 1 - round;
+    if (b < 0) return;
+    const int c0 = 32 * b + side + 64 * strip;
+    if (c0 >= L) return;
+    qpb_strip_block(P, side, b, c0, Vs, Ts, Wp, W2);
 }
 
 size_t qpb_scratch_floats(int M, int N) { return 2 * 96 * (size_t)(M > N ? M : N); }
