@@ -16,6 +16,8 @@
 #include "paco_common.hpp"
 #include "paco_kernels.hpp"
 
+typedef float tf_floatx16 __attribute__((ext_vector_type(16)));
+
 namespace paco {
 
 #define BT 256
@@ -562,10 +564,15 @@ __global__ __launch_bounds__(BT) void tfactor_kernel(ProbList pl, const int *blk
     const int i0 = b * 32, nb = min(32, nref - i0);
     if (nb <= 0) return;
     __shared__ float Vs[64][33];
+    __shared__ float Gp[3][32][33];
     __shared__ float G[32][33];
     __shared__ float T[32][33];
-    const int tid = threadIdx.x;
-    float g[4] = {0.f, 0.f, 0.f, 0.f};
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, l32 = lane & 31, lh = lane >> 5;
+    // Gram matrix G = V^T V on the matrix cores: wavefront w takes rows 16 w .. 16 w + 15
+    // of every 64-row chunk (the A and B operands are the same rows of V)
+    tf_floatx16 acc;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.f;
     for (int r0 = i0 + side; r0 < L; r0 += 64) {
         for (int q = tid; q < 64 * 32; q += BT) {
             const int rr = q / 32, t = q % 32, r = r0 + rr;
@@ -573,32 +580,40 @@ __global__ __launch_bounds__(BT) void tfactor_kernel(ProbList pl, const int *blk
         }
         __syncthreads();
 #pragma unroll
-        for (int e4 = 0; e4 < 4; ++e4) {
-            const int q = tid + e4 * BT, s = q / 32, t = q % 32;
-            float acc = 0.f;
-            for (int rr = 0; rr < 64; ++rr) acc += Vs[rr][s] * Vs[rr][t];
-            g[e4] += acc;
+        for (int kk = 0; kk < 8; ++kk) {
+            const float v = Vs[16 * w + 2 * kk + lh][l32];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(v, v, acc, 0, 0, 0);
         }
         __syncthreads();
     }
+    // acc[j] holds G[8 (j / 4) + 4 lh + j % 4][l32] of this wavefront's rows
+    if (w > 0) {
 #pragma unroll
-    for (int e4 = 0; e4 < 4; ++e4) {
-        const int q = tid + e4 * BT;
-        G[q / 32][q % 32] = g[e4];
-        T[q / 32][q % 32] = 0.f;
+        for (int j = 0; j < 16; ++j) Gp[w - 1][(j >> 2) * 8 + lh * 4 + (j & 3)][l32] = acc[j];
     }
     __syncthreads();
-    // T[0:t, t] = -tau_t * T[0:t, 0:t] * G[0:t, t]; every thread takes part in the barriers
-    for (int t = 0; t < nb; ++t) {
-        const float tt = refl_tau(P, side, i0 + t);
-        float acc = 0.f;
-        if (tid < t)
-            for (int q = tid; q < t; ++q) acc += T[tid][q] * G[q][t];
-        __syncthreads();
-        if (tid < t) T[tid][t] = -tt * acc;
-        if (tid == t) T[t][t] = tt;
-        __syncthreads();
+    if (w == 0) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int r = (j >> 2) * 8 + lh * 4 + (j & 3);
+            G[r][l32] = ((acc[j] + Gp[0][r][l32]) + Gp[1][r][l32]) + Gp[2][r][l32];
+        }
+        for (int q = lane; q < 32 * 32; q += 64) T[q / 32][q % 32] = 0.f;
+        __builtin_amdgcn_wave_barrier();
+        // T[0:t, t] = -tau_t * T[0:t, 0:t] * G[0:t, t]: one wavefront, lane i < t forms
+        // row i; LDS accesses of one wavefront complete in order
+        for (int t = 0; t < nb; ++t) {
+            const float tt = refl_tau(P, side, i0 + t);
+            float s = 0.f;
+            if (lane < t)
+                for (int q = lane; q < t; ++q) s += T[lane][q] * G[q][t];
+            __builtin_amdgcn_wave_barrier();
+            if (lane < t) T[lane][t] = -tt * s;
+            if (lane == t) T[t][t] = tt;
+            __builtin_amdgcn_wave_barrier();
+        }
     }
+    __syncthreads();
     float *out = (side == 0 ? P.tq : P.tp) + (size_t)b * 1024;
     for (int q = tid; q < 1024; q += BT) out[q] = T[q / 32][q % 32];
 }

@@ -143,6 +143,41 @@ __device__ inline MergeCtx merge_ctx(const ProbList &pl, const LevelArgs &la, in
 }
 __host__ __device__ inline int strips_per_task(const LevelArgs &la) { return (la.stride + DT - 1) / DT; }
 
+// Column-parallel merge kernels (gather, finalize): a merge takes TPM = the power of two
+// >= stride threads when stride <= DT (DT / TPM merges per workgroup), else strips of DT
+// columns.  Thread -> (merge t, first column x, column step).
+__host__ __device__ inline int merge_tpm(const LevelArgs &la)
+{
+    int t = 1;
+    while (t < la.stride) t <<= 1;
+    return t < DT ? t : DT;
+}
+__host__ inline int column_blocks(const LevelArgs &la)
+{
+    return la.stride <= DT ? (la.ntask + DT / merge_tpm(la) - 1) / (DT / merge_tpm(la)) : la.ntask * strips_per_task(la);
+}
+struct ColumnSlot {
+    int t, x, step, first; // first: this slot's threads hold columns 0..
+};
+__device__ inline ColumnSlot column_slot(const LevelArgs &la)
+{
+    ColumnSlot c;
+    if (la.stride <= DT) {
+        const int tpm = merge_tpm(la);
+        c.t = blockIdx.x * (DT / tpm) + (int)threadIdx.x / tpm;
+        c.x = threadIdx.x % tpm;
+        c.step = tpm;
+        c.first = 1;
+    } else {
+        const int spt = strips_per_task(la), strip = blockIdx.x % spt;
+        c.t = blockIdx.x / spt;
+        c.x = strip * DT + threadIdx.x;
+        c.step = DT;
+        c.first = strip == 0;
+    }
+    return c;
+}
+
 // ---------------------------------------------------------------------------
 // deflate: one workgroup per merge (Deflation, svd.cpp:325-975; dlasd2 structure)
 // dynamic LDS: pd[n] (double), pz[n] (double), flag[n] (int), scan[DT+1] (int)
@@ -311,14 +346,14 @@ __global__ __launch_bounds__(DT) void dc_deflate_kernel(ProbList pl, LevelArgs l
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(DT) void dc_gather_kernel(ProbList pl, LevelArgs la)
 {
-    const int spt = strips_per_task(la);
-    const int strip = blockIdx.x % spt;
-    const MergeCtx T = merge_ctx(pl, la, blockIdx.x / spt);
+    const ColumnSlot cs = column_slot(la);
+    if (cs.t >= la.ntask) return;
+    const MergeCtx T = merge_ctx(pl, la, cs.t);
     const ProbDesc P = pl.get(T.prob);
     const MergeState S = *T.S;
     const int N = P.N, c0 = T.c0, c1 = T.c1;
     const int n = S.n, m = S.m, n1 = S.n1, m1 = S.m1, n2 = S.n2, m2 = S.m2;
-    const int x = strip * DT + threadIdx.x;
+    const int x = cs.x;
     if (x >= m) return;
     const Arena Ar = make_arena(P.idx, c0, n);
     const float *qt = P.qt, *wt = P.wt;
@@ -706,15 +741,15 @@ __global__ __launch_bounds__(DT) void dc_vectors_kernel(ProbList pl, LevelArgs l
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(DT) void dc_finalize_kernel(ProbList pl, LevelArgs la)
 {
-    const int spt = strips_per_task(la);
-    const int strip = blockIdx.x % spt;
-    const MergeCtx T = merge_ctx(pl, la, blockIdx.x / spt);
+    const ColumnSlot cs = column_slot(la);
+    if (cs.t >= la.ntask) return;
+    const MergeCtx T = merge_ctx(pl, la, cs.t);
     const ProbDesc P = pl.get(T.prob);
     const MergeState S = *T.S;
     const int N = P.N, c0 = T.c0, n = S.n, m = S.m, k = S.k;
     const Arena Ar = make_arena(P.idx, c0, n);
     const float *st = P.qt + (size_t)N * N, *gt = P.wt + (size_t)N * N;
-    const int x = strip * DT + threadIdx.x;
+    const int x = cs.x;
     if (x < m) {
         for (int t = k; t < n; ++t) {
             const int src = Ar.order[t], dst = Ar.pos[t];
@@ -723,8 +758,8 @@ __global__ __launch_bounds__(DT) void dc_finalize_kernel(ProbList pl, LevelArgs 
         }
         if (S.sq) P.wt[(size_t)(c0 + n) * N + c0 + x] = gt[(size_t)(c0 + n) * N + c0 + x];
     }
-    if (strip == 0) {
-        for (int t = threadIdx.x; t < n; t += DT) {
+    if (cs.first) {
+        for (int t = cs.x; t < n; t += cs.step) {
             double s;
             if (t < k) {
                 const int K = Ar.Kx[t];
@@ -868,11 +903,10 @@ hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s)
     if (la.ntask <= 0) return hipSuccess;
     const size_t lds = (size_t)la.maxn * (2 * sizeof(double) + sizeof(int)) + (DT + 1) * sizeof(int);
     if (lds > maxdyn) return hipErrorInvalidValue;
-    const int spt = strips_per_task(la);
     const int rows = la.ntask * la.stride;
     const int wave_blocks = (rows + DT / 64 - 1) / (DT / 64);
     hipLaunchKernelGGL(dc_deflate_kernel, dim3(la.ntask), dim3(DT), lds, s, pl, la);
-    hipLaunchKernelGGL(dc_gather_kernel, dim3(la.ntask * spt), dim3(DT), 0, s, pl, la);
+    hipLaunchKernelGGL(dc_gather_kernel, dim3(column_blocks(la)), dim3(DT), 0, s, pl, la);
     if (la.stride <= 96)
         hipLaunchKernelGGL(dc_secular_thread_kernel, dim3((rows + DT - 1) / DT), dim3(DT), 0, s, pl, la);
     else if (la.stride <= 192) // lanes per root: 16, 32, then a wavefront
@@ -900,7 +934,7 @@ hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s)
         launch_merge_gemms<2>(pl, la, s);
     else
         launch_merge_gemms<0>(pl, la, s);
-    hipLaunchKernelGGL(dc_finalize_kernel, dim3(la.ntask * spt), dim3(DT), 0, s, pl, la);
+    hipLaunchKernelGGL(dc_finalize_kernel, dim3(column_blocks(la)), dim3(DT), 0, s, pl, la);
     return hipGetLastError();
 }
 
