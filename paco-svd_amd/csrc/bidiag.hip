@@ -3,14 +3,14 @@
 // Reference: upper_bidiagonalization.cpp (Process :83-120, Householder :182-283,
 // rank-1 applications :426-563, GetUVt :774-893).  The reference streams one
 // reflector at a time through 40 vector cores with 4 grid barriers per column; here
-//  * bidiag_wg_kernel: one workgroup owns one matrix and fuses, per column, the left
-//    reflector's GEMV with the right reflector's GEMV + both rank-1 updates, so the
-//    trailing matrix is read twice per column (and written once) and no grid barrier
-//    is needed.  Used for the batched path and small matrices.
-//  * Q and P are formed by blocked WY (32 reflectors per block): T factors per block,
-//    then one launch in which each workgroup owns a column strip of the target and
-//    applies all blocks to it (strips are independent), replacing the reference's
-//    N rank-1 passes with barriers.
+//  * bidiag_wg_kernel: one workgroup owns one matrix and fuses, per column, the
+//    application of both reflectors with the gathering of the next column's GEMV data,
+//    so the trailing matrix is read and written once per column and no grid barrier is
+//    needed.  Used for the batched path and small matrices.
+//  * Q and P are formed by blocked WY (32 reflectors per block): T factors per block
+//    (tfactor_kernel), applied by the strip kernels of qp_gemm.hip or, with FMA, by
+//    wy_apply_kernel (each workgroup owns a column strip of the target and applies all
+//    blocks to it), replacing the reference's N rank-1 passes with barriers.
 #include <algorithm>
 
 #include "paco_common.hpp"

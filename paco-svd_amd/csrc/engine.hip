@@ -32,8 +32,8 @@ __global__ void zero_ints_kernel(int *p, int n)
     if (i < n) p[i] = 0;
 }
 
-// a <- A (row-major M x N) for every problem; grid over (problem, row).
-// rows [ROW_GROUP g, ROW_GROUP (g + 1)) of every input into its working copy
+// a <- A (row-major M x N) for every problem: rows [ROW_GROUP g, ROW_GROUP (g + 1)) of
+// every input into its working copy
 __global__ __launch_bounds__(256) void copy_in_kernel(ProbList pl, const float *const *src, const int *row_off)
 {
     const int pid = find_task(row_off, pl.count, blockIdx.x);
