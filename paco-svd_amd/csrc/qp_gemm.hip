@@ -425,7 +425,7 @@ __device__ inline void qpb_strip_block(const ProbDesc &P, int side, int b, int c
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void qpb_strip_kernel(ProbList pl, const int *off, int side, int round)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void qpb_strip_kernel(ProbList pl, const int *off, int side, int round)
 {
     __shared__ float Vs[64][33];
     __shared__ float Ts[32][33];
