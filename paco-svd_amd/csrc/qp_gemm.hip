@@ -348,7 +348,8 @@ __device__ inline void qpb_strip_block(const ProbDesc &P, int side, int b, int c
     const int l32 = lane & 31, lh = lane >> 5;
     const int cx = c0 + 32 * h + l32; // this lane's column of X
     const bool cok = cx < L;
-    // the chunk's loads are issued one chunk ahead: V (staged through LDS) and X
+    // the chunk's loads are issued one chunk ahead: V (staged through LDS) and X (after
+    // the current chunk's products)
     float vn[8], xn[16];
     qpb_load_v(vn, A, N, side, i0, nb, R0, L);
     qpb_load_x(xn, X, L, cx, cok, R0, 32 * kh + lh);
@@ -359,17 +360,13 @@ __device__ inline void qpb_strip_block(const ProbDesc &P, int side, int b, int c
     for (int base = R0; base < L; base += 64) {
         __syncthreads();
         qpb_store_v(Vs, vn, side);
-        float bx[16];
-#pragma unroll
-        for (int kk = 0; kk < 16; ++kk) bx[kk] = xn[kk];
-        if (base + 64 < L) {
-            qpb_load_v(vn, A, N, side, i0, nb, base + 64, L);
-            qpb_load_x(xn, X, L, cx, cok, base + 64, 32 * kh + lh);
-        }
+        if (base + 64 < L) qpb_load_v(vn, A, N, side, i0, nb, base + 64, L);
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk)
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Vs[32 * kh + 2 * kk + lh][l32], bx[kk], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Vs[32 * kh + 2 * kk + lh][l32], xn[kk], acc, 0, 0, 0);
+        // the next chunk's X entries, in flight over the next V store and barrier
+        if (base + 64 < L) qpb_load_x(xn, X, L, cx, cok, base + 64, 32 * kh + lh);
     }
     // first chunk of the update, loaded while W is reduced; the update's X entries of a
     // lane are rows base + 32 kh + 8 (j / 4) + 4 lh + j % 4 (the accumulator layout)
