@@ -176,8 +176,8 @@ __device__ inline void pass_group(T *base, int ld, int rbase, int r0, int k, int
 // Wave partials of pw go to pwr[wave]; the return value is lane 0's partial sum of squares.
 template <int NT, int C>
 __device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int split, int M, int N, int i,
-                                      float tau, float inv, float tau2, lds_f32 *cbuf, const lds_f32 *wbuf,
-                                      const lds_f32 *ubuf, lds_f32 *rowbuf, lds_f32 *pwr)
+                                      float tau, float inv, float tau2, float inv2, lds_f32 *cbuf,
+                                      const lds_f32 *wbuf, const lds_f32 *ybuf, lds_f32 *rowbuf, lds_f32 *pwr)
 {
     constexpr int NW = NT / 64, RQ = PassRows<NT, C>::RQ;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, k = i + 1;
@@ -187,7 +187,8 @@ __device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G,
     for (int c = 0; c < C; ++c) {
         const int j = k + lane + 64 * c;
         wr[c] = (upd && j < N) ? wbuf[j] : 0.f;
-        ur[c] = (upd && j < N) ? ubuf[j] : 0.f;
+        // u of the row reflector from y = row i - w: unit at column k, y / v1 beyond
+        ur[c] = !(upd && j < N) ? 0.f : j == k ? 1.f : (tau2 != 0.f) ? ybuf[j] * inv2 : ybuf[j];
         pwp[c] = 0.f;
     }
     double ss = 0.0;
@@ -225,11 +226,11 @@ __device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G,
 
 template <int NT>
 __device__ __attribute__((always_inline)) inline double fused_pass(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int split, int M, int N, int i,
-                                    float tau, float inv, float tau2, lds_f32 *cbuf, const lds_f32 *wbuf,
-                                    const lds_f32 *ubuf, lds_f32 *rowbuf, lds_f32 *pwr)
+                                    float tau, float inv, float tau2, float inv2, lds_f32 *cbuf,
+                                    const lds_f32 *wbuf, const lds_f32 *ybuf, lds_f32 *rowbuf, lds_f32 *pwr)
 {
 #define PACO_PASS(CC)                                                                                              \
-    return fused_pass_c<NT, CC>(G, ldg, L, ldl, split, M, N, i, tau, inv, tau2, cbuf, wbuf, ubuf, rowbuf, pwr)
+    return fused_pass_c<NT, CC>(G, ldg, L, ldl, split, M, N, i, tau, inv, tau2, inv2, cbuf, wbuf, ybuf, rowbuf, pwr)
     switch ((N - i - 1 + 63) >> 6) {
     case 1: PACO_PASS(1);
     case 2: PACO_PASS(2);
@@ -280,13 +281,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
     __syncthreads();
 
     // pass i = -1 is the read-only gathering pass of column 0; pass i >= 0 applies the
-    // reflectors of column i to rows > i and gathers column i + 1.  Per column three
-    // barriers: after the pass; after the row reflector's partial sums; after its vector.
+    // reflectors of column i to rows > i and gathers column i + 1.  Per column two
+    // barriers: after the pass; after the row reflector's partial sums (and y, w).
     // Every thread forms both reflectors' parameters from the same sums in the same order.
     constexpr int JPT = (WG_MAXN + NT - 1) / NT; // columns of row i per thread
-    float tau = 0.f, inv = 1.f, tau2 = 0.f;
+    float tau = 0.f, inv = 1.f, tau2 = 0.f, inv2 = 1.f;
     for (int i = -1;;) {
-        const double ss = fused_pass<NT>(G, ldg, L, ldl, split, M, N, i, tau, inv, tau2, cbuf, wbuf, ubuf, rowbuf, pwr);
+        const double ss =
+            fused_pass<NT>(G, ldg, L, ldl, split, M, N, i, tau, inv, tau2, inv2, cbuf, wbuf, ubuf, rowbuf, pwr);
         ++i;
         // the previous column's readers of dred finished before the barrier ahead of the pass
         if (lane == 0) dred[wv] = ss;
@@ -321,6 +323,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
                 const float wj = (tau != 0.f) ? tau * (rowbuf[j] + inv * a) : 0.f;
                 wbuf[j] = wj;
                 y = rowbuf[j] - wj;
+                ubuf[j] = y;
                 if (j > i + 1) part += (double)y * y;
                 if (j == i + 1) dred[2 * NW] = (double)y;
             }
@@ -333,22 +336,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
         double sigma2 = 0.0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) sigma2 += dred[NW + w];
-        float beta2, inv2;
+        float beta2;
         house_params_d(dred[2 * NW], sigma2, &beta2, &tau2, &inv2);
 #pragma unroll
         for (int t = 0; t < JPT; ++t) {
             const int j = i + 1 + tid + t * NT;
-            if (j < N) {
-                const float v = j == i + 1 ? beta2 : (tau2 != 0.f) ? yv[t] * inv2 : yv[t];
-                ubuf[j] = j == i + 1 ? 1.f : v;
-                store_elem(G, ldg, L, ldl, split, i, j, v);
-            }
+            if (j < N) store_elem(G, ldg, L, ldl, split, i, j, j == i + 1 ? beta2 : (tau2 != 0.f) ? yv[t] * inv2 : yv[t]);
         }
         if (tid == 0) {
             P.e[i] = beta2;
             P.taup[i] = tau2;
         }
-        __syncthreads();
+        // no barrier before the next pass: it reads y (ubuf) and w (wbuf), published before
+        // the barrier above, and forms u itself; this section reads nothing the pass writes
     }
     __syncthreads();
     for (int r = split + wv; r < M; r += NW)
