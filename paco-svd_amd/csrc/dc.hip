@@ -308,9 +308,10 @@ __global__ __launch_bounds__(DT) void dc_deflate_kernel(ProbList pl, LevelArgs l
     for (int j = j0; j < j1; ++j) cnt += (flag[j] == 0);
     scan[tid] = cnt;
     __syncthreads();
-    if (tid == 0) {
+    if (tid == 0) { // only the first ceil(n / chunk) threads hold columns
         int acc = 0;
-        for (int t = 0; t < DT; ++t) {
+        const int nch = (n + chunk - 1) / chunk;
+        for (int t = 0; t < nch; ++t) {
             int v = scan[t];
             scan[t] = acc;
             acc += v;
