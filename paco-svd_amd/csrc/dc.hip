@@ -362,22 +362,37 @@ __global__ __launch_bounds__(DT) void dc_gather_kernel(ProbList pl, LevelArgs la
     const float nc = (float)S.nc, ns = (float)S.ns;
     const float v1n = (x < m1) ? wt[(size_t)(c0 + n1) * N + c0 + x] : 0.f;
     const float v2n = (S.sq && x >= m1) ? wt[(size_t)(c1 + n2) * N + c1 + (x - m1)] : 0.f;
-    for (int t = 0; t < n; ++t) {
+    // entries of row t of the gathered bases (the sources are the first halves of qt, wt,
+    // the destinations the second halves: four rows are read before they are written)
+    auto fetch = [&](int t, float &qv, float &wv) {
         const int s = Ar.src[t];
-        float qv = 0.f, wv = 0.f;
-        if (s == SRC_POLE0) {
+        const bool pole = s == SRC_POLE0, left = !pole && s >= 0;
+        const int q = -1 - s;
+        const bool qok = !pole && (left ? x < n1 : (x > n1 && x < n));
+        const bool wok = !pole && (left ? x < m1 : x >= m1);
+        qv = qok ? qt[left ? (size_t)(c0 + s) * N + c0 + x : (size_t)(c1 + q) * N + c1 + (x - n1 - 1)] : 0.f;
+        wv = wok ? wt[left ? (size_t)(c0 + s) * N + c0 + x : (size_t)(c1 + q) * N + c1 + (x - m1)] : 0.f;
+        if (pole) {
             qv = (x == n1) ? 1.f : 0.f;
             wv = S.sq ? (x < m1 ? nc * v1n : ns * v2n) : v1n;
-        } else if (s >= 0) {
-            if (x < n1) qv = qt[(size_t)(c0 + s) * N + c0 + x];
-            if (x < m1) wv = wt[(size_t)(c0 + s) * N + c0 + x];
-        } else {
-            const int q = -1 - s;
-            if (x > n1 && x < n) qv = qt[(size_t)(c1 + q) * N + c1 + (x - n1 - 1)];
-            if (x >= m1) wv = wt[(size_t)(c1 + q) * N + c1 + (x - m1)];
         }
+    };
+    auto put = [&](int t, float qv, float wv) {
         if (x < n) st[(size_t)(c0 + t) * N + c0 + x] = qv;
         gt[(size_t)(c0 + t) * N + c0 + x] = wv;
+    };
+    int t = 0;
+    for (; t + 4 <= n; t += 4) {
+        float qv[4], wv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) fetch(t + u, qv[u], wv[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) put(t + u, qv[u], wv[u]);
+    }
+    for (; t < n; ++t) {
+        float qv, wv;
+        fetch(t, qv, wv);
+        put(t, qv, wv);
     }
     if (S.sq) gt[(size_t)(c0 + n) * N + c0 + x] = (x < m1) ? -ns * v1n : nc * v2n;
     for (int r = 0; r < S.nrot; ++r) {
