@@ -573,12 +573,24 @@ __global__ __launch_bounds__(BT) void tfactor_kernel(ProbList pl, const int *blk
     tf_floatx16 acc;
 #pragma unroll
     for (int j = 0; j < 16; ++j) acc[j] = 0.f;
+    // the next chunk's eight entries per thread are loaded while this chunk's products run
+    float vn[8];
+    auto load_chunk = [&](int r0) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int q = tid + e * BT, rr = q / 32, t = q % 32, r = r0 + rr;
+            vn[e] = (t < nb && r < L) ? refl_elem(P, side, i0 + t, r) : 0.f;
+        }
+    };
+    load_chunk(i0 + side);
     for (int r0 = i0 + side; r0 < L; r0 += 64) {
-        for (int q = tid; q < 64 * 32; q += BT) {
-            const int rr = q / 32, t = q % 32, r = r0 + rr;
-            Vs[rr][t] = (t < nb && r < L) ? refl_elem(P, side, i0 + t, r) : 0.f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int q = tid + e * BT;
+            Vs[q / 32][q % 32] = vn[e];
         }
         __syncthreads();
+        if (r0 + 64 < L) load_chunk(r0 + 64);
 #pragma unroll
         for (int kk = 0; kk < 8; ++kk) {
             const float v = Vs[16 * w + 2 * kk + lh][l32];
