@@ -725,10 +725,10 @@ __global__ __launch_bounds__(BT) void identity_kernel(ProbList pl, const int *ro
     float *X = side == 0 ? P.u : P.vt;
     const int r0 = g * ROW_GROUP, nr = min(ROW_GROUP, L - r0);
     if (nr <= 0) return;
-    for (int q = threadIdx.x; q < nr * L; q += BT) {
-        const int r = r0 + q / L, c = q % L;
-        X[(size_t)r * L + c] = (c == r) ? 1.f : 0.f;
-    }
+    // 32 threads per row (ROW_GROUP * 32 == BT)
+    const int r = r0 + (int)(threadIdx.x >> 5);
+    if (r >= r0 + nr) return;
+    for (int c = threadIdx.x & 31; c < L; c += 32) X[(size_t)r * L + c] = (c == r) ? 1.f : 0.f;
 }
 
 // In-place transpose of vt (N x N) by 32x32 tile pairs; grid over (problem, tile pair).

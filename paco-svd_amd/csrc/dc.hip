@@ -876,15 +876,12 @@ __global__ __launch_bounds__(DT) void dc_bt_copy_kernel(ProbList pl, const int *
     const int M = P.M, N = P.N, r0 = g * ROW_GROUP;
     const int nr = min(ROW_GROUP, (side == 0 ? M : N) - r0);
     if (nr <= 0) return;
-    if (side == 0) {
-        for (int q = threadIdx.x; q < nr * N; q += DT) {
-            const int r = r0 + q / N, c = q % N;
-            P.u[(size_t)r * M + c] = P.a[(size_t)r * N + c];
-        }
+    if (side == 0) { // 32 threads per row (ROW_GROUP * 32 == DT)
+        const int r = r0 + (int)(threadIdx.x >> 5);
+        if (r >= r0 + nr) return;
+        for (int c = threadIdx.x & 31; c < N; c += 32) P.u[(size_t)r * M + c] = P.a[(size_t)r * N + c];
     } else {
-        const float *src = P.qt + (size_t)N * N + (size_t)r0 * N;
-        float *dst = P.vt + (size_t)r0 * N;
-        for (int q = threadIdx.x; q < nr * N; q += DT) dst[q] = src[q];
+        copy_block(P.vt + (size_t)r0 * N, P.qt + (size_t)N * N + (size_t)r0 * N, nr * N);
     }
 }
 

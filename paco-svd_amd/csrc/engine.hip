@@ -42,9 +42,7 @@ __global__ __launch_bounds__(256) void copy_in_kernel(ProbList pl, const float *
     const int r0 = g * ROW_GROUP, nr = min(ROW_GROUP, P.M - r0);
     if (nr <= 0) return;
     // the rows of a group are contiguous in both (leading dimension N)
-    const float *s = src[pid] + (size_t)r0 * P.N;
-    float *d = P.a + (size_t)r0 * P.N;
-    for (int q = threadIdx.x; q < nr * P.N; q += 256) d[q] = s[q];
+    copy_block(P.a + (size_t)r0 * P.N, src[pid] + (size_t)r0 * P.N, nr * P.N);
 }
 
 // ---------------------------------------------------------------------------

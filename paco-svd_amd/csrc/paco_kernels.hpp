@@ -68,6 +68,21 @@ hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int s
 // Row-wise kernels (identity, copies) take ROW_GROUP rows per workgroup: one workgroup
 // per row made the launch bound by workgroup dispatch.  Their offsets count groups.
 constexpr int ROW_GROUP = 8;
+static_assert(ROW_GROUP * 32 == 256, "row-wise kernels: 32 threads per row of a 256-thread workgroup");
+// n contiguous floats by one workgroup of 256 threads: 16-byte vectors when both ends are
+// aligned, else single floats
+__device__ inline void copy_block(float *dst, const float *src, int n)
+{
+    if ((((uintptr_t)dst | (uintptr_t)src) & 15) == 0) {
+        const int n4 = n >> 2;
+        float4 *d4 = reinterpret_cast<float4 *>(dst);
+        const float4 *s4 = reinterpret_cast<const float4 *>(src);
+        for (int q = threadIdx.x; q < n4; q += 256) d4[q] = s4[q];
+        for (int q = 4 * n4 + threadIdx.x; q < n; q += 256) dst[q] = src[q];
+    } else {
+        for (int q = threadIdx.x; q < n; q += 256) dst[q] = src[q];
+    }
+}
 // X = I; row_off counts groups of ROW_GROUP rows per problem
 hipError_t launch_identity(ProbList pl, const int *row_off, int ngroups, int side, hipStream_t s);
 hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hipStream_t s);
