@@ -169,6 +169,21 @@ __device__ inline void pass_group(T *base, int ld, int rbase, int r0, int k, int
     }
 }
 
+// Pairwise sum of the NW values p[0], p[stride], ..., p[(NW - 1) stride] (NW a power of two),
+// in a fixed order, so every thread that sums the same values gets the same result.
+template <int NW, typename T, typename P>
+__device__ inline T tree_sum(const P *p, int stride)
+{
+    T v[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) v[w] = p[(size_t)w * stride];
+#pragma unroll
+    for (int h = NW / 2; h > 0; h >>= 1)
+#pragma unroll
+        for (int w = 0; w < h; ++w) v[w] += v[w + h];
+    return v[0];
+}
+
 // Fused pass over rows r > i with C = ceil((N - i - 1) / 64) chunks per row.  Rows are
 // taken in groups of RQ at multiples of RQ, group g by wavefront g % NW, so that every
 // wavefront visits its rows in increasing order whatever the LDS split (the reductions
@@ -294,9 +309,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
         if (lane == 0) dred[wv] = ss;
         __syncthreads();
         // ---- column reflector of column i
-        double sg = 0.0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) sg += dred[w];
+        const double sg = tree_sum<NW, double>(dred, 1);
         float beta;
         house_params_d(rowbuf[i], sg, &beta, &tau, &inv);
         if (tid == 0) {
@@ -317,9 +330,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
             const int j = i + 1 + tid + t * NT;
             float y = 0.f;
             if (j < N) {
-                float a = 0.f;
-#pragma unroll
-                for (int w = 0; w < NW; ++w) a += pwr[(size_t)w * N + j];
+                const float a = tree_sum<NW, float>(pwr + j, N);
                 const float wj = (tau != 0.f) ? tau * (rowbuf[j] + inv * a) : 0.f;
                 wbuf[j] = wj;
                 y = rowbuf[j] - wj;
@@ -333,9 +344,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
         if (lane == 0) dred[NW + wv] = part;
         __syncthreads();
         // ---- row reflector of row i
-        double sigma2 = 0.0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) sigma2 += dred[NW + w];
+        const double sigma2 = tree_sum<NW, double>(dred + NW, 1);
         float beta2;
         house_params_d(dred[2 * NW], sigma2, &beta2, &tau2, &inv2);
 #pragma unroll

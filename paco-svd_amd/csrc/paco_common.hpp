@@ -193,13 +193,42 @@ __device__ inline double wave_max(double v)
 // would leave tau inconsistent with the stored vector c / v1 (H = I - tau v v^T no longer
 // orthogonal).  When tau rounds to 0 in FP32 or 1/v1 overflows it, the reflector is the
 // identity (tau = 0, beta = x1), so no stored vector entry can be infinite.
+//
+// The operands of the square root and the divisions are nonzero normal doubles (a nonzero
+// sum of at most 2^12 squares of FP32 values lies in [2^-298, 2^268]; v1 < 0 always), so
+// they are formed from the hardware estimates without the range scaling and special cases
+// of the general routines: 1/b by two Newton steps on v_rcp_f64, a/b by a residual
+// correction of a * (1/b), sqrt(s) by Goldschmidt's iteration on v_rsq_f64 with two residual
+// corrections.  Each is within an ulp or two of FP64, far below the FP32 rounding of the
+// results.
+__device__ inline double rcp_normal_d(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return fma(r, fma(-b, r, 1.0), r);
+}
+__device__ inline double div_normal_d(double a, double b)
+{
+    const double r = rcp_normal_d(b), q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+__device__ inline double sqrt_normal_d(double s)
+{
+    const double y = __builtin_amdgcn_rsq(s);
+    double g = s * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    g = fma(fma(-g, g, s), h, g);
+    return fma(fma(-g, g, s), h, g);
+}
 __device__ inline void house_params_d(double x1, double sigma, float *beta, float *tau, float *inv)
 {
     if (sigma > 0.0) {
-        const double mu = sqrt(x1 * x1 + sigma);
-        const double v1 = (x1 <= 0.0) ? x1 - mu : -sigma / (x1 + mu);
+        const double mu = sqrt_normal_d(fma(x1, x1, sigma));
+        const double v1 = (x1 <= 0.0) ? x1 - mu : -div_normal_d(sigma, x1 + mu);
         const double v1sq = v1 * v1;
-        const float t = (float)(2.0 * v1sq / (sigma + v1sq)), iv = (float)(1.0 / v1);
+        const float t = (float)div_normal_d(2.0 * v1sq, sigma + v1sq), iv = (float)rcp_normal_d(v1);
         if (t != 0.f && fabsf(iv) <= 3.0e38f) {
             *tau = t;
             *inv = iv;
