@@ -176,7 +176,7 @@ __device__ inline T tree_sum(const P *p, int stride)
 {
     T v[NW];
 #pragma unroll
-    for (int w = 0; w < NW; ++w) v[w] = p[(size_t)w * stride];
+    for (int w = 0; w < NW; ++w, p += stride) v[w] = *p;
 #pragma unroll
     for (int h = NW / 2; h > 0; h >>= 1)
 #pragma unroll

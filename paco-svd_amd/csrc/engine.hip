@@ -353,8 +353,10 @@ static BatchScratch scratch_layout(int M, int N)
 // LDS classes (0..3: <= 1/8, 1/4, 1/2, all of the LDS) whose resident matrices take the
 // one-wavefront kernel by default
 static constexpr int kWaveMaxClassDefault = -1;
-// workgroup size of the LDS-resident launches per LDS class
-static constexpr int kResThreadsDefault[4] = {1024, 1024, 1024, 1024};
+// workgroup size of the LDS-resident launches per LDS class: the two small classes hold up
+// to 8 and 4 matrices per CU's LDS, and at 512 threads two of them share a CU, so the
+// per-column section that every wavefront repeats runs on 8 wavefronts per matrix, not 16
+static constexpr int kResThreadsDefault[4] = {512, 512, 1024, 1024};
 
 // Host-side plan of a batch: a byte blob (uploaded once) + launch sizes.
 struct BatchPlan {
@@ -463,7 +465,8 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     const int wave_max_class = (wenv && wenv[0] == '0') ? -1 : cenv ? atoi(cenv) : kWaveMaxClassDefault;
     // PACO_RES_THREADS: workgroup size of the resident launches, one value for all LDS
     // classes or four comma-separated ones (classes <= 1/8, 1/4, 1/2, all of the LDS);
-    // 1024 measured best (bidiagonalization 62.1 ms against 63.7 at 512, 66.7 at 256)
+    // 512,512,1024,1024 measured best (bidiagonalization 59.3 ms against 60.1 at 1024 for
+    // all classes; one value for all: 62.1 at 1024, 63.7 at 512, 66.7 at 256)
     for (int cl = 0; cl < BatchPlan::kWaveClasses; ++cl) bp.res_threads[cl] = kResThreadsDefault[cl];
     if (const char *tenv = getenv("PACO_RES_THREADS")) {
         const char *p = tenv;
