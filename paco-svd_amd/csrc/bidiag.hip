@@ -32,8 +32,8 @@ namespace paco {
 // Rows [split, M) are kept in LDS (all rows when the matrix fits), the rest is
 // streamed; every reduction runs in an order that does not depend on the split.
 // Sums of squares of the Householder vectors are taken in FP64 (see house_params_d).
-// LDS: FP64 partial sums [2 NW + 2], [cached rows, leading dimension N|1] cbuf[M] wbuf[N]
-//      ubuf[N] rowbuf[N] (spare N) pwr[NW][N] (+ spare).
+// LDS: FP64 partial sums [2 NW + 2], cbuf[M] wbuf[N] ubuf[N] rowbuf[N] (spare N) pwr[NW][N]
+//      (+ spare), then the cached rows (columns [c0, N), leading dimension (N - c0) | 1).
 // ---------------------------------------------------------------------------
 // LDS- and global-typed pointers: accesses through them compile to LDS (ds_*) and
 // global (global_*) instructions, never to FLAT ones that must wait on both counters.
@@ -85,10 +85,11 @@ struct PassRows {
 // last chunk can reach past column N - 1, so every other chunk is read and written
 // without a test, at immediate offsets from one address per row.
 //
-// Loads the group of RQ rows starting at row r0 into x.  base points at row rbase; rows
-// outside [k, lim) are replaced by the nearest row inside (read, never written back).
+// Loads the group of RQ rows starting at row r0 into x.  base points at column cb of row
+// rbase; rows outside [k, lim) are replaced by the nearest row inside (read, never written
+// back).
 template <int C, int RQ, typename T>
-__device__ inline void pass_load(const T *base, int ld, int rbase, int r0, int k, int lim, int N, int lane,
+__device__ inline void pass_load(const T *base, int ld, int rbase, int cb, int r0, int k, int lim, int N, int lane,
                                  float (&x)[RQ][C])
 {
     const bool tail = k + lane + 64 * (C - 1) < N;
@@ -97,7 +98,7 @@ __device__ inline void pass_load(const T *base, int ld, int rbase, int r0, int k
         int rr = r0 + q;
         rr = rr < k ? k : rr;
         rr = rr >= lim ? lim - 1 : rr;
-        const T *row = base + (size_t)(rr - rbase) * ld + k + lane;
+        const T *row = base + (size_t)(rr - rbase) * ld + (k - cb) + lane;
 #pragma unroll
         for (int c = 0; c < C - 1; ++c) x[q][c] = row[64 * c];
         x[q][C - 1] = tail ? row[64 * (C - 1)] : 0.f;
@@ -110,7 +111,7 @@ __device__ inline void pass_load(const T *base, int ld, int rbase, int r0, int k
 // squares (FP64, lane 0) and the pw partials of this lane (the entry of column k itself
 // is never read).  Columns past N - 1 hold zeros (w, u are zero there).
 template <int C, int RQ, typename T>
-__device__ inline void pass_group(T *base, int ld, int rbase, int r0, int k, int M, int N, int lane, bool upd,
+__device__ inline void pass_group(T *base, int ld, int rbase, int cb, int r0, int k, int M, int N, int lane, bool upd,
                                   float tau, float inv, float tau2, const float (&wr)[C], const float (&ur)[C],
                                   float (&x)[RQ][C], lds_f32 *cbuf, lds_f32 *rowbuf, float (&pwp)[C], double &ss)
 {
@@ -131,7 +132,7 @@ __device__ inline void pass_group(T *base, int ld, int rbase, int r0, int k, int
                 a += x[q][c] * ur[c];
             }
             acc[q] = a;
-            if (on && lane == 0) base[(size_t)(rr - rbase) * ld + i] = (tau != 0.f) ? vr : cr;
+            if (on && lane == 0) base[(size_t)(rr - rbase) * ld + (i - cb)] = (tau != 0.f) ? vr : cr;
         }
 #pragma unroll
         for (int q = 0; q < RQ; ++q) acc[q] = wave_sum_dpp(acc[q]) * tau2;
@@ -141,7 +142,7 @@ __device__ inline void pass_group(T *base, int ld, int rbase, int r0, int k, int
 #pragma unroll
             for (int c = 0; c < C; ++c) x[q][c] -= acc[q] * ur[c];
             if (rr >= k && rr < M) {
-                T *row = base + (size_t)(rr - rbase) * ld + k + lane;
+                T *row = base + (size_t)(rr - rbase) * ld + (k - cb) + lane;
 #pragma unroll
                 for (int c = 0; c < C - 1; ++c) row[64 * c] = x[q][c];
                 if (tail) row[64 * (C - 1)] = x[q][C - 1];
@@ -190,7 +191,7 @@ __device__ inline T tree_sum(const P *p, int stride)
 // do not depend on it) and no group straddles the split (a multiple of RQ below M).
 // Wave partials of pw go to pwr[wave]; the return value is lane 0's partial sum of squares.
 template <int NT, int C>
-__device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int split, int M, int N, int i,
+__device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0, int split, int M, int N, int i,
                                       float tau, float inv, float tau2, float inv2, lds_f32 *cbuf,
                                       const lds_f32 *wbuf, const lds_f32 *ybuf, lds_f32 *rowbuf, lds_f32 *pwr)
 {
@@ -211,21 +212,21 @@ __device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G,
     int g = g0 + ((wv - g0 % NW) % NW + NW) % NW; // first group of this wavefront
     float xa[RQ][C], xb[RQ][C];
     // rows in global memory: [k, split)
-    if (RQ * g < split) pass_load<C, RQ>(G, ldg, 0, RQ * g, k, split, N, lane, xa);
+    if (RQ * g < split) pass_load<C, RQ>(G, ldg, 0, 0, RQ * g, k, split, N, lane, xa);
     for (; RQ * g < split; g += NW) {
-        if (RQ * (g + NW) < split) pass_load<C, RQ>(G, ldg, 0, RQ * (g + NW), k, split, N, lane, xb);
-        pass_group<C, RQ>(G, ldg, 0, RQ * g, k, M, N, lane, upd, tau, inv, tau2, wr, ur, xa, cbuf, rowbuf, pwp, ss);
+        if (RQ * (g + NW) < split) pass_load<C, RQ>(G, ldg, 0, 0, RQ * (g + NW), k, split, N, lane, xb);
+        pass_group<C, RQ>(G, ldg, 0, 0, RQ * g, k, M, N, lane, upd, tau, inv, tau2, wr, ur, xa, cbuf, rowbuf, pwp, ss);
 #pragma unroll
         for (int q = 0; q < RQ; ++q)
 #pragma unroll
             for (int c = 0; c < C; ++c) xa[q][c] = xb[q][c];
     }
     // rows in LDS: [max(k, split), M)
-    if (RQ * g < M) pass_load<C, RQ>(L, ldl, split, RQ * g, k, M, N, lane, xa);
+    if (RQ * g < M) pass_load<C, RQ>(L, ldl, split, c0, RQ * g, k, M, N, lane, xa);
     for (; RQ * g < M; g += NW) {
-        if (RQ * (g + NW) < M) pass_load<C, RQ>(L, ldl, split, RQ * (g + NW), k, M, N, lane, xb);
-        pass_group<C, RQ>(L, ldl, split, RQ * g, k, M, N, lane, upd, tau, inv, tau2, wr, ur, xa, cbuf, rowbuf, pwp,
-                          ss);
+        if (RQ * (g + NW) < M) pass_load<C, RQ>(L, ldl, split, c0, RQ * (g + NW), k, M, N, lane, xb);
+        pass_group<C, RQ>(L, ldl, split, c0, RQ * g, k, M, N, lane, upd, tau, inv, tau2, wr, ur, xa, cbuf, rowbuf,
+                          pwp, ss);
 #pragma unroll
         for (int q = 0; q < RQ; ++q)
 #pragma unroll
@@ -240,12 +241,13 @@ __device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G,
 }
 
 template <int NT>
-__device__ __attribute__((always_inline)) inline double fused_pass(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int split, int M, int N, int i,
+__device__ __attribute__((always_inline)) inline double fused_pass(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0, int split, int M, int N, int i,
                                     float tau, float inv, float tau2, float inv2, lds_f32 *cbuf,
                                     const lds_f32 *wbuf, const lds_f32 *ybuf, lds_f32 *rowbuf, lds_f32 *pwr)
 {
 #define PACO_PASS(CC)                                                                                              \
-    return fused_pass_c<NT, CC>(G, ldg, L, ldl, split, M, N, i, tau, inv, tau2, inv2, cbuf, wbuf, ybuf, rowbuf, pwr)
+    return fused_pass_c<NT, CC>(G, ldg, L, ldl, c0, split, M, N, i, tau, inv, tau2, inv2, cbuf, wbuf, ybuf, rowbuf, \
+                                pwr)
     switch ((N - i - 1 + 63) >> 6) {
     case 1: PACO_PASS(1);
     case 2: PACO_PASS(2);
@@ -259,13 +261,13 @@ __device__ __attribute__((always_inline)) inline double fused_pass(glb_f32 *G, i
 #undef PACO_PASS
 }
 
-// element (r, j) of the matrix whose rows [split, M) are in LDS
-__device__ inline void store_elem(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int split, int r, int j, float v)
+// element (r, j) of the matrix whose rows [split, M) are in LDS from column c0 on (j >= c0)
+__device__ inline void store_elem(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0, int split, int r, int j, float v)
 {
     if (r < split)
         G[(size_t)r * ldg + j] = v;
     else
-        L[(size_t)(r - split) * ldl + j] = v;
+        L[(size_t)(r - split) * ldl + (j - c0)] = v;
 }
 
 // 4 wavefronts per SIMD whatever NT: up to 4 workgroups of 256 threads or 2 of 512 share
@@ -285,11 +287,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
     // the row reflector
     double *dred = reinterpret_cast<double *>(smem);
     lds_f32 *sm = lsm + 2 * (2 * NW + 2);
-    const int split = bidiag_split_row(M, N, NT, lds_bytes), nl = M - split, ldl = N | 1;
-    glb_f32 *G = (glb_f32 *)P.a;
-    lds_f32 *L = sm;
-    lds_f32 *cbuf = sm + (size_t)nl * ldl, *wbuf = cbuf + M, *ubuf = wbuf + N, *rowbuf = ubuf + N;
+    lds_f32 *cbuf = sm, *wbuf = cbuf + M, *ubuf = wbuf + N, *rowbuf = ubuf + N;
     lds_f32 *pwr = rowbuf + 2 * N;
+    // rows [split, M) of A, columns [c0, N), leading dimension ldl, after all the vectors
+    const size_t vecf = bidiag_vec_floats(M, N, NT);
+    lds_f32 *L = lsm + vecf;
+    const int cap = lds_bytes / sizeof(float) > vecf ? (int)(lds_bytes / sizeof(float) - vecf) : 0;
+    int split = bidiag_split_row(M, N, NT, lds_bytes), ldl = N | 1, c0 = 0;
+    glb_f32 *G = (glb_f32 *)P.a;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int r = split + wv; r < M; r += NW)
         for (int c = lane; c < N; c += 64) L[(size_t)(r - split) * ldl + c] = G[(size_t)r * ldg + c];
@@ -302,8 +307,35 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
     constexpr int JPT = (WG_MAXN + NT - 1) / NT; // columns of row i per thread
     float tau = 0.f, inv = 1.f, tau2 = 0.f, inv2 = 1.f;
     for (int i = -1;;) {
+        // As the trailing matrix narrows, more of its rows fit in LDS at leading dimension
+        // (N - i) | 1: once at least max(16, 1/8 of them) more rows fit, the cached rows go
+        // back to A and rows [split', M), columns [i, N) are cached instead (split' a
+        // multiple of WG_SPLIT_ALIGN, never above the first live row's multiple).  The pass
+        // visits rows in an order that does not depend on the split, so the results do not
+        // change; only fewer rows are streamed.  The decision depends on (i, split) alone,
+        // so every thread takes it alike.
+        if (i >= 0 && split > i + 1) {
+            const int ldl2 = (N - i) | 1, rows2 = min(M, cap / ldl2);
+            const int lo = (i + 1) / WG_SPLIT_ALIGN * WG_SPLIT_ALIGN;
+            const int s2 = max(lo, (M - rows2 + WG_SPLIT_ALIGN - 1) / WG_SPLIT_ALIGN * WG_SPLIT_ALIGN);
+            if (split - s2 >= max(16, rows2 / 8)) {
+                // row i and the entries stored since the last barrier are visible to all
+                __syncthreads();
+                for (int r = split + wv; r < M; r += NW)
+                    for (int c = c0 + lane; c < N; c += 64)
+                        G[(size_t)r * ldg + c] = L[(size_t)(r - split) * ldl + (c - c0)];
+                __syncthreads();
+                split = s2;
+                ldl = ldl2;
+                c0 = i;
+                for (int r = split + wv; r < M; r += NW)
+                    for (int c = c0 + lane; c < N; c += 64)
+                        L[(size_t)(r - split) * ldl + (c - c0)] = G[(size_t)r * ldg + c];
+                __syncthreads();
+            }
+        }
         const double ss =
-            fused_pass<NT>(G, ldg, L, ldl, split, M, N, i, tau, inv, tau2, inv2, cbuf, wbuf, ubuf, rowbuf, pwr);
+            fused_pass<NT>(G, ldg, L, ldl, c0, split, M, N, i, tau, inv, tau2, inv2, cbuf, wbuf, ubuf, rowbuf, pwr);
         ++i;
         // the previous column's readers of dred finished before the barrier ahead of the pass
         if (lane == 0) dred[wv] = ss;
@@ -313,13 +345,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
         float beta;
         house_params_d(rowbuf[i], sg, &beta, &tau, &inv);
         if (tid == 0) {
-            store_elem(G, ldg, L, ldl, split, i, i, beta);
+            store_elem(G, ldg, L, ldl, c0, split, i, i, beta);
             P.d[i] = beta;
             P.tauq[i] = tau;
         }
         if (i == N - 1) {
             for (int r = i + 1 + tid; r < M; r += NT)
-                store_elem(G, ldg, L, ldl, split, r, i, (tau != 0.f) ? cbuf[r] * inv : cbuf[r]);
+                store_elem(G, ldg, L, ldl, c0, split, r, i, (tau != 0.f) ? cbuf[r] * inv : cbuf[r]);
             break;
         }
         // ---- w = tau (row i + (1/v1) pw) and y = row i - w on this thread's columns
@@ -350,7 +382,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
 #pragma unroll
         for (int t = 0; t < JPT; ++t) {
             const int j = i + 1 + tid + t * NT;
-            if (j < N) store_elem(G, ldg, L, ldl, split, i, j, j == i + 1 ? beta2 : (tau2 != 0.f) ? yv[t] * inv2 : yv[t]);
+            if (j < N) store_elem(G, ldg, L, ldl, c0, split, i, j, j == i + 1 ? beta2 : (tau2 != 0.f) ? yv[t] * inv2 : yv[t]);
         }
         if (tid == 0) {
             P.e[i] = beta2;
@@ -361,7 +393,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
     }
     __syncthreads();
     for (int r = split + wv; r < M; r += NW)
-        for (int c = lane; c < N; c += 64) G[(size_t)r * ldg + c] = L[(size_t)(r - split) * ldl + c];
+        for (int c = c0 + lane; c < N; c += 64) G[(size_t)r * ldg + c] = L[(size_t)(r - split) * ldl + (c - c0)];
 }
 
 // ---------------------------------------------------------------------------
