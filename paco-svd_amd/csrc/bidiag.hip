@@ -43,6 +43,11 @@ typedef __attribute__((address_space(1))) float glb_f32;
 constexpr int WG_MAXN = 512; // 8 chunks of 64 columns per lane
 // the first LDS-cached row is a multiple of this (every group size of the pass divides it)
 constexpr int WG_SPLIT_ALIGN = 8;
+// re-pack the cached rows once max(PACO_COMPACT_MIN, rows / PACO_COMPACT_DIV) more fit
+#ifndef PACO_COMPACT_MIN
+#define PACO_COMPACT_MIN 16
+#define PACO_COMPACT_DIV 8
+#endif
 #ifndef RQ1024_2
 #define RQ1024_2 4
 #define RQ1024_4 2
@@ -318,7 +323,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
             const int ldl2 = (N - i) | 1, rows2 = min(M, cap / ldl2);
             const int lo = (i + 1) / WG_SPLIT_ALIGN * WG_SPLIT_ALIGN;
             const int s2 = max(lo, (M - rows2 + WG_SPLIT_ALIGN - 1) / WG_SPLIT_ALIGN * WG_SPLIT_ALIGN);
-            if (split - s2 >= max(16, rows2 / 8)) {
+            if (split - s2 >= max(PACO_COMPACT_MIN, rows2 / PACO_COMPACT_DIV)) {
                 // row i and the entries stored since the last barrier are visible to all
                 __syncthreads();
                 for (int r = split + wv; r < M; r += NW)
