@@ -190,6 +190,23 @@ __device__ inline T tree_sum(const P *p, int stride)
     return v[0];
 }
 
+// True in the wavefront that arrives last at this point of the workgroup (every one of the
+// NW wavefronts passes it once, then a barrier follows); that wavefront also resets the LDS
+// counter, which is next used after the barrier.  Writes to LDS made before the call by any
+// wavefront are visible to the last one after it.
+template <int NW>
+__device__ inline bool arrive_last(unsigned *cnt, int lane)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    unsigned old = 0;
+    if (lane == 0) old = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    old = __builtin_amdgcn_readfirstlane(old);
+    if (old != NW - 1) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (lane == 0) *cnt = 0;
+    return true;
+}
+
 // Fused pass over rows r > i with C = ceil((N - i - 1) / 64) chunks per row.  Rows are
 // taken in groups of RQ at multiples of RQ, group g by wavefront g % NW, so that every
 // wavefront visits its rows in increasing order whatever the LDS split (the reductions
@@ -294,6 +311,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
     lds_f32 *sm = lsm + 2 * (2 * NW + 2);
     lds_f32 *cbuf = sm, *wbuf = cbuf + M, *ubuf = wbuf + N, *rowbuf = ubuf + N;
     lds_f32 *pwr = rowbuf + 2 * N;
+    // reflector parameters published by the last-arriving wavefront: [0..2] beta, tau, 1/v1
+    // of the column reflector, [4..6] of the row reflector, [7] the arrival counter
+    const size_t poff = 2 * (2 * NW + 2) + (size_t)M + 4 * (size_t)N + (size_t)NW * N;
+    float *prm = smem + poff;
+    unsigned *wcnt = reinterpret_cast<unsigned *>(smem + poff + 7);
     // rows [split, M) of A, columns [c0, N), leading dimension ldl, after all the vectors
     const size_t vecf = bidiag_vec_floats(M, N, NT);
     lds_f32 *L = lsm + vecf;
@@ -303,12 +325,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int r = split + wv; r < M; r += NW)
         for (int c = lane; c < N; c += 64) L[(size_t)(r - split) * ldl + c] = G[(size_t)r * ldg + c];
+    if (tid == 0) *wcnt = 0;
     __syncthreads();
 
     // pass i = -1 is the read-only gathering pass of column 0; pass i >= 0 applies the
     // reflectors of column i to rows > i and gathers column i + 1.  Per column two
     // barriers: after the pass; after the row reflector's partial sums (and y, w).
-    // Every thread forms both reflectors' parameters from the same sums in the same order.
+    // The wavefront that reaches a barrier last forms the reflector's FP64 parameters from
+    // the partial sums (always in the same order) and publishes them in LDS: the other
+    // wavefronts, which wait at the barrier anyway, do not repeat the FP64 section.
     constexpr int JPT = (WG_MAXN + NT - 1) / NT; // columns of row i per thread
     float tau = 0.f, inv = 1.f, tau2 = 0.f, inv2 = 1.f;
     for (int i = -1;;) {
@@ -344,11 +369,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
         ++i;
         // the previous column's readers of dred finished before the barrier ahead of the pass
         if (lane == 0) dred[wv] = ss;
-        __syncthreads();
         // ---- column reflector of column i
-        const double sg = tree_sum<NW, double>(dred, 1);
-        float beta;
-        house_params_d(rowbuf[i], sg, &beta, &tau, &inv);
+        if (arrive_last<NW>(wcnt, lane)) {
+            const double sg = tree_sum<NW, double>(dred, 1);
+            float b, t, v;
+            house_params_d(rowbuf[i], sg, &b, &t, &v);
+            if (lane == 0) {
+                prm[0] = b;
+                prm[1] = t;
+                prm[2] = v;
+            }
+        }
+        __syncthreads();
+        const float beta = prm[0];
+        tau = prm[1];
+        inv = prm[2];
         if (tid == 0) {
             store_elem(G, ldg, L, ldl, c0, split, i, i, beta);
             P.d[i] = beta;
@@ -379,11 +414,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
         }
         part = wave_sum_dpp_d(part);
         if (lane == 0) dred[NW + wv] = part;
-        __syncthreads();
         // ---- row reflector of row i
-        const double sigma2 = tree_sum<NW, double>(dred + NW, 1);
-        float beta2;
-        house_params_d(dred[2 * NW], sigma2, &beta2, &tau2, &inv2);
+        if (arrive_last<NW>(wcnt, lane)) {
+            const double sigma2 = tree_sum<NW, double>(dred + NW, 1);
+            float b, t, v;
+            house_params_d(dred[2 * NW], sigma2, &b, &t, &v);
+            if (lane == 0) {
+                prm[4] = b;
+                prm[5] = t;
+                prm[6] = v;
+            }
+        }
+        __syncthreads();
+        const float beta2 = prm[4];
+        tau2 = prm[5];
+        inv2 = prm[6];
 #pragma unroll
         for (int t = 0; t < JPT; ++t) {
             const int j = i + 1 + tid + t * NT;
