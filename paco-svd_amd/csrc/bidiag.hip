@@ -412,7 +412,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
             }
             yv[t] = y;
         }
-        part = wave_sum_dpp_d(part);
+        // a wavefront without columns (the first is column i + 1 + 64 wv) has part = 0
+        if (i + 1 + 64 * wv < N) part = wave_sum_dpp_d(part);
         if (lane == 0) dred[NW + wv] = part;
         // ---- row reflector of row i
         if (arrive_last<NW>(wcnt, lane)) {
