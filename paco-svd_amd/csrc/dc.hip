@@ -135,7 +135,7 @@ struct MergeCtx {
 __device__ inline MergeCtx merge_ctx(const ProbList &pl, const LevelArgs &la, int t)
 {
     MergeCtx c;
-    c.prob = find_task(la.task_off, pl.count, t);
+    c.prob = la.task_prob ? la.task_prob[t] : find_task(la.task_off, pl.count, t);
     const int j = t - off_at(la.task_off, c.prob);
     merge_cols(pl.get(c.prob).N, la.r, j, c.c0, c.c1, c.c2);
     c.S = la.states + t;

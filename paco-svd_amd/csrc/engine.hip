@@ -236,6 +236,7 @@ struct DcOffsets {
     const int *leaf_off = nullptr;
     int nleaf = 0;
     std::vector<const int *> task_off; // per round
+    std::vector<const int *> task_prob; // per round (empty: search task_off)
     std::vector<int> ntask, maxn;
     std::vector<MergeState *> states;
     const int *bt_tu = nullptr, *bt_tv = nullptr, *bt_ru = nullptr, *bt_rv = nullptr;
@@ -249,6 +250,7 @@ static hipError_t run_dc_levels(ProbList pl, const DcOffsets &o, const uint32_t 
     for (size_t r = 1; r <= o.ntask.size(); ++r) {
         LevelArgs la;
         la.task_off = o.task_off[r - 1];
+        la.task_prob = o.task_prob.empty() ? nullptr : o.task_prob[r - 1];
         la.states = o.states[r - 1];
         la.ntask = o.ntask[r - 1];
         la.r = (int)r;
@@ -381,7 +383,7 @@ struct BatchPlan {
     int res_threads[kWaveClasses] = {1024, 1024, 1024, 1024};
     size_t o_rowin = 0, o_idq = 0, o_idp = 0, o_bq = 0, o_bp = 0, o_sq = 0, o_sp = 0, o_tp = 0;
     size_t o_leaf = 0, o_tu = 0, o_tv = 0, o_ru = 0, o_rv = 0;
-    std::vector<size_t> o_task;
+    std::vector<size_t> o_task, o_tprob;
     int nrowin = 0, nidq = 0, nidp = 0, nbq = 0, nbp = 0, nsq = 0, nsp = 0, ntp = 0, nleaf = 0;
     int ntu = 0, ntv = 0, nru = 0, nrv = 0;
     std::vector<int> ntask, maxn;
@@ -560,6 +562,7 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     int R = 0;
     for (int i = 0; i < count; ++i) R = std::max(R, rounds_of(jobs[i].N));
     bp.o_task.clear();
+    bp.o_tprob.clear();
     bp.ntask.clear();
     bp.maxn.clear();
     bp.state_off.clear();
@@ -567,6 +570,11 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     for (int r = 1; r <= R; ++r) {
         int nt = 0;
         bp.o_task.push_back(blob_put(b, prefix(count, [&](int i) { return merges_in_round(jobs[i].N, r); }, nt)));
+        std::vector<int> tprob;
+        tprob.reserve(nt);
+        for (int i = 0; i < count; ++i)
+            for (int q = merges_in_round(jobs[i].N, r); q > 0; --q) tprob.push_back(i);
+        bp.o_tprob.push_back(blob_put(b, tprob));
         bp.ntask.push_back(nt);
         int mx = 0;
         for (int i = 0; i < count; ++i)
@@ -757,6 +765,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     MergeState *st = reinterpret_cast<MergeState *>(dev + pr.o_states);
     for (size_t r = 0; r < bp.ntask.size(); ++r) {
         o.task_off.push_back(I(bp.o_task[r]));
+        o.task_prob.push_back(I(bp.o_tprob[r]));
         o.ntask.push_back(bp.ntask[r]);
         o.maxn.push_back(bp.maxn[r]);
         o.states.push_back(st + bp.state_off[r]);

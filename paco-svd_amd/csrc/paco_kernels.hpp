@@ -15,10 +15,12 @@ struct ProbList {
 
 // One D&C level (merge round r) over every problem of the launch.  Merge t belongs to
 // problem p with task_off[p] <= t < task_off[p+1] (task_off may be null for one
-// problem); its columns follow from (N, r, t - task_off[p]).  stride bounds the rows
-// and columns of every merge of the round (min(3 * 2^r, largest N of the launch)).
+// problem); its columns follow from (N, r, t - task_off[p]).  task_prob[t] = p when
+// given (one load instead of a binary search over task_off per workgroup).  stride bounds
+// the rows and columns of every merge of the round (min(3 * 2^r, largest N of the launch)).
 struct LevelArgs {
     const int *task_off;
+    const int *task_prob; // [ntask] or null
     MergeState *states; // [ntask]
     int ntask, r, stride;
     int maxn; // largest merged row count of the level (sizes the deflation LDS)
