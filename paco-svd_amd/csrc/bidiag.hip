@@ -646,10 +646,11 @@ __device__ inline int refl_len(const ProbDesc &P, int side) { return side == 0 ?
 __device__ inline float refl_tau(const ProbDesc &P, int side, int i) { return side == 0 ? P.tauq[i] : P.taup[i]; }
 
 // T factor (forward, columnwise; H_b ... H_{b+nb-1} = I - V T V^T) for every block of
-// 32 reflectors.  One workgroup per (problem, block); grid mapping through offsets.
+// 32 reflectors.  One workgroup per (problem, block); grid mapping through offsets that
+// carry the block -> problem table (task_of).
 __global__ __launch_bounds__(BT) void tfactor_kernel(ProbList pl, const int *blk_off, int side)
 {
-    const int pid = find_task(blk_off, pl.count, blockIdx.x);
+    const int pid = task_of(blk_off, pl.count, blockIdx.x);
     const int b = blockIdx.x - off_at(blk_off, pid);
     const ProbDesc P = pl.get(pid);
     const int nref = refl_count(P, side), L = refl_len(P, side);

@@ -420,6 +420,17 @@ static std::vector<int> prefix(int count, F f, int &total)
     return off;
 }
 
+// prefix() followed by the problem of every task (read by task_of)
+template <class F>
+static std::vector<int> prefix_table(int count, F f, int &total)
+{
+    std::vector<int> v = prefix(count, f, total);
+    v.resize((size_t)count + 1 + total);
+    for (int i = 0; i < count; ++i)
+        for (int q = v[i]; q < v[i + 1]; ++q) v[(size_t)count + 1 + q] = i;
+    return v;
+}
+
 static double svd_cost(int M, int N) { return 4.0 * M * M * N + 2.0 * M * N * N + 6.0 * (double)N * N * N; }
 
 static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::vector<size_t> &scr_off)
@@ -550,8 +561,9 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
             }
         }
     }
-    bp.o_bq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, 32); }, bp.nbq));
-    bp.o_bp = blob_put(b, prefix(count, [&](int i) { return cdiv(std::max(jobs[i].N - 1, 0), 32); }, bp.nbp));
+    // with the block -> problem table that tfactor_kernel reads
+    bp.o_bq = blob_put(b, prefix_table(count, [&](int i) { return cdiv(jobs[i].N, 32); }, bp.nbq));
+    bp.o_bp = blob_put(b, prefix_table(count, [&](int i) { return cdiv(std::max(jobs[i].N - 1, 0), 32); }, bp.nbp));
     bp.o_sq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, 64); }, bp.nsq));
     bp.o_sp = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, 64); }, bp.nsp));
     bp.o_tp = blob_put(b, prefix(count, [&](int i) {

@@ -259,6 +259,14 @@ __host__ __device__ inline int find_task(const int *offs, int ntask, int x)
 
 __host__ __device__ inline int off_at(const int *offs, int i) { return offs ? offs[i] : 0; }
 
+// Problem of task x for offsets that the batch plan writes as count + 1 prefix entries
+// followed by one entry per task (its problem): one load instead of a binary search.  A
+// single problem needs neither (offs may be null).
+__host__ __device__ inline int task_of(const int *offs, int count, int x)
+{
+    return (count <= 1 || !offs) ? 0 : offs[count + 1 + x];
+}
+
 // Leaf partition of N columns (svd_tiling.cpp:83-130): L = N/3 + 1 leaves for N >= 3,
 // leaf l starts at 3l; the last starts one column early when N % 3 == 0.  N <= 2 is
 // one leaf.  Leaf l covers [leaf_start(l), leaf_start(l+1)) with leaf_start(L) = N.
