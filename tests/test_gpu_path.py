@@ -366,7 +366,9 @@ def test_bidiag_one_wavefront(torch_dev):
 @pytest.mark.parametrize("threads", ["512", "1024"])
 def test_bidiag_tiers_bitwise_equal(torch_dev, threads):
     """Rows streamed from global memory or cached in LDS take the same arithmetic in the
-    same order: every split of a matrix between the two gives the same bits."""
+    same order: every split of a matrix between the two gives the same bits, including
+    the re-packing of the cached rows as the trailing matrix narrows (all of the LDS and
+    64 KiB re-pack the larger shapes here; 0 bytes caches no row)."""
     torch = torch_dev
     import pacosvd
     rng = np.random.default_rng(5)
