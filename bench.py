@@ -23,24 +23,14 @@ for _p in (os.path.join(ROOT, "paco-svd_amd"), os.path.join(ROOT, "oracle"), os.
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
+import workloads  # noqa: E402  (tests/workloads.py: the configurations' input generators)
+
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFS = 157.3
 
 
-def workload_shapes(count=10000, seed=3):
-    rng = np.random.default_rng(seed)
-    Ms = rng.integers(16, 513, size=count)
-    Ns = np.array([rng.integers(8, m + 1) for m in Ms])
-    return rng, Ms, Ns
-
-
 def make_workload(torch, count=10000, seed=3):
-    rng, Ms, Ns = workload_shapes(count, seed)
-    sizes = Ms * Ns
-    offs = np.concatenate([[0], np.cumsum(sizes)])
-    host = np.empty(int(offs[-1]), dtype=np.float32)
-    for i in range(count):
-        host[offs[i]:offs[i + 1]] = rng.standard_normal((int(Ms[i]), int(Ns[i]))).astype(np.float32).ravel()
+    host, offs, Ms, Ns = workloads.c4_host(count, seed)
     dev = torch.from_numpy(host).cuda()
     As = [dev[offs[i]:offs[i + 1]].view(int(Ms[i]), int(Ns[i])) for i in range(count)]
     return host, offs, Ms, Ns, As
@@ -74,10 +64,27 @@ def svd_flops(M, N):
     return 4.0 * M * M * N + 2.0 * M * N * N + 6.0 * N ** 3
 
 
+def _pool_dgesdd(A):
+    """Worker of the LAPACK process pool (one BLAS thread per process)."""
+    from scipy.linalg import lapack
+    lapack.dgesdd(A, full_matrices=1)
+    return 0
+
+
+def _pool_init():
+    from threadpoolctl import threadpool_limits
+    threadpool_limits(1)
+
+
 def cpu_baseline(host, offs, Ms, Ns, budget_s=12.0):
     """The oracle (FP64 C restatement of the reference path, one thread) on the first
-    matrices of the same workload, bounded by budget_s of CPU time."""
+    matrices of the same workload, bounded by budget_s of CPU time; LAPACK beside it on
+    the same sample: dgesdd with every host core and with one, sgesdd with every core, and
+    dgesdd in a process pool (one BLAS thread per process, one process per core)."""
+    import multiprocessing as mp
     import oracle
+    from scipy.linalg import lapack
+    from threadpoolctl import threadpool_limits
     oracle.lib()
     done = 0
     t0 = time.perf_counter()
@@ -86,16 +93,33 @@ def cpu_baseline(host, offs, Ms, Ns, budget_s=12.0):
         oracle.svd(A)
         done += 1
     dt = time.perf_counter() - t0
-    # LAPACK dgesdd (all cores) on the same sample, for reference
-    from scipy.linalg import lapack
-    t1 = time.perf_counter()
-    for i in range(done):
-        A = host[offs[i]:offs[i + 1]].reshape(int(Ms[i]), int(Ns[i])).astype(np.float64)
-        lapack.dgesdd(A, full_matrices=1)
-    dt2 = time.perf_counter() - t1
+    sample = [host[offs[i]:offs[i + 1]].reshape(int(Ms[i]), int(Ns[i])) for i in range(done)]
+    cores = min(len(os.sched_getaffinity(0)), 16)
+
+    def timed(fn):
+        t = time.perf_counter()
+        for A in sample:
+            fn(A)
+        return done / (time.perf_counter() - t)
+
+    with threadpool_limits(cores):
+        d_all = timed(lambda A: lapack.dgesdd(A.astype(np.float64), full_matrices=1))
+        s_all = timed(lambda A: lapack.sgesdd(A, full_matrices=1))
+    with threadpool_limits(1):
+        d_one = timed(lambda A: lapack.dgesdd(A.astype(np.float64), full_matrices=1))
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(cores, initializer=_pool_init) as pool:
+        pool.map(_pool_dgesdd, [sample[0].astype(np.float64)] * cores)  # start-up
+        t = time.perf_counter()
+        pool.map(_pool_dgesdd, [A.astype(np.float64) for A in sample], chunksize=4)
+        d_pool = done / (time.perf_counter() - t)
     return dict(value=done / dt, unit="SVDs/s", cores=1, kind="port",
-                sample="first %d matrices of the workload (oracle FP64 bidiag + D&C, 1 thread, %.1f s)" % (done, dt),
-                lapack_dgesdd_svds_per_s=done / dt2, lapack_threads=os.cpu_count())
+                sample="first %d matrices of the batched workload (oracle FP64 bidiag + D&C, 1 thread, %.1f s)"
+                       % (done, dt),
+                host_cores=cores, os_cpu_count=os.cpu_count(),
+                lapack_svds_per_s={"dgesdd_all_cores": round(d_all, 1), "dgesdd_1_thread": round(d_one, 1),
+                                   "sgesdd_all_cores": round(s_all, 1),
+                                   "dgesdd_process_pool": round(d_pool, 1)})
 
 
 def accuracy_sample(torch, host, offs, Ms, Ns, outs, n=40):
@@ -112,49 +136,21 @@ def accuracy_sample(torch, host, offs, Ms, Ns, outs, n=40):
     return worst
 
 
-def serving_requests(count, rate=2000.0, seed=5):
-    """Request stream of the serving configuration: Poisson arrivals at `rate` req/s,
-    M log-uniform in [32, 4096], aspect M/N uniform in [1, 4], 10% of requests with
-    clustered sigma (min(1000, N) values 1 + 1e-7 u), the rest log-uniform in [1e-3, 1].
-    Drawn in order per request from numpy default_rng(seed)."""
-    rng = np.random.default_rng(seed)
-    reqs = []
-    t = 0.0
-    for i in range(count):
-        t += rng.exponential(1.0 / rate)
-        M = int(round(np.exp(rng.uniform(np.log(32), np.log(4096)))))
-        a = rng.uniform(1.0, 4.0)
-        N = max(1, int(round(M / a)))
-        clustered = bool(rng.uniform() < 0.1)
-        sig = 10.0 ** rng.uniform(-3.0, 0.0, N)
-        if clustered:
-            k = min(1000, N)
-            sig[:k] = 1.0 + 1e-7 * rng.uniform(size=k)
-        reqs.append(dict(t=t, M=M, N=N, clustered=clustered, sigma=np.sort(sig)[::-1].copy()))
-    return reqs
-
-
-def haar_matrix(torch, M, N, sigma, seed):
-    """A = Q1 diag(sigma) Q2^T with Haar factors (QR of Gaussian matrices with the sign
-    of diag(R) folded in), generated on the GPU in FP32."""
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    G1 = torch.randn((M, N), generator=g, device="cuda")
-    G2 = torch.randn((N, N), generator=g, device="cuda")
-    Q1, R1 = torch.linalg.qr(G1)
-    Q2, R2 = torch.linalg.qr(G2)
-    Q1 = Q1 * torch.sign(torch.diagonal(R1))[None, :]
-    Q2 = Q2 * torch.sign(torch.diagonal(R2))[None, :]
-    s = torch.from_numpy(sigma.astype(np.float32)).cuda()
-    return ((Q1 * s[None, :]) @ Q2.T.contiguous()).contiguous()
-
-
 def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0):
-    from svdcheck import measures
-    reqs = serving_requests(count, rate)
+    """The serving configuration: Poisson arrivals through the serving loop (continuous
+    batching), then the same requests through the plain path (one matrix per launch
+    sequence, one after the other) for the served / plain ratio.  Accuracy of every
+    request against FP64 LAPACK (dgesdd sigma, computed on the host while the GPU runs;
+    residual and orthogonality in FP64 on the GPU)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from svdcheck import measures_gpu
+    reqs = workloads.c5_requests(count, rate)
     print("[bench] serving: generating %d requests" % count, flush=True)
-    As = [haar_matrix(torch, r["M"], r["N"], r["sigma"], 500000 + i) for i, r in enumerate(reqs)]
+    As = workloads.c5_matrices(torch, reqs)
     outs = pacosvd.alloc_outputs(As)
     torch.cuda.synchronize()
+    ex = ThreadPoolExecutor(2)
+    refs = [ex.submit(workloads.lapack_sigma, A.cpu().numpy()) for A in As]
     srv = pacosvd.Server(int(budget_gb * (1 << 30)), max_batch=512, batch_lanes=2, wide_lanes=2,
                          max_M=4096, max_N=4096)
     try:
@@ -173,23 +169,35 @@ def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0):
     finally:
         srv.close()
     torch.cuda.synchronize()
-    # accuracy on a sample of the requests vs FP64 LAPACK
-    worst = dict(sigma_err=0.0, resid_ratio=0.0, orth_u=0.0, orth_v=0.0)
-    sample = [i for i in range(count) if reqs[i]["M"] <= 1200][:: max(1, count // 40)][:12]
-    for i in sample:
-        m = measures(As[i].cpu().numpy(), *(t.cpu().numpy() for t in outs[i]))
-        for k in worst:
-            worst[k] = max(worst[k], m[k])
-    worst["matrices_checked"] = len(sample)
+    # the same requests through the plain path, back to back on one stream
+    plain_outs = pacosvd.alloc_outputs(As)
+    pacosvd.svd_plain(As[:2], outs=plain_outs[:2])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pacosvd.svd_plain(As, outs=plain_outs)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    del plain_outs
+    print("[bench] serving: plain path %.3f s for the same %d requests" % (t_plain, count), flush=True)
+    worst = {}
+    for i in range(count):
+        m = measures_gpu(torch, As[i], *outs[i], refs[i].result())
+        for k, v in m.items():
+            worst[k] = max(worst.get(k, 0.0), v)
+    ex.shutdown()
+    worst["matrices_checked"] = count
     flops = sum(svd_flops(r["M"], r["N"]) for r in reqs)
+    served = st.get("svds_per_s", 0.0)
     return dict(requests=count, offered_rate=rate, submit_span_s=round(t_sub, 3), total_s=round(t_all, 3),
-                achieved_svds_per_s=round(st.get("svds_per_s", 0.0), 1),
+                achieved_svds_per_s=round(served, 1),
                 p50_ms=round(st.get("p50_ms", 0.0), 2), p99_ms=round(st.get("p99_ms", 0.0), 2),
                 mean_ms=round(st.get("mean_ms", 0.0), 2), batches=int(st.get("batches", 0)),
                 mean_batch=round(st.get("mean_batch", 0.0), 1), max_lanes_busy=int(st.get("max_lanes_busy", 0)),
                 wide_requests=int(st.get("wide_requests", 0)), clustered=sum(r["clustered"] for r in reqs),
                 offered_tflops=round(flops * rate / count / 1e12, 2),
                 achieved_tflops=round(flops / max(t_all, 1e-9) / 1e12, 3),
+                plain_total_s=round(t_plain, 3), plain_svds_per_s=round(count / t_plain, 1),
+                served_vs_plain=round(served / (count / t_plain), 3),
                 accuracy_vs_fp64_lapack=worst)
 
 
@@ -205,50 +213,71 @@ def single_svd(torch, pacosvd, A):
     return d, u, vt, ph
 
 
-def plain_large_bench(torch, pacosvd, M=4096, N=4096, seed=1, check=True):
-    """Plain path on one FP32 M x N N(0,1) matrix (default_rng(seed)): whole SVD through
-    the two entry points; bidiagonalization in HBM GB/s, back-transform GEMMs in TFLOP/s."""
-    from svdcheck import measures
-    A_np = np.random.default_rng(seed).standard_normal((M, N)).astype(np.float32)
-    A = torch.from_numpy(A_np).cuda()
-    single_svd(torch, pacosvd, A)  # warmup
-    t0 = time.perf_counter()
-    d, u, vt, ph = single_svd(torch, pacosvd, A)
-    wall = time.perf_counter() - t0
-    ph.pop("copy_in", None)
+def large_phase_figures(M, N, ph):
+    """Roofline figures of a single large SVD's phases."""
     bb = bidiag_bytes(M, N)
+    qp_flops = 4.0 * M * M * N - 4.0 * M * N * N + 4.0 * N ** 3 / 3 + 4.0 * N ** 3 / 3
     bt_flops = 2.0 * M * N * N + 2.0 * N ** 3
-    out = dict(shape=[M, N], wall_ms=round(1000 * wall, 2), phase_ms={k: round(v, 3) for k, v in ph.items()},
-               bidiag_gbs=round(bb / (ph["bidiag"] / 1000) / 1e9, 1),
-               bidiag_hbm_frac=round(bb / (ph["bidiag"] / 1000) / 1e9 / HBM_PEAK_GBS, 4),
-               backtransform_tflops=round(bt_flops / (ph["backtransform"] / 1000) / 1e12, 2),
-               backtransform_mfma_frac=round(bt_flops / (ph["backtransform"] / 1000) / 1e12 / FP32_MFMA_PEAK_TFS, 4))
-    if check:
-        out["accuracy_vs_fp64_lapack"] = measures(A_np, d.cpu().numpy(), u.cpu().numpy(), vt.cpu().numpy())
+    t = {k: max(v, 1e-6) / 1000 for k, v in ph.items()}
+    return dict(bidiag_gbs=round(bb / t["bidiag"] / 1e9, 1),
+                bidiag_hbm_frac=round(bb / t["bidiag"] / 1e9 / HBM_PEAK_GBS, 4),
+                form_qp_tflops=round(qp_flops / t["form_qp"] / 1e12, 2),
+                form_qp_mfma_frac=round(qp_flops / t["form_qp"] / 1e12 / FP32_MFMA_PEAK_TFS, 4),
+                backtransform_tflops=round(bt_flops / t["backtransform"] / 1e12, 2),
+                backtransform_mfma_frac=round(bt_flops / t["backtransform"] / 1e12 / FP32_MFMA_PEAK_TFS, 4))
+
+
+def plain_large_bench(torch, pacosvd, M=4096, N=4096, seed=1):
+    """The 4096 x 4096 N(0, 1) configuration through the two entry points: wall time, phases
+    (hipEvents), the bidiagonalization in HBM GB/s, the GEMM phases in TFLOP/s, accuracy
+    against FP64 LAPACK."""
+    from concurrent.futures import ThreadPoolExecutor
+    from svdcheck import measures_gpu
+    A_np = workloads.c2_matrix(M, N, seed)
+    with ThreadPoolExecutor(1) as ex:
+        ref = ex.submit(workloads.lapack_sigma, A_np)
+        A = torch.from_numpy(A_np).cuda()
+        single_svd(torch, pacosvd, A)  # warmup
+        t0 = time.perf_counter()
+        d, u, vt, ph = single_svd(torch, pacosvd, A)
+        wall = time.perf_counter() - t0
+        ph.pop("copy_in", None)
+        out = dict(shape=[M, N], wall_ms=round(1000 * wall, 2), phase_ms={k: round(v, 3) for k, v in ph.items()})
+        out.update(large_phase_figures(M, N, ph))
+        out["accuracy_vs_fp64_lapack"] = measures_gpu(torch, A, d, u, vt, ref.result())
     return out
 
 
 def precision_bench(torch, pacosvd, M=8192, N=4096, seed=2):
-    """Graded sigma_i = 10^(-6 i / (N-1)), A = Q1 diag(sigma) Q2^T (Haar factors from the
-    GPU QR of Gaussian matrices, torch generator seed `seed`); FP32 matrix-core GEMMs vs
-    BF16x3 split, both against FP64 LAPACK dgesdd of the FP32 matrix."""
-    from svdcheck import measures, lapack_sigma
-    sig = 10.0 ** (-6.0 * np.arange(N) / (N - 1))
-    A = haar_matrix(torch, M, N, sig, seed)
-    A_np = A.cpu().numpy()
-    s_ref = lapack_sigma(A_np)
+    """Graded sigma_i = 10^(-6 i / (N-1)), A = Q1 diag(sigma) Q2^T with Haar factors
+    (tests/workloads.py c3_matrix); FP32 matrix-core GEMMs vs BF16x3 split, both against
+    FP64 LAPACK dgesdd of the FP32 matrix."""
+    from concurrent.futures import ThreadPoolExecutor
+    from svdcheck import measures_gpu
+    A_np, _ = workloads.c3_matrix(M, N, seed)
     res = {}
-    for mode, name in [(0, "fp32_mfma"), (1, "bf16x3_split")]:
-        old = pacosvd.set_precision(mode)
-        try:
-            d, u, vt, ph = single_svd(torch, pacosvd, A)
-        finally:
-            pacosvd.set_precision(old)
-        s = d.cpu().numpy()
-        m = measures(A_np, s, u.cpu().numpy(), vt.cpu().numpy(), s_ref=s_ref)
-        m["sigma_rel_err_per_value_max"] = float(np.max(np.abs(s - s_ref) / s_ref))
-        m["phase_ms"] = {k: round(v, 3) for k, v in ph.items()}
-        res[name] = m
+    with ThreadPoolExecutor(1) as ex:
+        ref = ex.submit(workloads.lapack_sigma, A_np)
+        A = torch.from_numpy(A_np).cuda()
+        runs = {}
+        for mode, name in [(0, "fp32_mfma"), (1, "bf16x3_split")]:
+            old = pacosvd.set_precision(mode)
+            try:
+                single_svd(torch, pacosvd, A)  # warmup
+                t0 = time.perf_counter()
+                d, u, vt, ph = single_svd(torch, pacosvd, A)
+                wall = time.perf_counter() - t0
+            finally:
+                pacosvd.set_precision(old)
+            ph.pop("copy_in", None)
+            runs[name] = (d, u, vt, ph, wall)
+        s_ref = ref.result()
+        for name, (d, u, vt, ph, wall) in runs.items():
+            m = measures_gpu(torch, A, d, u, vt, s_ref)
+            m["wall_ms"] = round(1000 * wall, 2)
+            m["phase_ms"] = {k: round(v, 3) for k, v in ph.items()}
+            m.update(large_phase_figures(M, N, ph))
+            res[name] = m
     return res
 
 
@@ -263,8 +292,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
-    ap.add_argument("--plain-large", action="store_true", help="also run the 4096^2 plain-path configuration")
-    ap.add_argument("--precision", action="store_true", help="also run the 8192x4096 precision configuration")
+    ap.add_argument("--no-plain-large", action="store_true", help="skip the 4096^2 plain-path configuration")
+    ap.add_argument("--no-precision", action="store_true", help="skip the 8192x4096 precision configuration")
     ap.add_argument("--profile-batched", action="store_true",
                     help="only run the batched step (warmup + steps), for counter collection")
     args = ap.parse_args()
@@ -340,11 +369,11 @@ def main():
         print("[bench] serving %s" % serving, flush=True)
 
     plain_large = None
-    if args.plain_large:
+    if not args.no_plain_large:
         plain_large = plain_large_bench(torch, pacosvd)
         print("[bench] plain 4096^2 %s" % plain_large, flush=True)
     precision = None
-    if args.precision:
+    if not args.no_precision:
         precision = precision_bench(torch, pacosvd)
         print("[bench] precision %s" % precision, flush=True)
 

@@ -5,15 +5,34 @@
  * global-memory buffer layouts (row-major FP32, M >= N; sizes as main.cpp:33-43):
  *   a[M*N] u[M*M] vt[N*N] d[N] e[N-1] tauq[N] taup[N-1] qt[2*N*N] wt[2*N*N] idx[16*N].
  * All pointers passed to the entry points are device pointers; every entry point is
- * stream-ordered and asynchronous (no host synchronisation inside), so a caller may
- * capture it in a hipGraph.  Return value: 0 on success, otherwise a hipError_t code
- * (launch failure) or one of the PACO_E* codes below (argument error).
+ * stream-ordered and asynchronous (no host synchronisation inside).  Return value: 0 on
+ * success, otherwise a hipError_t code (launch failure) or one of the PACO_E* codes below
+ * (argument error, checked before anything is launched).
+ *
+ * Scratch and reentrancy.  Scratch the library owns (the workspace of the two entry
+ * points when the caller passes NULL, the plain path's scratch, the batched path's plan)
+ * is kept per stream: calls on different streams never share it, calls on one stream are
+ * ordered by the stream.  Such a buffer is allocated on the first call on a stream and
+ * grown when a later call needs more; hipMalloc / hipFree synchronise the device, so a
+ * call that grows a buffer is neither asynchronous nor capturable.  The two entry points
+ * with a caller workspace allocate nothing and may be captured in a hipGraph;
+ * paco_svd_batched_graph records its own graph.
  *
  * Deviation from the reference: the reference passes a 40*32-byte user workspace
  * (main.cpp:46-48).  These entry points need the workspace sized by
  * paco_workspace_size(M, N) (block reflector factors, secular vectors, merge states).
- * Passing NULL makes the library use an internal buffer it owns (grown on demand,
- * never freed before exit); that convenience allocates on first use.
+ *
+ * Status.  Each problem has a LAPACK-style info word: 0 ok, -1 a NaN or infinite entry
+ * in the bidiagonal handed to the divide and conquer (the problem is then solved as the
+ * zero matrix), -2 a grid barrier of the chip-wide bidiagonalization timed out (results
+ * invalid), > 0 the number of secular roots that reached the iteration limit.  The entry
+ * points keep it in their workspace at paco_info_offset(M, N): entry point (a) resets
+ * it, (b) adds to it.  The whole-SVD paths report it through paco_last_info.
+ *
+ * Size limits: N <= PACO_MAX_N; the bidiagonalization keeps one column of M floats and a
+ * few rows in the LDS of one workgroup (M + 21 min(N, 384) + 1100 floats <= 40960, i.e.
+ * M up to about 31,000 for wide matrices, about 40,000 for narrow ones); larger problems
+ * return PACO_EINVAL.
  */
 #ifndef PACO_SVD_H
 #define PACO_SVD_H
@@ -27,6 +46,7 @@ extern "C" {
 
 #define PACO_EINVAL (-1)   /* bad argument (M < N, N < 1, null pointer, size over the limits) */
 #define PACO_ENOMEM (-2)   /* an internal buffer could not be allocated */
+#define PACO_EINJECTED (-3) /* serving loop: failure injected by PACO_SERVE_FAIL_EVERY (tests) */
 #define PACO_MAX_N 7680    /* D&C deflation stage keeps one merge in LDS (20 bytes per row) */
 
 /* Entry point (a): Golub-Kahan upper bidiagonalization, A = Q * B * P^T.
@@ -95,10 +115,20 @@ int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream);
 /* Device bytes the batched path needs for these shapes (scratch + plan). */
 size_t paco_batch_workspace_size(const int *M, const int *N, int count);
 
-/* Status word per matrix of the last batched call on this thread: LAPACK-style info
- * (0 ok, > 0 secular iterations that hit the limit).  Copies min(count, last) values
- * (synchronises the stream). Returns the count copied. */
+/* Info word per job (in call order) of the last paco_svd_plain / paco_svd_batched /
+ * paco_svd_batched_graph call on this thread (see Status above).  Copies min(count, that
+ * call's count) values (synchronises the stream); returns the count copied, 0 after
+ * paco_release. */
 int paco_last_info(int *info_host, int count, void *stream);
+
+/* Byte offset of the int info word inside a workspace of paco_workspace_size(M, N)
+ * bytes (written by the two entry points).  No counterpart in the reference. */
+size_t paco_info_offset(int M, int N);
+
+/* Iteration limit of the secular solver per root (default 200; a root that reaches it
+ * adds 1 to its problem's info).  Returns the previous limit; values < 1 only query.
+ * No counterpart in the reference (svd.cpp:1144-1156 iterates without a check). */
+int paco_set_secular_maxiter(int maxit);
 
 /* C = alpha * op(A) * op(B) + beta * C on one stream with the engine's GEMM tile
  * (precision mode of paco_set_precision); row-major, op = transpose when trans != 0.
@@ -130,13 +160,13 @@ void *paco_server_create(size_t hbm_budget_bytes, int max_batch, int batch_lanes
  * PACO_EINVAL when the request could never be admitted (bad shape, over the budget). */
 int paco_server_submit(void *server, const paco_svd_job *job, uint64_t tag);
 /* Copy up to max finished requests (tag, latency in ms from submit to observed
- * completion) and forget them; returns the count. */
+ * completion; -1 for a request whose enqueue failed) and forget them; returns the count. */
 int paco_server_poll(void *server, uint64_t *tags, double *latency_ms, int max);
-/* Block until every queued request has finished; returns the first error or 0. */
+/* Block until every queued request has finished (or failed); returns the first error or 0. */
 int paco_server_drain(void *server);
 /* Statistics since creation: [completed, p50 ms, p99 ms, mean ms, SVDs/s over first
  * submit .. last completion, batches, mean batch size, most lanes busy at once,
- * wide requests]; returns the count written (at most 9). */
+ * wide requests, failed requests]; returns the count written (at most 10). */
 int paco_server_stats(void *server, double *out, int max);
 /* Drain and release the server. */
 void paco_server_destroy(void *server);

@@ -882,6 +882,8 @@ hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds
     if (count <= 0) return hipSuccess;
     if (lds > bidiag_wg_lds_limit()) return hipErrorInvalidValue;
     if (!pl.ptr && pl.single.N > WG_MAXN) return hipErrorInvalidValue;
+    // the per-column vectors (cbuf[M], row vectors) must fit the LDS the launch gets
+    if (!pl.ptr && lds < bidiag_wg_lds_bytes(pl.single.M, pl.single.N, nt)) return hipErrorInvalidValue;
     const unsigned lb = (unsigned)lds;
     if (nt == 1024)
         hipLaunchKernelGGL(bidiag_wg_kernel<1024>, dim3(count), dim3(1024), lds, s, pl, order, lb);

@@ -454,6 +454,21 @@ static void launch_trail(const ProbDesc &P, const GridWs &W, int q, int p, hipSt
     hipLaunchKernelGGL(gb_trail_kernel<MODE>, dim3(tiles), dim3(GT), 0, s, t, tn);
 }
 
+// first column of the one-workgroup finish: panels of NB columns while more than SWITCH
+// columns remain
+static int grid_tail_start(int N)
+{
+    int p = 0;
+    while (N - p > SWITCH) p += NB;
+    return p;
+}
+
+bool bidiag_grid_supported(int M, int N)
+{
+    const int p = grid_tail_start(N);
+    return bidiag_wg_lds_bytes(M - p, N - p, 1024) <= bidiag_wg_lds_limit();
+}
+
 hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s)
 {
     const int M = P.M, N = P.N;
@@ -511,6 +526,7 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s)
     pl.single.e = P.e + p;
     pl.single.tauq = P.tauq + p;
     pl.single.taup = P.taup + p;
+    if (bidiag_wg_lds_bytes(M - p, N - p, 1024) > bidiag_wg_lds_limit()) return hipErrorInvalidValue;
     return launch_bidiag_wg(pl, nullptr, 1, bidiag_wg_lds_limit(), s, 1024);
 }
 

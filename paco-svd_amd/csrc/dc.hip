@@ -473,7 +473,7 @@ __global__ __launch_bounds__(DT) void dc_secular_kernel(ProbList pl, LevelArgs l
     const double dK = d[K];
     double mu = 0.5 * (lo + hi);
     int it;
-    for (it = 1; it <= 200; ++it) {
+    for (it = 1; it <= la.maxit; ++it) {
         double psi = 0, dpsi = 0, phi = 0, dphi = 0, erretm = 0;
         for (int j = lane; j < k; j += G) {
             const double t = (d[j] - dK) * (d[j] + dK) - mu;
@@ -529,13 +529,14 @@ __global__ __launch_bounds__(DT) void dc_secular_kernel(ProbList pl, LevelArgs l
     if (lane == 0) {
         Ar.Kx[i] = K;
         Ar.mu[i] = mu;
-        if (it > 200) atomicAdd(P.info, 1);
+        if (it > la.maxit) atomicAdd(P.info, 1);
     }
 }
 
 // Same iteration, one thread per root with serial pole loops: used for the small merges
 // of the low levels (k <= 96), where a wavefront per root would leave most lanes idle.
-__device__ inline int secular_serial(int k, const double *d, const double *z, int i, int &Kout, double &muout)
+__device__ inline int secular_serial(int k, const double *d, const double *z, int i, int maxit, int &Kout,
+                                     double &muout)
 {
     const double eps = DBL_EPSILON;
     if (k == 1) {
@@ -568,7 +569,7 @@ __device__ inline int secular_serial(int k, const double *d, const double *z, in
     const double dK = d[K];
     double mu = 0.5 * (lo + hi);
     int it;
-    for (it = 1; it <= 200; ++it) {
+    for (it = 1; it <= maxit; ++it) {
         double psi = 0, dpsi = 0, phi = 0, dphi = 0, erretm = 0;
         for (int j = 0; j < k; ++j) {
             const double t = (d[j] - dK) * (d[j] + dK) - mu;
@@ -634,10 +635,10 @@ __global__ __launch_bounds__(DT) void dc_secular_thread_kernel(ProbList pl, Leve
     const Arena Ar = make_arena(P.idx, T.c0, S.n);
     int K;
     double mu;
-    const int it = secular_serial(k, Ar.dk, Ar.zk, i, K, mu);
+    const int it = secular_serial(k, Ar.dk, Ar.zk, i, la.maxit, K, mu);
     Ar.Kx[i] = K;
     Ar.mu[i] = mu;
-    if (it > 200) atomicAdd(P.info, 1);
+    if (it > la.maxit) atomicAdd(P.info, 1);
 }
 
 // ---------------------------------------------------------------------------

@@ -32,6 +32,33 @@ __global__ void zero_ints_kernel(int *p, int n)
     if (i < n) p[i] = 0;
 }
 
+__global__ void copy_int_kernel(int *dst, const int *src)
+{
+    if (threadIdx.x == 0) *dst = *src;
+}
+
+// Input guard of the divide and conquer, one workgroup per problem: a bidiagonal with a
+// NaN or infinite entry (non-finite input, or overflow in the bidiagonalization) sets
+// info = -1 and is replaced by zeros, so the deflation's sorting and compaction only
+// ever see finite poles.  The reference checks nothing (svd.cpp:134-195).
+__global__ __launch_bounds__(256) void dc_guard_kernel(ProbList pl)
+{
+    const ProbDesc P = pl.get(blockIdx.x);
+    const int N = P.N;
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    int b = 0;
+    for (int i = threadIdx.x; i < N; i += 256) b |= !isfinite(P.d[i]);
+    for (int i = threadIdx.x; i < N - 1; i += 256) b |= !isfinite(P.e[i]);
+    if (b) bad = 1;
+    __syncthreads();
+    if (!bad) return;
+    for (int i = threadIdx.x; i < N; i += 256) P.d[i] = 0.f;
+    for (int i = threadIdx.x; i < N - 1; i += 256) P.e[i] = 0.f;
+    if (threadIdx.x == 0) *P.info = -1;
+}
+
 // a <- A (row-major M x N) for every problem: rows [ROW_GROUP g, ROW_GROUP (g + 1)) of
 // every input into its working copy
 __global__ __launch_bounds__(256) void copy_in_kernel(ProbList pl, const float *const *src, const int *row_off)
@@ -82,21 +109,43 @@ static WsLayout ws_layout(int M, int N)
 int tuned_bidiag_threads(int M, int N);
 
 static std::mutex g_mu;
-static void *g_ws = nullptr;
-static size_t g_ws_size = 0;
+// bumped by paco_release: per-thread pointers into freed pools compare against it
+static unsigned g_release_gen = 0;
 
-static void *internal_ws(size_t bytes)
-{
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (bytes > g_ws_size) {
-        if (g_ws) (void)hipFree(g_ws);
-        g_ws = nullptr;
-        g_ws_size = 0;
-        if (hipMalloc(&g_ws, bytes) != hipSuccess) return nullptr;
-        g_ws_size = bytes;
+// Device buffers the library owns on behalf of one stream: callers on different streams
+// never share scratch, and calls on one stream are ordered by the stream itself.  A buffer
+// grows when a call needs more (hipFree / hipMalloc, which synchronise the device);
+// calls that fit the current sizes allocate nothing.
+struct BatchRes;
+static void batch_res_destroy(BatchRes &r);
+struct DevBuf {
+    void *p = nullptr;
+    size_t size = 0;
+    void *get(size_t bytes)
+    {
+        if (bytes > size) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            size = 0;
+            if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+            size = bytes;
+        }
+        return p;
     }
-    return g_ws;
-}
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        size = 0;
+    }
+};
+struct StreamRes;
+static StreamRes *stream_res(hipStream_t s);
+
+static void *internal_ws(hipStream_t s, size_t bytes);
+
+// iteration limit of the secular solver (paco_set_secular_maxiter)
+static int g_secular_maxit = 200;
 
 static bool use_grid_bidiag(int M, int N)
 {
@@ -170,6 +219,15 @@ static BidiagChoice bidiag_choice(int M, int N)
     return c;
 }
 
+// Whether the kernels hold an M x N bidiagonalization: the one-workgroup kernel keeps
+// cbuf[M] and its row vectors in LDS (bidiag_wg_lds_bytes), also for the last columns of
+// the chip-wide path.  Checked before anything is launched.
+static bool bidiag_fits(int M, int N)
+{
+    if (use_grid_bidiag(M, N)) return bidiag_grid_supported(M, N);
+    return bidiag_choice(M, N).lds <= bidiag_wg_lds_limit();
+}
+
 // Q and P^T from the reflectors (GetUVt, upper_bidiagonalization.cpp:774-893), blocked WY.
 static hipError_t form_q_pt(ProbList pl, const int *idq, int nidq, const int *idp, int nidp, const int *bq, int nbq,
                             const int *bp, int nbp, const int *sq, int nsq, const int *sp, int nsp, const int *tpair,
@@ -189,13 +247,15 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
                          void *workspace, hipStream_t s)
 {
     const WsLayout L = ws_layout(M, N);
-    char *ws = static_cast<char *>(workspace ? workspace : internal_ws(L.total));
+    char *ws = static_cast<char *>(workspace ? workspace : internal_ws(s, L.total));
     if (!ws) return PACO_ENOMEM;
     ProbList pl;
     pl.ptr = nullptr;
     pl.count = 1;
     pl.single = bidiag_desc(M, N, a, u, vt, d, e, tauq, taup, ws, L);
     hipError_t err;
+    // entry point (a) starts the problem's info word; (b) adds to it
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, pl.single.info, 1);
     const bool tm = g_timing;
     if (tm) {
         g_tev_recorded = true;
@@ -258,6 +318,7 @@ static hipError_t run_dc_levels(ProbList pl, const DcOffsets &o, const uint32_t 
         // launches give each merge that many threads / tiles
         la.maxn = o.maxn[r - 1];
         la.stride = std::max(1, std::min(3 << r, la.maxn));
+        la.maxit = g_secular_maxit;
         if ((e = launch_dc_level(pl, la, s)) != hipSuccess) return e;
     }
     return hipSuccess;
@@ -279,7 +340,7 @@ static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, floa
                      uint32_t *idx, void *workspace, const void *tiling_dev, hipStream_t s)
 {
     const WsLayout L = ws_layout(M, N);
-    char *ws = static_cast<char *>(workspace ? workspace : internal_ws(L.total));
+    char *ws = static_cast<char *>(workspace ? workspace : internal_ws(s, L.total));
     if (!ws) return PACO_ENOMEM;
     ProbList pl;
     pl.ptr = nullptr;
@@ -289,7 +350,7 @@ static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, floa
     P.qt = qt;
     P.wt = wt;
     P.idx = idx;
-    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, P.info, 1);
+    hipLaunchKernelGGL(dc_guard_kernel, dim3(1), dim3(256), 0, s, pl);
     DcOffsets o;
     o.nleaf = leaf_count(N);
     MergeState *st = reinterpret_cast<MergeState *>(ws + L.states);
@@ -616,6 +677,18 @@ struct BatchRes {
     hipEvent_t evf = nullptr, evj = nullptr;
 };
 
+// Frees everything a BatchRes holds (device and pinned memory, the forked stream, events).
+static void batch_res_destroy(BatchRes &r)
+{
+    if (r.dev) (void)hipFree(r.dev);
+    if (r.pinned) (void)hipHostFree(r.pinned);
+    if (r.ev) (void)hipEventDestroy(r.ev);
+    if (r.evf) (void)hipEventDestroy(r.evf);
+    if (r.evj) (void)hipEventDestroy(r.evj);
+    if (r.side) (void)hipStreamDestroy(r.side);
+    r = BatchRes();
+}
+
 static hipError_t ensure(BatchRes &r, size_t dev_bytes, size_t pin_bytes)
 {
     if (dev_bytes > r.dev_size) {
@@ -645,12 +718,15 @@ static hipError_t ensure(BatchRes &r, size_t dev_bytes, size_t pin_bytes)
 // Prepared batch: device layout fixed, descriptors patched, blob staged in pinned memory.
 struct Prepared {
     size_t o_states = 0;
-    int *info = nullptr;
+    int *info = nullptr;   // per-job info inside the plan buffer (zeroed by the launch sequence)
+    bool own_info = true;  // false: the jobs' info words live in a caller array (already zeroed)
 };
 
 // Lay out device memory [blob | states | scratch...], patch descriptors and stage the
-// blob in pinned memory.  Allocates (outside any stream capture).
-static hipError_t prepare_batch(const paco_svd_job *jobs, int count, BatchPlan &bp, BatchRes &res, Prepared &pr)
+// blob in pinned memory.  Allocates (outside any stream capture).  With info_ext, job i
+// reports into info_ext[info_idx[i]] instead of the plan's own array.
+static hipError_t prepare_batch(const paco_svd_job *jobs, int count, BatchPlan &bp, BatchRes &res, Prepared &pr,
+                                int *info_ext = nullptr, const int *info_idx = nullptr)
 {
     std::vector<size_t> scr_off;
     build_plan(jobs, count, bp, scr_off);
@@ -687,9 +763,10 @@ static hipError_t prepare_batch(const paco_svd_job *jobs, int count, BatchPlan &
         P.vmt = reinterpret_cast<float *>(base + L.vmt);
         P.tq = reinterpret_cast<float *>(base + L.tq);
         P.tp = reinterpret_cast<float *>(base + L.tp);
-        P.info = pr.info + i;
+        P.info = info_ext ? info_ext + info_idx[i] : pr.info + i;
         reinterpret_cast<float **>(bp.blob.data() + bp.o_qp)[i] = reinterpret_cast<float *>(base + L.qp);
     }
+    if (info_ext) pr.own_info = false;
     memcpy(res.pinned, bp.blob.data(), bp.blob.size());
     return hipSuccess;
 }
@@ -710,7 +787,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     pl.count = count;
     memset(&pl.single, 0, sizeof(pl.single));
     tmark(s, 0, tm);
-    hipLaunchKernelGGL(zero_ints_kernel, dim3(cdiv(count, 256)), dim3(256), 0, s, pr.info, count);
+    if (pr.own_info) hipLaunchKernelGGL(zero_ints_kernel, dim3(cdiv(count, 256)), dim3(256), 0, s, pr.info, count);
     hipLaunchKernelGGL(copy_in_kernel, dim3(bp.nrowin), dim3(256), 0, s, pl,
                        reinterpret_cast<const float *const *>(dev + bp.o_src), I(bp.o_rowin));
     tmark(s, 1, tm);
@@ -790,6 +867,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     o.nru = bp.nru;
     o.bt_rv = I(bp.o_rv);
     o.nrv = bp.nrv;
+    hipLaunchKernelGGL(dc_guard_kernel, dim3(count), dim3(256), 0, s, pl);
     if ((e = run_dc_levels(pl, o, nullptr, s)) != hipSuccess) return e;
     tmark(s, 4, tm);
     e = run_dc_backtransform(pl, o, s);
@@ -797,10 +875,39 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     return e;
 }
 
+// info words of the last batched or plain call on this thread (valid while t_last_gen ==
+// g_release_gen: paco_release and graph eviction free the arrays)
 static thread_local int *t_last_info = nullptr;
 static thread_local int t_last_count = 0;
+static thread_local unsigned t_last_gen = 0;
 
-static BatchRes g_batch_res;
+static void set_last_info(int *info, int count)
+{
+    t_last_info = info;
+    t_last_count = count;
+    t_last_gen = g_release_gen;
+}
+
+struct StreamRes {
+    DevBuf ws;      // workspace of the entry points when the caller passes none
+    DevBuf plain;   // scratch of paco_svd_plain and of the wide jobs of a batched call
+    DevBuf info;    // per-job info of a plain call or of a batched call with wide jobs
+    BatchRes batch; // plan, scratch and forked stream of the batched path
+};
+static std::unordered_map<hipStream_t, StreamRes *> g_streams; // guarded by g_mu
+
+static StreamRes *stream_res(hipStream_t s)
+{
+    StreamRes *&r = g_streams[s];
+    if (!r) r = new StreamRes();
+    return r;
+}
+
+static void *internal_ws(hipStream_t s, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return stream_res(s)->ws.get(bytes);
+}
 
 struct GraphEntry {
     BatchRes res;
@@ -826,12 +933,13 @@ static uint64_t batch_signature(const paco_svd_job *jobs, int count)
         mix((uint64_t)(uintptr_t)jobs[i].Vt);
     }
     mix((uint64_t)gemm_get_mode());
+    mix((uint64_t)g_secular_maxit);
     return h;
 }
 
 static bool job_ok(const paco_svd_job &j)
 {
-    return j.M >= j.N && j.N >= 1 && j.A && j.S && j.U && j.Vt && j.N <= PACO_MAX_N;
+    return j.M >= j.N && j.N >= 1 && j.A && j.S && j.U && j.Vt && j.N <= PACO_MAX_N && bidiag_fits(j.M, j.N);
 }
 
 // ---------------------------------------------------------------------------
@@ -868,12 +976,7 @@ Lane *lane_create(size_t capacity, int max_jobs)
 void lane_destroy(Lane *l)
 {
     if (!l) return;
-    if (l->res.dev) (void)hipFree(l->res.dev);
-    if (l->res.pinned) (void)hipHostFree(l->res.pinned);
-    if (l->res.ev) (void)hipEventDestroy(l->res.ev);
-    if (l->res.evf) (void)hipEventDestroy(l->res.evf);
-    if (l->res.evj) (void)hipEventDestroy(l->res.evj);
-    if (l->res.side) (void)hipStreamDestroy(l->res.side);
+    batch_res_destroy(l->res);
     delete l;
 }
 
@@ -890,6 +993,8 @@ int lane_enqueue(Lane *l, const paco_svd_job *jobs, int count, hipStream_t s)
 }
 
 size_t wide_scratch_bytes(int M, int N) { return scratch_layout(M, N).total + ws_layout(M, N).total; }
+
+bool job_supported(int M, int N) { return M >= N && N >= 1 && N <= PACO_MAX_N && bidiag_fits(M, N); }
 
 int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s)
 {
@@ -908,6 +1013,51 @@ int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s)
                      reinterpret_cast<uint32_t *>(base + L.idx), ws, nullptr, s);
 }
 
+// Runs jobs through the two entry points, one after the other on stream s (the plain
+// path); job i reports its info word into info[idx ? idx[i] : i].
+static int run_plain(const paco_svd_job *jobs, int count, hipStream_t s, int *info, const int *idx)
+{
+    size_t need = 0;
+    for (int i = 0; i < count; ++i)
+        need = std::max(need, scratch_layout(jobs[i].M, jobs[i].N).total + ws_layout(jobs[i].M, jobs[i].N).total);
+    char *buf;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        buf = static_cast<char *>(stream_res(s)->plain.get(need));
+    }
+    if (!buf) return PACO_ENOMEM;
+    for (int i = 0; i < count; ++i) {
+        const int M = jobs[i].M, N = jobs[i].N;
+        const BatchScratch L = scratch_layout(M, N);
+        char *ws = buf + L.total;
+        float *a = reinterpret_cast<float *>(buf + L.a);
+        hipError_t e = hipMemcpyAsync(a, jobs[i].A, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return (int)e;
+        int rc = bidiag_single(M, N, a, jobs[i].U, jobs[i].Vt, jobs[i].S, reinterpret_cast<float *>(buf + L.e),
+                               reinterpret_cast<float *>(buf + L.tauq), reinterpret_cast<float *>(buf + L.taup), ws, s);
+        if (rc) return rc;
+        rc = dc_single(M, N, a, jobs[i].U, jobs[i].Vt, jobs[i].S, reinterpret_cast<float *>(buf + L.e),
+                       reinterpret_cast<float *>(buf + L.qt), reinterpret_cast<float *>(buf + L.wt),
+                       reinterpret_cast<uint32_t *>(buf + L.idx), ws, nullptr, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(copy_int_kernel, dim3(1), dim3(64), 0, s, info + (idx ? idx[i] : i),
+                           reinterpret_cast<const int *>(ws + ws_layout(M, N).info));
+    }
+    return (int)hipGetLastError();
+}
+
+// A zeroed per-job info array of the stream (count entries).
+static int *stream_info(hipStream_t s, int count)
+{
+    int *info;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        info = static_cast<int *>(stream_res(s)->info.get(sizeof(int) * (size_t)std::max(count, 1)));
+    }
+    if (info) hipLaunchKernelGGL(zero_ints_kernel, dim3(cdiv(count, 256)), dim3(256), 0, s, info, count);
+    return info;
+}
+
 } // namespace paco
 
 using namespace paco;
@@ -919,6 +1069,7 @@ int paco_upper_bidiagonalization(unsigned blockDim, void *stream, int M, int N, 
 {
     (void)blockDim;
     if (M < N || N < 1 || !a || !u || !vt || !d || !tauq || (N > 1 && (!e || !taup))) return PACO_EINVAL;
+    if (!bidiag_fits(M, N)) return PACO_EINVAL;
     return bidiag_single(M, N, a, u, vt, d, e, tauq, taup, workspace, static_cast<hipStream_t>(stream));
 }
 
@@ -948,81 +1099,70 @@ int paco_svd_plain(const paco_svd_job *jobs, int count, void *stream)
 {
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (count <= 0) return 0;
-    size_t need = 0;
-    for (int i = 0; i < count; ++i) {
+    for (int i = 0; i < count; ++i)
         if (!job_ok(jobs[i])) return PACO_EINVAL;
-        need = std::max(need, scratch_layout(jobs[i].M, jobs[i].N).total + ws_layout(jobs[i].M, jobs[i].N).total);
-    }
-    static void *buf = nullptr;
-    static size_t bufsz = 0;
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        if (need > bufsz) {
-            if (buf) (void)hipFree(buf);
-            buf = nullptr;
-            bufsz = 0;
-            if (hipMalloc(&buf, need) != hipSuccess) return PACO_ENOMEM;
-            bufsz = need;
-        }
-    }
-    for (int i = 0; i < count; ++i) {
-        const int M = jobs[i].M, N = jobs[i].N;
-        const BatchScratch L = scratch_layout(M, N);
-        char *base = static_cast<char *>(buf);
-        char *ws = base + L.total;
-        float *a = reinterpret_cast<float *>(base + L.a);
-        hipError_t e = hipMemcpyAsync(a, jobs[i].A, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) return (int)e;
-        int rc = paco_upper_bidiagonalization(40, s, M, N, a, jobs[i].U, jobs[i].Vt, jobs[i].S,
-                                              reinterpret_cast<float *>(base + L.e),
-                                              reinterpret_cast<float *>(base + L.tauq),
-                                              reinterpret_cast<float *>(base + L.taup), ws);
-        if (rc) return rc;
-        rc = paco_svd_DC(1, s, M, N, a, jobs[i].U, jobs[i].Vt, jobs[i].S, reinterpret_cast<float *>(base + L.e),
-                         reinterpret_cast<float *>(base + L.qt), reinterpret_cast<float *>(base + L.wt),
-                         reinterpret_cast<uint32_t *>(base + L.idx), ws, nullptr);
-        if (rc) return rc;
-    }
+    int *info = stream_info(s, count);
+    if (!info) return PACO_ENOMEM;
+    const int rc = run_plain(jobs, count, s, info, nullptr);
+    if (rc) return rc;
+    set_last_info(info, count);
     return 0;
 }
 
-// Jobs wider than the one-workgroup bidiagonalization go through the chip-wide path.
-static bool split_wide(const paco_svd_job *jobs, int count, std::vector<paco_svd_job> &narrow,
-                       std::vector<paco_svd_job> &wide)
+// Jobs wider than the one-workgroup bidiagonalization go through the chip-wide path;
+// narrow / wide hold the jobs' indices in the call.
+static bool split_wide(const paco_svd_job *jobs, int count, std::vector<int> &narrow, std::vector<int> &wide)
 {
-    for (int i = 0; i < count; ++i) (jobs[i].N > BIDIAG_WG_MAXN ? wide : narrow).push_back(jobs[i]);
+    for (int i = 0; i < count; ++i) (jobs[i].N > BIDIAG_WG_MAXN ? wide : narrow).push_back(i);
     return !wide.empty();
 }
 
-static int batched_narrow(const paco_svd_job *jobs, int count, void *stream);
+// The batched launch sequence of jobs on stream s.  With info_ext, job i reports into
+// info_ext[info_idx[i]]; otherwise into the plan's own array, which becomes the thread's
+// last info.
+static int batched_narrow(const paco_svd_job *jobs, int count, hipStream_t s, int *info_ext, const int *info_idx)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    BatchRes &res = stream_res(s)->batch;
+    BatchPlan bp;
+    Prepared pr;
+    hipError_t e = prepare_batch(jobs, count, bp, res, pr, info_ext, info_idx);
+    if (e == hipErrorInvalidValue && !bp.fits) return PACO_EINVAL;
+    if (e != hipSuccess) return (int)e;
+    e = enqueue_batch(bp, res, pr, s, true);
+    if (e != hipSuccess) return (int)e;
+    if (!info_ext) set_last_info(pr.info, count);
+    return 0;
+}
 
 int paco_svd_batched(const paco_svd_job *jobs, int count, void *stream)
 {
+    hipStream_t s = static_cast<hipStream_t>(stream);
     if (count <= 0) return 0;
     for (int i = 0; i < count; ++i)
         if (!job_ok(jobs[i])) return PACO_EINVAL;
-    std::vector<paco_svd_job> narrow, wide;
-    if (split_wide(jobs, count, narrow, wide)) {
-        int rc = narrow.empty() ? 0 : batched_narrow(narrow.data(), (int)narrow.size(), stream);
-        if (rc) return rc;
-        return paco_svd_plain(wide.data(), (int)wide.size(), stream);
-    }
-    return batched_narrow(jobs, count, stream);
+    std::vector<int> ni, wi;
+    if (!split_wide(jobs, count, ni, wi)) return batched_narrow(jobs, count, s, nullptr, nullptr);
+    // mixed: one info array in call order; the narrow jobs as one batch, the wide ones
+    // through the entry points
+    int *info = stream_info(s, count);
+    if (!info) return PACO_ENOMEM;
+    std::vector<paco_svd_job> nj, wj;
+    for (int i : ni) nj.push_back(jobs[i]);
+    for (int i : wi) wj.push_back(jobs[i]);
+    int rc = nj.empty() ? 0 : batched_narrow(nj.data(), (int)nj.size(), s, info, ni.data());
+    if (rc) return rc;
+    rc = run_plain(wj.data(), (int)wj.size(), s, info, wi.data());
+    if (rc) return rc;
+    set_last_info(info, count);
+    return 0;
 }
 
-static int batched_narrow(const paco_svd_job *jobs, int count, void *stream)
+static void graph_entry_destroy(GraphEntry *g)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    BatchPlan bp;
-    Prepared pr;
-    hipError_t e = prepare_batch(jobs, count, bp, g_batch_res, pr);
-    if (e == hipErrorInvalidValue && !bp.fits) return PACO_EINVAL;
-    if (e != hipSuccess) return (int)e;
-    e = enqueue_batch(bp, g_batch_res, pr, static_cast<hipStream_t>(stream), true);
-    if (e != hipSuccess) return (int)e;
-    t_last_info = pr.info;
-    t_last_count = count;
-    return 0;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    batch_res_destroy(g->res);
+    delete g;
 }
 
 int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
@@ -1031,8 +1171,8 @@ int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
     for (int i = 0; i < count; ++i)
         if (!job_ok(jobs[i])) return PACO_EINVAL;
     {
-        std::vector<paco_svd_job> narrow, wide;
-        if (split_wide(jobs, count, narrow, wide)) return paco_svd_batched(jobs, count, stream);
+        std::vector<int> ni, wi;
+        if (split_wide(jobs, count, ni, wi)) return paco_svd_batched(jobs, count, stream);
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(g_mu);
@@ -1041,21 +1181,17 @@ int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
     GraphEntry *g = nullptr;
     if (it == g_graphs.end()) {
         if (g_graphs.size() >= 8) {
-            for (auto &kv : g_graphs) {
-                (void)hipStreamSynchronize(s);
-                if (kv.second->exec) (void)hipGraphExecDestroy(kv.second->exec);
-                if (kv.second->res.dev) (void)hipFree(kv.second->res.dev);
-                if (kv.second->res.pinned) (void)hipHostFree(kv.second->res.pinned);
-                if (kv.second->res.ev) (void)hipEventDestroy(kv.second->res.ev);
-                delete kv.second;
-            }
+            // the evicted graphs may still run on any stream
+            (void)hipDeviceSynchronize();
+            for (auto &kv : g_graphs) graph_entry_destroy(kv.second);
             g_graphs.clear();
+            ++g_release_gen;
         }
         g = new GraphEntry();
         Prepared pr;
         hipError_t e = prepare_batch(jobs, count, g->plan, g->res, pr);
         if (e != hipSuccess) {
-            delete g;
+            graph_entry_destroy(g);
             return (e == hipErrorInvalidValue) ? PACO_EINVAL : (int)e;
         }
         g->info = pr.info;
@@ -1063,25 +1199,26 @@ int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
         // cannot be captured); the graph is launched on the caller's stream below
         static hipStream_t cap = nullptr;
         if (!cap && (e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)) != hipSuccess) {
-            delete g;
+            graph_entry_destroy(g);
             return (int)e;
         }
         hipGraph_t graph;
         e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) {
-            delete g;
+            graph_entry_destroy(g);
             return (int)e;
         }
         e = enqueue_batch(g->plan, g->res, pr, cap, false);
         hipError_t e2 = hipStreamEndCapture(cap, &graph);
         if (e != hipSuccess || e2 != hipSuccess) {
-            delete g;
+            if (e2 == hipSuccess) (void)hipGraphDestroy(graph);
+            graph_entry_destroy(g);
             return (int)(e != hipSuccess ? e : e2);
         }
         e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) {
-            delete g;
+            graph_entry_destroy(g);
             return (int)e;
         }
         g_graphs[sig] = g;
@@ -1090,8 +1227,7 @@ int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
     }
     hipError_t e = hipGraphLaunch(g->exec, s);
     if (e != hipSuccess) return (int)e;
-    t_last_info = g->info;
-    t_last_count = count;
+    set_last_info(g->info, count);
     return 0;
 }
 
@@ -1104,7 +1240,7 @@ size_t paco_batch_workspace_size(const int *M, const int *N, int count)
 
 int paco_last_info(int *info_host, int count, void *stream)
 {
-    if (!t_last_info || !info_host) return 0;
+    if (!t_last_info || !info_host || t_last_gen != g_release_gen) return 0;
     const int n = std::min(count, t_last_count);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (hipMemcpyAsync(info_host, t_last_info, sizeof(int) * n, hipMemcpyDeviceToHost, s) != hipSuccess) return 0;
@@ -1182,21 +1318,31 @@ void paco_release(void)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     (void)hipDeviceSynchronize();
-    for (auto &kv : g_graphs) {
-        if (kv.second->exec) (void)hipGraphExecDestroy(kv.second->exec);
-        if (kv.second->res.dev) (void)hipFree(kv.second->res.dev);
-        if (kv.second->res.pinned) (void)hipHostFree(kv.second->res.pinned);
-        if (kv.second->res.ev) (void)hipEventDestroy(kv.second->res.ev);
+    for (auto &kv : g_graphs) graph_entry_destroy(kv.second);
+    g_graphs.clear();
+    for (auto &kv : g_streams) {
+        batch_res_destroy(kv.second->batch);
+        kv.second->ws.release();
+        kv.second->plain.release();
+        kv.second->info.release();
         delete kv.second;
     }
-    g_graphs.clear();
-    if (g_batch_res.dev) (void)hipFree(g_batch_res.dev);
-    if (g_batch_res.pinned) (void)hipHostFree(g_batch_res.pinned);
-    if (g_batch_res.ev) (void)hipEventDestroy(g_batch_res.ev);
-    g_batch_res = BatchRes();
-    if (g_ws) (void)hipFree(g_ws);
-    g_ws = nullptr;
-    g_ws_size = 0;
+    g_streams.clear();
+    ++g_release_gen;
+}
+
+int paco_set_secular_maxiter(int maxit)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    const int old = g_secular_maxit;
+    if (maxit >= 1) g_secular_maxit = maxit;
+    return old;
+}
+
+size_t paco_info_offset(int M, int N)
+{
+    if (M < N || N < 1) return 0;
+    return ws_layout(M, N).info;
 }
 
 } // extern "C"

@@ -21,6 +21,8 @@ void lane_destroy(Lane *l);
 // Enqueue the batched pipeline of the jobs on stream s (stream-ordered, no host sync
 // beyond the lane's previous plan upload).  Returns 0 or an error code.
 int lane_enqueue(Lane *l, const paco_svd_job *jobs, int count, hipStream_t s);
+// Whether the engine takes an M x N job (sizes within the kernels' limits).
+bool job_supported(int M, int N);
 // Scratch for one wide job through the two entry points.
 size_t wide_scratch_bytes(int M, int N);
 int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s);

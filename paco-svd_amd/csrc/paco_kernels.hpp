@@ -24,6 +24,7 @@ struct LevelArgs {
     MergeState *states; // [ntask]
     int ntask, r, stride;
     int maxn; // largest merged row count of the level (sizes the deflation LDS)
+    int maxit; // iteration limit of the secular solver per root (a root that reaches it adds 1 to info)
 };
 
 struct CopyDesc {
@@ -91,6 +92,8 @@ hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hip
 
 // bidiag_grid.hip (large single matrices)
 size_t bidiag_grid_ws_bytes(int M, int N);
+// whether the LDS of the chip-wide path's kernels holds an M x N problem
+bool bidiag_grid_supported(int M, int N);
 hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s);
 
 // dc.hip

@@ -16,6 +16,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -39,6 +40,7 @@ struct Req {
     double t_submit;
     size_t fp;
     bool wide;
+    int err; // nonzero: its enqueue failed; reported with latency -1
 };
 
 struct ServeLane {
@@ -67,8 +69,11 @@ struct Server {
     bool stop = false;
     int error = 0;
     double t_first = 0.0, t_last = 0.0;
-    long batches = 0, batched_jobs = 0, wide_jobs = 0;
+    long batches = 0, batched_jobs = 0, wide_jobs = 0, failed = 0;
     int max_inflight = 0;
+    // PACO_SERVE_FAIL_EVERY=n: every n-th admission fails before anything is enqueued
+    // (tests of the error path)
+    long fail_every = 0, admissions = 0;
 
     void loop();
     bool admit_locked();
@@ -104,7 +109,10 @@ bool Server::admit_locked()
         }
         if (take.empty()) continue;
         int rc = 0;
-        if (L.wide) {
+        ++admissions;
+        if (fail_every > 0 && admissions % fail_every == 0) {
+            rc = PACO_EINJECTED;
+        } else if (L.wide) {
             rc = wide_enqueue(take[0].job, L.scratch, L.s);
             ++wide_jobs;
         } else {
@@ -114,9 +122,23 @@ bool Server::admit_locked()
             ++batches;
             batched_jobs += (long)jobs.size();
         }
-        if (rc == 0) rc = (int)hipEventRecord(L.done, L.s);
         if (rc != 0) {
-            error = rc;
+            // the requests are reported as failed once the lane's stream has passed
+            // whatever part of them was enqueued (their buffers stay in use until then)
+            if (error == 0) error = rc;
+            for (auto &r : take) r.err = rc;
+        }
+        const hipError_t er = hipEventRecord(L.done, L.s);
+        if (er != hipSuccess) {
+            if (error == 0) error = (int)er;
+            (void)hipStreamSynchronize(L.s);
+            const double t = now_s();
+            for (auto &r : take) {
+                finished.emplace_back(r.tag, -1.0);
+                ++failed;
+            }
+            t_last = t;
+            cv.notify_all();
             continue;
         }
         L.busy = true;
@@ -145,6 +167,11 @@ void Server::loop()
             for (int i : done_idx) {
                 ServeLane &L = lanes[i];
                 for (auto &r : L.reqs) {
+                    if (r.err) {
+                        finished.emplace_back(r.tag, -1.0);
+                        ++failed;
+                        continue;
+                    }
                     const double ms = 1000.0 * (t - r.t_submit);
                     lat_ms.push_back(ms);
                     finished.emplace_back(r.tag, ms);
@@ -183,6 +210,7 @@ void *paco_server_create(size_t hbm_budget_bytes, int max_batch, int batch_lanes
     Server *S = new Server();
     S->budget = hbm_budget_bytes;
     S->max_batch = max_batch;
+    if (const char *fe = getenv("PACO_SERVE_FAIL_EVERY")) S->fail_every = atol(fe);
     const size_t cap = hbm_budget_bytes / (size_t)batch_lanes;
     for (int i = 0; i < batch_lanes + wide_lanes; ++i) {
         ServeLane L;
@@ -211,10 +239,10 @@ void *paco_server_create(size_t hbm_budget_bytes, int max_batch, int batch_lanes
 int paco_server_submit(void *srv, const paco_svd_job *job, uint64_t tag)
 {
     Server *S = static_cast<Server *>(srv);
-    if (!S || !job || job->M < job->N || job->N < 1 || job->N > PACO_MAX_N || !job->A || !job->S || !job->U ||
-        !job->Vt)
+    if (!S || !job || !job->A || !job->S || !job->U || !job->Vt || !job_supported(job->M, job->N))
         return PACO_EINVAL;
     Req r;
+    r.err = 0;
     r.job = *job;
     r.tag = tag;
     r.t_submit = now_s();
@@ -255,7 +283,7 @@ int paco_server_drain(void *srv)
     S->cv.wait(lk, [S] {
         bool busy = false;
         for (auto &L : S->lanes) busy |= L.busy;
-        return (S->pending.empty() && !busy) || S->error != 0;
+        return S->pending.empty() && !busy;
     });
     return S->error;
 }
@@ -276,7 +304,7 @@ int paco_server_stats(void *srv, double *out, int max)
     for (double x : v) mean += x;
     if (!v.empty()) mean /= (double)v.size();
     const double span = S->t_last > S->t_first ? S->t_last - S->t_first : 0.0;
-    const double vals[9] = {(double)v.size(),
+    const double vals[10] = {(double)v.size(),
                             pct(0.50),
                             pct(0.99),
                             mean,
@@ -284,8 +312,9 @@ int paco_server_stats(void *srv, double *out, int max)
                             (double)S->batches,
                             S->batches ? (double)S->batched_jobs / (double)S->batches : 0.0,
                             (double)S->max_inflight,
-                            (double)S->wide_jobs};
-    const int n = std::min(max, 9);
+                            (double)S->wide_jobs,
+                            (double)S->failed};
+    const int n = std::min(max, 10);
     for (int i = 0; i < n; ++i) out[i] = vals[i];
     return n;
 }

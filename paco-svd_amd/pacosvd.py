@@ -14,7 +14,7 @@ EXPORTS = [
     "paco_upper_bidiagonalization", "paco_svd_DC", "getTilingSize", "GenerateTiling",
     "paco_workspace_size", "paco_set_precision", "paco_svd_plain", "paco_svd_batched",
     "paco_svd_batched_graph", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
-    "paco_timing_enable", "paco_timing_read",
+    "paco_timing_enable", "paco_timing_read", "paco_info_offset", "paco_set_secular_maxiter",
     "paco_server_create", "paco_server_submit", "paco_server_poll", "paco_server_drain", "paco_server_stats",
     "paco_server_destroy",
 ]
@@ -51,6 +51,9 @@ def lib():
     L.paco_batch_workspace_size.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     L.paco_batch_workspace_size.restype = ctypes.c_size_t
     L.paco_last_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, _vp]
+    L.paco_info_offset.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.paco_info_offset.restype = ctypes.c_size_t
+    L.paco_set_secular_maxiter.argtypes = [ctypes.c_int]
     L.paco_gemm_f32.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_float, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, _vp, ctypes.c_int]
     L.paco_timing_enable.argtypes = [ctypes.c_int]
@@ -199,6 +202,18 @@ def set_precision(mode):
     return lib().paco_set_precision(mode)
 
 
+def set_secular_maxiter(maxit):
+    """Iteration limit of the secular solver per root; returns the previous one."""
+    return lib().paco_set_secular_maxiter(maxit)
+
+
+def entry_info(ws, M, N):
+    """Info word the two entry points wrote into workspace tensor ws (synchronises)."""
+    import torch
+    off = lib().paco_info_offset(M, N)
+    return int(ws[off:off + 4].view(torch.int32).item())
+
+
 def timing_enable(on=True):
     _check(lib().paco_timing_enable(1 if on else 0), "paco_timing_enable")
 
@@ -211,7 +226,7 @@ def timing_read():
 
 
 SERVER_STATS = ["completed", "p50_ms", "p99_ms", "mean_ms", "svds_per_s", "batches", "mean_batch", "max_lanes_busy",
-                "wide_requests"]
+                "wide_requests", "failed"]
 
 
 class Server:

@@ -1,0 +1,217 @@
+"""Boundary behaviour of the C ABI on the GPU: status words, size limits, streams.
+
+* info < 0: a NaN / infinite input is reported per job (-1) and solved as the zero
+  matrix; the other jobs of the same call are unaffected.  The reference checks nothing
+  (svd.cpp:134-195, data_utils.h:41-47), so the expected values come from the ABI's own
+  contract, not from the reference.
+* info > 0: the secular iteration limit is lowered from the host (paco_set_secular_maxiter)
+  so roots stop early; every kernel still ends normally and info counts the roots.
+* mixed batches (narrow + wide jobs) report info in call order.
+* calls on two streams at once use separate library scratch (results equal the serial ones).
+* a shape beyond the kernels' LDS limits is refused before anything is launched.
+* the serving loop reports a request whose enqueue failed (PACO_SERVE_FAIL_EVERY) and
+  still completes the others.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from svdcheck import measures
+
+pytestmark = pytest.mark.gpu
+
+SIG_TOL = 1e-5
+ORTH_TOL = 1e-5
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    import pacosvd
+    pacosvd.lib()
+    return torch
+
+
+def np_of(t):
+    return t.detach().cpu().numpy()
+
+
+def good(A, out):
+    m = measures(A, *(np_of(x) for x in out))
+    return m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL, m
+
+
+@pytest.mark.parametrize("path", ["batched", "plain", "graph"])
+def test_nonfinite_input_sets_negative_info(torch_dev, path):
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(41)
+    mats = [rng.standard_normal(sh).astype(np.float32) for sh in [(60, 40), (90, 90), (33, 20), (128, 96)]]
+    mats[1][5, 7] = np.nan
+    mats[3][0, 0] = np.inf
+    say("nonfinite", path, [m.shape for m in mats])
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    if path == "plain":
+        outs = pacosvd.svd_plain(As)
+    else:
+        outs = pacosvd.svd_batched(As, graph=(path == "graph"))
+    torch.cuda.synchronize()
+    info = pacosvd.last_info(len(As))
+    say("info", info)
+    assert info == [0, -1, 0, -1]
+    for i in (0, 2):
+        ok, m = good(mats[i], outs[i])
+        assert ok, (i, m)
+    for i in (1, 3):
+        assert np.all(np_of(outs[i][0]) == 0.0)  # solved as the zero matrix
+
+
+def test_entry_point_info_word(torch_dev):
+    torch = torch_dev
+    import pacosvd
+    A = np.random.default_rng(3).standard_normal((50, 30)).astype(np.float32)
+    A[10, 3] = np.nan
+    say("entry info 50x30 with NaN")
+    a = torch.from_numpy(A).cuda()
+    r = pacosvd.upper_bidiagonalization(a)
+    pacosvd.svd_DC(50, 30, r["a"], r["u"], r["vt"], r["d"], r["e"], ws=r["ws"])
+    torch.cuda.synchronize()
+    assert pacosvd.entry_info(r["ws"], 50, 30) == -1
+    A2 = np.random.default_rng(4).standard_normal((50, 30)).astype(np.float32)
+    r = pacosvd.upper_bidiagonalization(torch.from_numpy(A2).cuda(), ws=r["ws"])
+    pacosvd.svd_DC(50, 30, r["a"], r["u"], r["vt"], r["d"], r["e"], ws=r["ws"])
+    torch.cuda.synchronize()
+    assert pacosvd.entry_info(r["ws"], 50, 30) == 0
+
+
+def test_secular_iteration_limit_sets_positive_info(torch_dev):
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(8)
+    mats = [rng.standard_normal(sh).astype(np.float32) for sh in [(200, 150), (64, 64), (300, 299)]]
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    say("maxiter 1", [m.shape for m in mats])
+    old = pacosvd.set_secular_maxiter(1)
+    try:
+        assert old == 200
+        pacosvd.svd_batched(As)
+        torch.cuda.synchronize()
+        info = pacosvd.last_info(len(As))
+    finally:
+        pacosvd.set_secular_maxiter(old)
+    say("info", info)
+    assert all(x > 0 for x in info)
+    outs = pacosvd.svd_batched(As)
+    torch.cuda.synchronize()
+    assert pacosvd.last_info(len(As)) == [0, 0, 0]
+    for i, A in enumerate(mats):
+        ok, m = good(A, outs[i])
+        assert ok, (i, m)
+
+
+def test_mixed_batch_info_in_call_order(torch_dev):
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(12)
+    shapes = [(40, 30), (700, 600), (64, 64), (800, 520), (20, 10)]
+    mats = [rng.standard_normal(sh).astype(np.float32) for sh in shapes]
+    mats[3][100, 200] = np.nan  # a wide job
+    mats[4][0, 0] = np.nan      # a narrow job
+    say("mixed", shapes)
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    outs = pacosvd.svd_batched(As)
+    torch.cuda.synchronize()
+    info = pacosvd.last_info(len(As))
+    say("info", info)
+    assert info == [0, 0, 0, -1, -1]
+    for i in (0, 1, 2):
+        ok, m = good(mats[i], outs[i])
+        assert ok, (i, m)
+
+
+def test_two_streams_at_once(torch_dev):
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(21)
+    sets = [[rng.standard_normal(sh).astype(np.float32) for sh in [(100, 80), (300, 250), (37, 37), (700, 520)]]
+            for _ in range(2)]
+    say("two streams")
+    As = [[torch.from_numpy(m).cuda() for m in s] for s in sets]
+    ref = [pacosvd.svd_batched(a) for a in As]
+    ref_plain = [pacosvd.svd_plain(a) for a in As]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs, outs_plain = [None, None], [None, None]
+    for k in range(2):
+        with torch.cuda.stream(streams[k]):
+            outs[k] = pacosvd.svd_batched(As[k])
+            outs_plain[k] = pacosvd.svd_plain(As[k])
+    torch.cuda.synchronize()
+    for k in range(2):
+        for i in range(len(sets[k])):
+            for x, y in zip(outs[k][i], ref[k][i]):
+                assert torch.equal(x, y), (k, i)
+            for x, y in zip(outs_plain[k][i], ref_plain[k][i]):
+                assert torch.equal(x, y), (k, i)
+
+
+def test_over_limit_shape_refused_before_launch(torch_dev):
+    """40000 x 64 takes the chip-wide path (M N >= 2^20) and its last columns the
+    one-workgroup kernel, whose column vector of 40000 floats does not fit the LDS."""
+    torch = torch_dev
+    import ctypes
+    import pacosvd
+    M, N = 40000, 64
+    say("over limit", (M, N))
+    a = torch.ones((M, N), device="cuda")
+    u = torch.zeros(1, device="cuda")
+    z = torch.zeros(N, device="cuda")
+    L = pacosvd.lib()
+    p = ctypes.c_void_p
+    rc = L.paco_upper_bidiagonalization(40, pacosvd._stream(), M, N, p(a.data_ptr()), p(u.data_ptr()), p(u.data_ptr()),
+                                        p(z.data_ptr()), p(z.data_ptr()), p(z.data_ptr()), p(z.data_ptr()), p(0))
+    assert rc == -1
+    torch.cuda.synchronize()
+    assert torch.all(a == 1.0)  # nothing ran
+    with pytest.raises(RuntimeError):  # refused by the batched path too (outputs never touched)
+        pacosvd.svd_batched([a], outs=[(z, u, u)])
+
+
+def test_server_reports_failed_enqueue(torch_dev):
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(31)
+    shapes = [(40, 30), (200, 150), (1100, 1000), (64, 64), (300, 120), (17, 9)]
+    mats = [rng.standard_normal(sh).astype(np.float32) for sh in shapes]
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    outs = pacosvd.alloc_outputs(As)
+    say("server fail every 2", shapes)
+    os.environ["PACO_SERVE_FAIL_EVERY"] = "2"
+    try:
+        srv = pacosvd.Server(2 << 30, max_batch=1, batch_lanes=1, wide_lanes=1, max_M=1100, max_N=1100)
+    finally:
+        del os.environ["PACO_SERVE_FAIL_EVERY"]
+    try:
+        for i, (A, (S, U, Vt)) in enumerate(zip(As, outs)):
+            assert srv.submit(A, S, U, Vt, i) == 0
+        with pytest.raises(RuntimeError):
+            srv.drain()  # returns the first error once everything has finished or failed
+        done = dict(srv.poll())
+        st = srv.stats()
+    finally:
+        srv.close()
+    say("done", done, st)
+    assert sorted(done) == list(range(len(As)))
+    failed = [t for t, ms in done.items() if ms < 0]
+    assert len(failed) == st["failed"] and 1 <= len(failed) < len(As)
+    for t, ms in done.items():
+        if ms >= 0:
+            ok, m = good(mats[t], outs[t])
+            assert ok, (t, m)
