@@ -4,32 +4,65 @@
 // rank-1 updates of the whole trailing matrix on 40 cores, 4 grid barriers per column).
 // Here: a blocked panel of NB columns (LAPACK dlabrd structure).  Inside a panel the
 // trailing matrix is only read (two GEMVs per column) and the panel's reflectors are
-// applied once per panel as a rank-2*NB update on the matrix cores.  The workgroups
-// form a GR x GC grid of tiles over the trailing matrix.  The Householder scaling of
+// applied once per panel as a rank-2*NB update on the matrix cores.  The trailing
+// matrix of a panel is cut into a GR x GC grid of tiles.  The Householder scaling of
 // each column / row reflector is deferred (partial sums are taken against the
 // unscaled vector and scaled on reduction), so every column needs two grid-wide
-// synchronisation points, realised as two launches:
-//   col kernel (column i): finish x of column i-1 from its GEMV partials (every tile
+// synchronisation points:
+//   column phase (column i): finish x of column i-1 from its GEMV partials (every tile
 //     reduces only its own rows), column-i update, norm partials, GEMV A^T v partials
-//   row kernel (column i): column-i reflector, y for own columns, row-i update, norm
+//   row phase (column i): column-i reflector, y for own columns, row-i update, norm
 //     partials, GEMV A u partials
-// Below SWITCH columns the remaining trailing matrix is finished by the one-workgroup
-// kernel (bidiag_wg_kernel) on the sub-matrix.
+// One persistent launch per panel runs both phases of all NB columns, separated by grid
+// barriers.  Its workgroups register when they start; the first to register closes the
+// registration once every workgroup of the grid has joined (or after a short wait, when
+// other work holds part of the chip) and the participants share the tiles, so two such
+// launches on different streams never wait for each other's workgroups.  Every tile's
+// partial sums sit in their own slot and are reduced in tile order, so the results do
+// not depend on how many workgroups took part.  A barrier wait is bounded: on a timeout
+// the launch (and every later panel of the call) stops and the problem's info is -2.
+// Below the switch column the remaining trailing matrix is finished by the
+// one-workgroup kernel (bidiag_wg_kernel) on the sub-matrix.
+#include <algorithm>
+#include <cstdlib>
+
 #include "gemm_tile.hpp"
 #include "paco_kernels.hpp"
 
 namespace paco {
 
-#define GBT 256
+// threads of a panel workgroup: one row (and one column) of its tile each, 256 VGPRs
+#define GBT 512
 constexpr int NB = 32;
 // workgroup grid over the trailing matrix of a panel: gr x gc tiles (runtime, at most
 // GMAX x GMAX; the workspace holds GMAX partial sums per row / column)
-constexpr int GMAX = 32;
-constexpr int SWITCH = 384;
+constexpr int GMAX = 64;
+constexpr int SWITCH_DEFAULT = 384;
+// columns of a tile at most (its Y and U columns sit in LDS: 2 NB TW_MAX floats)
+constexpr int TW_MAX = 512;
+static inline size_t panel_dyn_floats_host(int th, int tw) { return 2 * (size_t)NB * tw + th + tw; }
+constexpr unsigned REG_CLOSED = 1u << 30;
+constexpr unsigned NPARTS_READY = 1u << 30;
+
+// Synchronisation words of the persistent panel launches (in the workspace).
+struct GridSync {
+    unsigned reg;    // registrations so far | REG_CLOSED once closed
+    unsigned nparts; // participants | NPARTS_READY (written by the first registrant)
+    unsigned count;  // arrivals at the current barrier
+    unsigned gen;    // barriers completed
+    unsigned abort;  // set when a wait timed out: every later wait of the call returns at once
+    unsigned pad[59];
+};
+
+// participants of a panel launch at most (one arrival flag each)
+constexpr int MAX_PARTS = 1024;
+constexpr int FLAG_STRIDE = 16; // unsigned words between two flags (one 64-byte line each)
 
 struct GridWs {
     float *X, *Y, *wcol, *wrow, *g1p, *g2p, *t1p, *t2p, *t3p, *t4p;
     double *ssp, *rsp; // FP64 partial sums of squares (see house_params_d)
+    GridSync *sync;
+    unsigned *flags; // [MAX_PARTS * FLAG_STRIDE] arrival flags of the grid barrier
 };
 
 static size_t carve(int M, int N, char *base, GridWs *W)
@@ -53,6 +86,8 @@ static size_t carve(int M, int N, char *base, GridWs *W)
     w.t2p = take(GMAX * NB);
     w.t3p = take(GMAX * NB);
     w.t4p = take(GMAX * NB);
+    w.sync = reinterpret_cast<GridSync *>(take(sizeof(GridSync) / sizeof(float)));
+    w.flags = reinterpret_cast<unsigned *>(take((size_t)MAX_PARTS * FLAG_STRIDE));
     if (W) *W = w;
     return o;
 }
@@ -62,171 +97,297 @@ size_t bidiag_grid_ws_bytes(int M, int N) { return carve(M, N, nullptr, nullptr)
 struct PanelArgs {
     ProbDesc P;
     GridWs W;
-    int p, j, th, tw, finish_only;
+    int p, th, tw;
     int gr, gc; // tile grid
+    long long timeout; // bound of one barrier wait, in wall-clock ticks
 };
+
+// Data one workgroup writes during a panel and another reads after a barrier (the
+// reflectors in A, X, Y, the reflector columns / rows and every partial sum) moves through
+// device-coherent accesses (agent-scope atomic loads and stores: they bypass the per-XCD
+// L2), so a barrier needs no cache write-back or invalidation.  The trailing matrix
+// itself does not change during a panel and is read through the caches.
+__device__ inline float ldc(const float *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline double ldc(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void stc(float *p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void stc(double *p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// p[0] + p[stride] + ... + p[(n - 1) stride] through device-coherent loads, added in that
+// order, with 16 loads in flight (a loop of dependent loads would wait for each one)
+template <typename T>
+__device__ __attribute__((always_inline)) inline T ldc_sum(const T *p, size_t stride, int n)
+{
+    T s = 0;
+    for (int q0 = 0; q0 < n; q0 += 16) {
+        T v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = ldc(p + (size_t)min(q0 + k, n - 1) * stride);
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (q0 + k < n) s += v[k];
+    }
+    return s;
+}
 
 // column reflector t of the panel (unit at row p+t), entry at row r
 __device__ inline float Vel(const float *A, int N, int p, int t, int r)
 {
     const int c = p + t;
-    return r > c ? A[(size_t)r * N + c] : (r == c ? 1.f : 0.f);
+    return r > c ? ldc(A + (size_t)r * N + c) : (r == c ? 1.f : 0.f);
 }
 // row reflector t of the panel (row p+t, unit at column p+t+1), entry at column col
 __device__ inline float Uel(const float *A, int N, int p, int t, int col)
 {
     const int u = p + t + 1;
-    return col > u ? A[(size_t)(p + t) * N + col] : (col == u ? 1.f : 0.f);
+    return col > u ? ldc(A + (size_t)(p + t) * N + col) : (col == u ? 1.f : 0.f);
 }
 
-__device__ inline double gb_block_sum(double v, double *red)
+// Per-tile state of a panel.  Row role: thread tid owns row r0 + tid (th <= GBT) and
+// keeps V[row][t] and X[row][t] of the panel's reflectors in registers, and c (column i
+// at its row) from the column phase to the row phase.  Column role: thread tid < tw owns
+// column c0 + tid; Y[col][t] and U[t][col] of its column sit in LDS (Yl, Ul), w (row i
+// at its column) stays in a register from the row phase to the next column phase.  When
+// a workgroup takes more than one tile (fewer participants than tiles, or one launch per
+// phase), the state is reloaded from the global copies at the start of every phase: the
+// same values, so the results are the same bit for bit.
+struct RowState {
+    float v[NB], x[NB];
+    float c, w;
+};
+
+// every entry zero: threads without a row or column multiply their (unused) state by zero
+// in the block reductions, which must not see stale non-finite register contents
+__device__ __attribute__((always_inline)) inline void row_state_zero(RowState &R)
 {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    v = wave_sum(v);
-    __syncthreads();
-    if (lane == 0) red[wv] = v;
-    __syncthreads();
-    double r = 0.0;
 #pragma unroll
-    for (int w = 0; w < GBT / 64; ++w) r += red[w];
-    return r;
+    for (int t = 0; t < NB; ++t) {
+        R.v[t] = 0.f;
+        R.x[t] = 0.f;
+    }
+    R.c = R.w = 0.f;
 }
 
-// dynamic LDS: xs[th] (x of the previous column), wc[th] (updated column i)
-__global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
+// LDS of the panel launch: the dynamic part holds Yl[NB][tw], Ul[NB][tw], wc[th], wl[tw].
+struct PhaseLds {
+    union {
+        float4 red4[GBT];   // GEMV1 column sums across the wavefronts
+        struct {            // partial sums of all tiles, staged for an in-order reduction
+            float pa[GMAX][NB], pb[GMAX][NB];
+            double pd[GMAX];
+        } st;
+    } u;
+    float redk[GBT / 64][2 * NB];
+    float crossa[NB], crossb[NB];               // row i / column i entries of the panel vectors
+    float ta[NB + 1], tb[NB + 1];               // combined coefficients of the current phase
+    double red[GBT / 64];
+};
+
+// a[j] = v for a runtime j (a select per entry keeps the array in registers)
+__device__ __attribute__((always_inline)) inline void set_at(float (&a)[NB], int j, float v)
 {
-    const int GR = a.gr, GC = a.gc;
-    const int rb = blockIdx.x / GC, cb = blockIdx.x % GC;
+#pragma unroll
+    for (int t = 0; t < NB; ++t) a[t] = (t == j) ? v : a[t];
+}
+
+// value of a[t] for a runtime t < NB (a select chain keeps the array in registers)
+__device__ __attribute__((always_inline)) inline float get_at(const float (&a)[NB], int t)
+{
+    float v = 0.f;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) v = (q == t) ? a[q] : v;
+    return v;
+}
+
+// Column phase of tile (rb, cb) for panel column j (j == NB: only the finish of x of
+// column NB - 1 and the row reflector of row p + NB - 1).
+__device__ __attribute__((always_inline)) inline void gb_col_phase(const PanelArgs &a, int j, int rb, int cb, float *dyn, PhaseLds &S, RowState &R,
+                             bool resident)
+{
+    const int GC = a.gc;
+    const bool finish_only = j == NB;
     const ProbDesc &P = a.P;
     const GridWs &W = a.W;
-    const int M = P.M, N = P.N, p = a.p, j = a.j, i = p + j;
+    const int M = P.M, N = P.N, p = a.p, i = p + j, ip = i - 1, jp = j - 1;
     float *A = P.a;
     const int r0 = p + rb * a.th, r1 = min(M, r0 + a.th);
     const int c0 = p + cb * a.tw, c1 = min(N, c0 + a.tw);
-    extern __shared__ float sm[];
-    float *xs = sm, *wc = sm + a.th;
-    __shared__ float t3s[NB], t4s[NB], ys[NB], us[NB];
-    __shared__ double red[GBT / 64];
-    __shared__ float s_tau2, s_inv2, s_beta2;
+    const int tw = a.tw;
+    float *Yl = dyn, *Ul = dyn + NB * tw, *wc = dyn + 2 * NB * tw;
     const int tid = threadIdx.x;
+    const int row = r0 + tid, col = c0 + tid;
+    const bool rowv = row < r1 && tid < a.th, colv = tid < tw && col < c1;
 
+    // ---- one round of loads, spread over the threads: every tile's partial sums into LDS
+    // (reduced in tile order below), the row i / column i entries, this thread's own row
+    float x12 = 0.f;
     if (j > 0) {
-        // ---- finish the row reflector and x of column ip = i - 1
-        const int ip = i - 1, jp = j - 1;
-        if (tid == 0) {
-            double sigma2 = 0.0;
-            for (int q = 0; q < GC; ++q) sigma2 += W.rsp[q];
-            float beta2, tau2, inv2;
-            house_params_d(W.wrow[ip + 1], sigma2, &beta2, &tau2, &inv2);
-            s_tau2 = tau2;
-            s_inv2 = inv2;
-            s_beta2 = beta2;
+        for (int q = tid; q < GC * NB; q += GBT) {
+            S.u.st.pa[q / NB][q % NB] = ldc(W.t3p + q);
+            S.u.st.pb[q / NB][q % NB] = ldc(W.t4p + q);
         }
-        __syncthreads();
-        const float tau2 = s_tau2, inv2 = s_inv2;
+        if (tid < GC) S.u.st.pd[tid] = ldc(W.rsp + tid);
+        if (tid == GBT - 1) x12 = ldc(W.wrow + ip + 1);
+    }
+    if (tid < NB) {
+        const int t = tid;
+        // Y[i][t] (t < j) and U[t][i] (t < j; the unit head for t = j - 1)
+        S.crossa[t] = t < j ? ldc(W.Y + (size_t)i * NB + t) : 0.f;
+        S.crossb[t] = t < j ? Uel(A, N, p, t, i) : 0.f;
+    }
+    float g2 = 0.f, ari = 0.f;
+    const bool live = rowv && row >= i;
+    if (live) {
+        if (j > 0) g2 = ldc_sum(W.g2p + row, (size_t)M, GC);
+        if (!finish_only) ari = A[(size_t)row * N + i];
+        else ari = A[(size_t)row * N + ip + 1];
+    }
+    if (!resident) {
+        if (rowv)
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                // X[row][t] exists for rows below p + t, Y[col][t] right of p + t (the
+                // workspace holds stale values elsewhere): the same zeros as the resident state
+                R.v[t] = t < j ? Vel(A, N, p, t, row) : 0.f;
+                R.x[t] = (t < jp && row > p + t) ? ldc(W.X + (size_t)row * NB + t) : 0.f;
+            }
+        if (colv) {
+            for (int t = 0; t < NB; ++t) {
+                Yl[t * tw + tid] = (t < j && col > p + t) ? ldc(W.Y + (size_t)col * NB + t) : 0.f;
+                Ul[t * tw + tid] = t < jp ? Uel(A, N, p, t, col) : 0.f;
+            }
+            R.w = (j > 0 && col > ip) ? ldc(W.wrow + col) : 0.f;
+        }
+    }
+    if (j > 0 && tid == GBT - 1) S.ta[NB] = x12;
+    __syncthreads();
+    if (j > 0) {
+        // every thread forms the row reflector's parameters from the staged sums (the same
+        // sums in the same order: the same values everywhere)
+        double sig2 = 0.0;
+        for (int q = 0; q < GC; ++q) sig2 += S.u.st.pd[q];
+        float beta2, tau2, inv2;
+        house_params_d(S.ta[NB], sig2, &beta2, &tau2, &inv2);
         if (tid < NB) {
             const int t = tid;
             float s3 = 0.f, s4 = 0.f;
             for (int q = 0; q < GC; ++q) {
-                s3 += W.t3p[q * NB + t];
-                s4 += W.t4p[q * NB + t];
+                s3 += S.u.st.pa[q][t];
+                s4 += S.u.st.pb[q][t];
             }
-            t3s[t] = (t <= jp) ? W.Y[(size_t)(ip + 1) * NB + t] + inv2 * s3 : 0.f;
-            t4s[t] = (t < jp) ? Uel(A, N, p, t, ip + 1) + inv2 * s4 : 0.f;
+            S.ta[t] = t <= jp ? S.crossa[t] + inv2 * s3 : 0.f; // t3: Y^T u
+            S.tb[t] = t < jp ? S.crossb[t] + inv2 * s4 : 0.f;  // t4: U u
         }
         __syncthreads();
-        for (int row = max(r0, i) + tid; row < r1; row += GBT) {
-            float g2 = 0.f;
-            for (int q = 0; q < GC; ++q) g2 += W.g2p[(size_t)q * M + row];
-            float acc = A[(size_t)row * N + ip + 1] + inv2 * g2;
-#pragma unroll 8
-            for (int t = 0; t <= jp; ++t) acc -= Vel(A, N, p, t, row) * t3s[t];
-#pragma unroll 8
-            for (int t = 0; t < jp; ++t) acc -= W.X[(size_t)row * NB + t] * t4s[t];
+        // ---- finish the row reflector of row ip and x of column ip
+        if (live) {
+            float acc = ari + inv2 * g2;
+#pragma unroll
+            for (int t = 0; t < NB; ++t) acc -= R.v[t] * S.ta[t];
+#pragma unroll
+            for (int t = 0; t < NB; ++t) acc -= R.x[t] * S.tb[t];
             const float x = tau2 * acc;
-            xs[row - r0] = x;
-            if (cb == 0) W.X[(size_t)row * NB + jp] = x;
+            set_at(R.x, jp, x);
+            if (cb == 0) stc(W.X + (size_t)row * NB + jp, x);
         }
-        // the row reflector u_{ip} into A (row ip), its head e_{ip}
-        if (rb == 0) {
-            for (int col = max(c0, ip + 1) + tid; col < c1; col += GBT) {
-                if (col == ip + 1) // unit head while the trailing update reads row ip as U
-                    A[(size_t)ip * N + col] = a.finish_only ? 1.f : s_beta2;
-                else
-                    A[(size_t)ip * N + col] = (tau2 != 0.f) ? W.wrow[col] * inv2 : W.wrow[col];
-            }
-            if (cb == 0 && tid == 0) {
-                P.e[ip] = s_beta2;
-                P.taup[ip] = tau2;
-            }
+        // the row reflector u_{ip} for own columns; into A (row ip) by the first tile row
+        if (colv && col >= ip + 1) {
+            const float u = (col == ip + 1) ? 1.f : (tau2 != 0.f) ? R.w * inv2 : R.w;
+            Ul[jp * tw + tid] = u;
+            if (rb == 0)
+                stc(A + (size_t)ip * N + col, col == ip + 1 ? (finish_only ? 1.f : beta2) : u);
+        } else if (colv) {
+            Ul[jp * tw + tid] = 0.f;
         }
+        if (rb == 0 && cb == 0 && tid == 0) {
+            P.e[ip] = beta2;
+            P.taup[ip] = tau2;
+        }
+    }
+    if (finish_only) {
         __syncthreads();
+        return;
     }
-    if (a.finish_only) return;
 
-    // ---- column i update for rows >= i of this block (redundant over cb)
-    if (tid < NB) {
-        ys[tid] = (tid < j) ? W.Y[(size_t)i * NB + tid] : 0.f;
-        us[tid] = (tid < j) ? Uel(A, N, p, tid, i) : 0.f;
-    }
-    __syncthreads();
+    // ---- column i for own rows >= i (redundant over cb)
     double part = 0.0;
-    for (int row = max(r0, i) + tid; row < r1; row += GBT) {
-        float val = A[(size_t)row * N + i];
-#pragma unroll 8
-        for (int t = 0; t < j; ++t) {
-            const float xv = (t == j - 1) ? xs[row - r0] : W.X[(size_t)row * NB + t];
-            val -= Vel(A, N, p, t, row) * ys[t] + xv * us[t];
-        }
-        wc[row - r0] = val;
-        if (cb == 0) W.wcol[row] = val;
-        if (row > i) part += (double)val * val;
+    if (live) {
+        float val = ari;
+#pragma unroll
+        for (int t = 0; t < NB; ++t) val -= R.v[t] * S.crossa[t] + R.x[t] * S.crossb[t];
+        wc[tid] = val;
+        R.c = val;
+        if (cb == 0) stc(W.wcol + row, val);
+        if (row > i) part = (double)val * val;
+    } else if (rowv) {
+        wc[tid] = 0.f;
     }
-    const double ss = gb_block_sum(part, red);
-    if (cb == 0 && tid == 0) W.ssp[rb] = ss;
+    // one reduction for the sum of squares and this tile's share of t1p[rb][t] =
+    // sum_{row > i} V[row][t] c[row], t2p[rb][t] = sum X[row][t] c[row] (t < j): the 2 NB
+    // entries are shared out over the GC tiles of the row block
+    const int K1 = (2 * NB + GC - 1) / GC, e01 = cb * K1, ke1 = max(0, min(K1, 2 * NB - e01));
+    {
+        const int lane = tid & 63, wv = tid >> 6;
+        const float cw = (rowv && row > i) ? R.c : 0.f;
+        const double ws = wave_sum(part);
+        if (lane == 0) S.red[wv] = ws;
+        for (int k = 0; k < ke1; ++k) {
+            const int e = e01 + k;
+            const float v = wave_sum_dpp((e < NB ? get_at(R.v, e) : get_at(R.x, e - NB)) * cw);
+            if (lane == 0) S.redk[wv][k] = v;
+        }
+    }
+    __syncthreads(); // (also publishes wc)
+    if (tid == 0) {
+        double ss = 0.0;
+#pragma unroll
+        for (int w = 0; w < GBT / 64; ++w) ss += S.red[w];
+        if (cb == 0) stc(W.ssp + rb, ss);
+    }
+    if (tid < ke1) {
+        float tot = 0.f;
+#pragma unroll
+        for (int q = 0; q < GBT / 64; ++q) tot += S.redk[q][tid];
+        const int e = e01 + tid, t = e % NB;
+        stc((e < NB ? W.t1p : W.t2p) + rb * NB + t, t < j ? tot : 0.f);
+    }
     // ---- GEMV1 partials: g1p[rb][col] = sum_{row > i} wc[row] A[row][col], col > i
     const int rs = max(r0, i + 1);
-    if ((N & 3) == 0) {
-        // float4 column groups x row groups, 8 loads in flight per thread, LDS reduction
-        __shared__ float4 red4[GBT];
+    if ((N & 3) == 0 && rs < r1) {
+        // lanes over float4 column groups (64 per pass), the 8 wavefronts over interleaved
+        // rows in batches of 8 loads in flight; addresses clamped into the tile, invalid
+        // rows weighted 0, invalid columns never stored
+        const int lane = tid & 63, wv = tid >> 6;
         const int ca = max(c0, i + 1) & ~3, ncg = (c1 - ca + 3) >> 2;
-        int cg = 1;
-        while (cg < ncg && cg < GBT) cg <<= 1;
-        const int RG = GBT / cg, g = tid / cg, c = tid % cg;
-        for (int cgb = 0; cgb < ncg; cgb += cg) {
-            const int col = ca + 4 * (cgb + c);
+        for (int cg0 = 0; cg0 < ncg; cg0 += 64) {
+            const int cc = ca + 4 * min(cg0 + lane, ncg - 1);
             float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (cgb + c < ncg) {
-                int row = rs + g;
-                for (; row + 7 * RG < r1; row += 8 * RG) {
-                    float4 v[8];
+            for (int rb0 = rs + wv; rb0 < r1; rb0 += 8 * (GBT / 64)) {
+                float4 v[8];
+                float wk[8];
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) v[k] = *reinterpret_cast<const float4 *>(A + (size_t)(row + k * RG) * N + col);
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        const float w = wc[row + k * RG - r0];
-                        acc.x += w * v[k].x;
-                        acc.y += w * v[k].y;
-                        acc.z += w * v[k].z;
-                        acc.w += w * v[k].w;
-                    }
+                for (int k = 0; k < 8; ++k) {
+                    const int rr = rb0 + k * (GBT / 64);
+                    v[k] = *reinterpret_cast<const float4 *>(A + (size_t)min(rr, r1 - 1) * N + cc);
+                    wk[k] = rr < r1 ? wc[rr - r0] : 0.f;
                 }
-                for (; row < r1; row += RG) {
-                    const float4 v = *reinterpret_cast<const float4 *>(A + (size_t)row * N + col);
-                    const float w = wc[row - r0];
-                    acc.x += w * v.x;
-                    acc.y += w * v.y;
-                    acc.z += w * v.z;
-                    acc.w += w * v.w;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    acc.x += wk[k] * v[k].x;
+                    acc.y += wk[k] * v[k].y;
+                    acc.z += wk[k] * v[k].z;
+                    acc.w += wk[k] * v[k].w;
                 }
             }
+            S.u.red4[wv * 64 + lane] = acc;
             __syncthreads();
-            red4[tid] = acc;
-            __syncthreads();
-            if (g == 0 && cgb + c < ncg) {
-                float4 t = red4[c];
-                for (int q = 1; q < RG; ++q) {
-                    const float4 u = red4[q * cg + c];
+            if (wv == 0 && cg0 + lane < ncg) {
+                float4 t = S.u.red4[lane];
+#pragma unroll
+                for (int q = 1; q < GBT / 64; ++q) {
+                    const float4 u = S.u.red4[q * 64 + lane];
                     t.x += u.x;
                     t.y += u.y;
                     t.z += u.z;
@@ -235,176 +396,422 @@ __global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a)
                 const float tv[4] = {t.x, t.y, t.z, t.w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) // own columns only (the aligned start may reach into the previous tile)
-                    if (col + e > i && col + e >= c0 && col + e < c1) W.g1p[(size_t)rb * N + col + e] = tv[e];
+                    if (cc + e > i && cc + e >= c0 && cc + e < c1) stc(W.g1p + (size_t)rb * N + cc + e, tv[e]);
             }
+            __syncthreads();
         }
-    } else {
-        for (int col = max(c0, i + 1) + tid; col < c1; col += GBT) {
+    } else if (rs < r1) {
+        for (int cc = max(c0, i + 1) + tid; cc < c1; cc += GBT) {
             float acc = 0.f;
 #pragma unroll 8
-            for (int row = rs; row < r1; ++row) acc += wc[row - r0] * A[(size_t)row * N + col];
-            W.g1p[(size_t)rb * N + col] = acc;
+            for (int rr = rs; rr < r1; ++rr) acc += wc[rr - r0] * A[(size_t)rr * N + cc];
+            stc(W.g1p + (size_t)rb * N + cc, acc);
         }
+    } else {
+        for (int cc = max(c0, i + 1) + tid; cc < c1; cc += GBT) stc(W.g1p + (size_t)rb * N + cc, 0.f);
     }
-    // t1p[rb][t] = sum_{row > i} V[row][t] wc[row], t2p[rb][t] = sum X[row][t] wc[row], t < j;
-    // 32 column lanes x 8 row groups, reduced through LDS
-    if (cb == 0) {
-        __shared__ float r1s[8][NB + 1], r2s[8][NB + 1];
-        const int t = tid % NB, g = tid / NB;
-        float s1 = 0.f, s2 = 0.f;
-        if (t < j) {
-#pragma unroll 4
-            for (int row = rs + g; row < r1; row += 8) {
-                const float w = wc[row - r0];
-                s1 += Vel(A, N, p, t, row) * w;
-                s2 += ((t == j - 1) ? xs[row - r0] : W.X[(size_t)row * NB + t]) * w;
-            }
-        }
-        r1s[g][t] = s1;
-        r2s[g][t] = s2;
-        __syncthreads();
-        if (tid < NB) {
-            float a1 = 0.f, a2 = 0.f;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                a1 += r1s[q][tid];
-                a2 += r2s[q][tid];
-            }
-            W.t1p[rb * NB + tid] = a1;
-            W.t2p[rb * NB + tid] = a2;
-        }
-    }
+    __syncthreads();
 }
 
-// dynamic LDS: yl[tw] (y of column i for own columns), wl[tw] (updated row i)
-__global__ __launch_bounds__(GBT) void gb_row_kernel(PanelArgs a)
+// Row phase of tile (rb, cb) for panel column j.
+__device__ __attribute__((always_inline)) inline void gb_row_phase(const PanelArgs &a, int j, int rb, int cb, float *dyn, PhaseLds &S, RowState &R,
+                             bool resident)
 {
-    const int GR = a.gr, GC = a.gc;
-    const int rb = blockIdx.x / GC, cb = blockIdx.x % GC;
+    const int GR = a.gr;
     const ProbDesc &P = a.P;
     const GridWs &W = a.W;
-    const int M = P.M, N = P.N, p = a.p, j = a.j, i = p + j;
+    const int M = P.M, N = P.N, p = a.p, i = p + j;
     float *A = P.a;
     const int r0 = p + rb * a.th, r1 = min(M, r0 + a.th);
     const int c0 = p + cb * a.tw, c1 = min(N, c0 + a.tw);
-    extern __shared__ float sm[];
-    float *yl = sm, *wl = sm + a.tw;
-    __shared__ float t1s[NB], t2s[NB], xi[NB], vi[NB];
-    __shared__ double red[GBT / 64];
-    __shared__ float s_tau, s_inv, s_beta;
+    const int tw = a.tw;
+    float *Yl = dyn, *Ul = dyn + NB * tw, *wl = dyn + 2 * NB * tw + a.th;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int row = r0 + tid, col = c0 + tid;
+    const bool rowv = row < r1 && tid < a.th, colv = tid < tw && col < c1;
 
-    // ---- column reflector of column i
-    if (tid == 0) {
-        double sigma = 0.0;
-        for (int q = 0; q < GR; ++q) sigma += W.ssp[q];
-        float beta, tau, inv;
-        house_params_d(W.wcol[i], sigma, &beta, &tau, &inv);
-        s_tau = tau;
-        s_inv = inv;
-        s_beta = beta;
+    // ---- one round of loads, spread over the threads (as in the column phase)
+    float x1 = 0.f;
+    for (int q = tid; q < GR * NB; q += GBT) {
+        S.u.st.pa[q / NB][q % NB] = ldc(W.t1p + q);
+        S.u.st.pb[q / NB][q % NB] = ldc(W.t2p + q);
     }
-    __syncthreads();
-    const float tau = s_tau, inv = s_inv;
-    if (cb == 0) {
-        for (int row = max(r0, i + 1) + tid; row < r1; row += GBT)
-            A[(size_t)row * N + i] = (tau != 0.f) ? W.wcol[row] * inv : W.wcol[row];
-        if (rb == 0 && tid == 0) {
-            A[(size_t)i * N + i] = s_beta;
-            P.d[i] = s_beta;
-            P.tauq[i] = tau;
+    if (tid < GR) S.u.st.pd[tid] = ldc(W.ssp + tid);
+    if (tid == GBT - 1) x1 = ldc(W.wcol + i);
+    if (tid < NB) {
+        const int t = tid;
+        // V[i][t] and X[i][t], t < j
+        S.crossa[t] = t < j ? Vel(A, N, p, t, i) : 0.f;
+        S.crossb[t] = t < j ? ldc(W.X + (size_t)i * NB + t) : 0.f;
+    }
+    float g1 = 0.f, aic = 0.f;
+    const bool clive = colv && col > i;
+    if (clive) {
+        g1 = ldc_sum(W.g1p + col, (size_t)N, GR);
+        aic = A[(size_t)i * N + col];
+    }
+    if (!resident) {
+        if (rowv) {
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                R.v[t] = t < j ? Vel(A, N, p, t, row) : 0.f;
+                R.x[t] = (t < j && row > p + t) ? ldc(W.X + (size_t)row * NB + t) : 0.f;
+            }
+            R.c = row >= i ? ldc(W.wcol + row) : 0.f;
+        }
+        if (colv) {
+            for (int t = 0; t < NB; ++t) {
+                Yl[t * tw + tid] = (t < j && col > p + t) ? ldc(W.Y + (size_t)col * NB + t) : 0.f;
+                Ul[t * tw + tid] = t < j ? Uel(A, N, p, t, col) : 0.f;
+            }
         }
     }
-    if (i == N - 1) return;
-
+    if (tid == GBT - 1) S.ta[NB] = x1;
+    __syncthreads();
+    // every thread forms the column reflector's parameters from the staged sums
+    double sig = 0.0;
+    for (int q = 0; q < GR; ++q) sig += S.u.st.pd[q];
+    float beta, tau, inv;
+    house_params_d(S.ta[NB], sig, &beta, &tau, &inv);
+    float ta_t = 0.f, tb_t = 0.f;
     if (tid < NB) {
         const int t = tid;
         float s1 = 0.f, s2 = 0.f;
         for (int q = 0; q < GR; ++q) {
-            s1 += W.t1p[q * NB + t];
-            s2 += W.t2p[q * NB + t];
+            s1 += S.u.st.pa[q][t];
+            s2 += S.u.st.pb[q][t];
         }
-        t1s[t] = (t < j) ? Vel(A, N, p, t, i) + inv * s1 : 0.f;
-        t2s[t] = (t < j) ? W.X[(size_t)i * NB + t] + inv * s2 : 0.f;
-        xi[t] = (t < j) ? W.X[(size_t)i * NB + t] : 0.f;
-        vi[t] = (t < j) ? Vel(A, N, p, t, i) : (t == j ? 1.f : 0.f);
+        ta_t = t < j ? S.crossa[t] + inv * s1 : 0.f; // V^T v
+        tb_t = t < j ? S.crossb[t] + inv * s2 : 0.f; // X^T v
+    }
+    // ---- column reflector of column i: V[row][j] for own rows, into A by the first tile column
+    if (rowv) {
+        float v = 0.f;
+        if (row > i) {
+            v = (tau != 0.f) ? R.c * inv : R.c;
+            if (cb == 0) stc(A + (size_t)row * N + i, v);
+        } else if (row == i) {
+            v = 1.f;
+        }
+        set_at(R.v, j, v);
+    }
+    if (rb == 0 && cb == 0 && tid == 0) {
+        stc(A + (size_t)i * N + i, beta);
+        P.d[i] = beta;
+        P.tauq[i] = tau;
+    }
+    if (i == N - 1) {
+        __syncthreads();
+        return;
+    }
+    __syncthreads(); // every thread has read S.ta[NB]
+    if (tid < NB) {
+        S.ta[tid] = ta_t;
+        S.tb[tid] = tb_t;
     }
     __syncthreads();
     // ---- y for own columns, row-i update
     double part = 0.0;
-    for (int col = max(c0, i + 1) + tid; col < c1; col += GBT) {
-        float g1 = 0.f;
-        for (int q = 0; q < GR; ++q) g1 += W.g1p[(size_t)q * N + col];
-        float acc = A[(size_t)i * N + col] + inv * g1;
-#pragma unroll 8
-        for (int t = 0; t < j; ++t) acc -= W.Y[(size_t)col * NB + t] * t1s[t] + Uel(A, N, p, t, col) * t2s[t];
+    if (clive) {
+        float acc = aic + inv * g1;
+        for (int t = 0; t < j; ++t) acc -= Yl[t * tw + tid] * S.ta[t] + Ul[t * tw + tid] * S.tb[t];
         const float y = tau * acc;
-        yl[col - c0] = y;
-        if (rb == 0) W.Y[(size_t)col * NB + j] = y;
-        float w = A[(size_t)i * N + col] - y * vi[j];
-#pragma unroll 8
-        for (int t = 0; t < j; ++t) w -= W.Y[(size_t)col * NB + t] * vi[t] + Uel(A, N, p, t, col) * xi[t];
-        wl[col - c0] = w;
-        if (rb == 0) W.wrow[col] = w;
-        if (col > i + 1) part += (double)w * w;
+        Yl[j * tw + tid] = y;
+        if (rb == 0) stc(W.Y + (size_t)col * NB + j, y);
+        float w = aic - y;
+        for (int t = 0; t < j; ++t) w -= Yl[t * tw + tid] * S.crossa[t] + Ul[t * tw + tid] * S.crossb[t];
+        wl[tid] = w;
+        R.w = w;
+        if (rb == 0) stc(W.wrow + col, w);
+        if (col > i + 1) part = (double)w * w;
+    } else if (colv) {
+        Yl[j * tw + tid] = 0.f;
+        wl[tid] = 0.f;
     }
-    const double ss = gb_block_sum(part, red);
-    if (rb == 0 && tid == 0) W.rsp[cb] = ss;
+    // one reduction for the sum of squares and this tile's share of t3p[cb][t] =
+    // sum_{col > i+1} Y[col][t] w (t <= j), t4p[cb][t] = sum U[t][col] w (t < j): the 2 NB
+    // entries are shared out over the GR tiles of the column block
+    const int K3 = (2 * NB + GR - 1) / GR, e03 = rb * K3, ke3 = max(0, min(K3, 2 * NB - e03));
+    {
+        const float ww = (colv && col > i + 1) ? R.w : 0.f;
+        const double ws = wave_sum(part);
+        if (lane == 0) S.red[wv] = ws;
+        for (int k = 0; k < ke3; ++k) {
+            const int e = e03 + k;
+            const float v = wave_sum_dpp(colv ? (e < NB ? Yl[e * tw + tid] : Ul[(e - NB) * tw + tid]) * ww : 0.f);
+            if (lane == 0) S.redk[wv][k] = v;
+        }
+    }
+    __syncthreads(); // (also publishes wl, Yl)
+    if (tid == 0) {
+        double ss = 0.0;
+#pragma unroll
+        for (int w = 0; w < GBT / 64; ++w) ss += S.red[w];
+        if (rb == 0) stc(W.rsp + cb, ss);
+    }
+    if (tid < ke3) {
+        float tot = 0.f;
+#pragma unroll
+        for (int q = 0; q < GBT / 64; ++q) tot += S.redk[q][tid];
+        const int e = e03 + tid;
+        if (e < NB)
+            stc(W.t3p + cb * NB + e, e <= j ? tot : 0.f);
+        else
+            stc(W.t4p + cb * NB + e - NB, e - NB < j ? tot : 0.f);
+    }
     // ---- GEMV2 partials: g2p[cb][row] = sum_{col > i+1} A[row][col] wl[col], row > i
     const int cs = max(c0, i + 2);
-    // eight rows per wavefront in flight; lanes over the tile's columns
-    for (int row = max(r0, i + 1) + 8 * wv; row < r1; row += 8 * (GBT / 64)) {
-        float acc[8];
+    const int rstart = max(r0, i + 1);
+    if ((N & 3) == 0 && cs < c1) {
+        // a wavefront takes 8 consecutive rows at a time, its lanes float4 column groups;
+        // the 8 row sums are reduced across the lanes by a transposing butterfly
+        const int ca = cs & ~3, ncg = (c1 - ca + 3) >> 2;
+        for (int rb0 = rstart + 8 * wv; rb0 < r1; rb0 += 8 * (GBT / 64)) {
+            float acc[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) acc[q] = 0.f;
-        for (int col = cs + lane; col < c1; col += 64) {
-            const float w = wl[col - c0];
-            float v[8];
+            for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+            for (int cg0 = 0; cg0 < ncg; cg0 += 64) {
+                const int cgi = cg0 + lane, cc = ca + 4 * min(cgi, ncg - 1);
+                float w4[4];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = (row + q < r1) ? A[(size_t)(row + q) * N + col] : 0.f;
+                for (int e = 0; e < 4; ++e) {
+                    const int col = cc + e;
+                    w4[e] = (cgi < ncg && col >= cs && col < c1) ? wl[col - c0] : 0.f;
+                }
+                float4 v[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) acc[q] += v[q] * w;
+                for (int k = 0; k < 8; ++k)
+                    v[k] = *reinterpret_cast<const float4 *>(A + (size_t)min(rb0 + k, r1 - 1) * N + cc);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) acc[k] += v[k].x * w4[0] + v[k].y * w4[1] + v[k].z * w4[2] + v[k].w * w4[3];
+            }
+            // lanes 8 g + k hold, after the butterfly over each group of 8 lanes, the group's
+            // sum for row k; three exchanges add the eight groups
+#pragma unroll
+            for (int l = 2; l >= 0; --l) {
+                const int sh = 1 << l;
+                const bool up = (lane & sh) != 0;
+#pragma unroll
+                for (int k = 0; k < sh; ++k) {
+                    const float lo = acc[k], hi = acc[k + sh];
+                    acc[k] = (up ? hi : lo) + __shfl_xor(up ? lo : hi, sh, 64);
+                }
+            }
+            float tot = acc[0];
+            tot += __shfl_xor(tot, 8, 64);
+            tot += __shfl_xor(tot, 16, 64);
+            tot += __shfl_xor(tot, 32, 64);
+            if (lane < 8 && rb0 + lane < r1) stc(W.g2p + (size_t)cb * M + rb0 + lane, tot);
         }
+    } else if (cs < c1) {
+        // eight rows per wavefront in flight; lanes over the tile's columns
+        for (int rr = rstart + 8 * wv; rr < r1; rr += 8 * (GBT / 64)) {
+            float acc[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) acc[q] = wave_sum(acc[q]);
-        if (lane < 8 && row + lane < r1) {
-            float a = acc[0];
+            for (int q = 0; q < 8; ++q) acc[q] = 0.f;
+            for (int cc = cs + lane; cc < c1; cc += 64) {
+                const float w = wl[cc - c0];
+                float v[8];
 #pragma unroll
-            for (int q = 1; q < 8; ++q)
-                if (lane == q) a = acc[q];
-            W.g2p[(size_t)cb * M + row + lane] = a;
-        }
-    }
-    // t3p[cb][t] = sum_{col > i+1} Y[col][t] w (t <= j), t4p[cb][t] = sum U[t][col] w (t < j);
-    // 32 lanes over t x 8 column groups, reduced through LDS
-    if (rb == 0) {
-        __shared__ float r3s[8][NB + 1], r4s[8][NB + 1];
-        const int t = tid % NB, g = tid / NB;
-        float s3 = 0.f, s4 = 0.f;
-#pragma unroll 4
-        for (int col = cs + g; col < c1; col += 8) {
-            const float w = wl[col - c0];
-            if (t < j) {
-                s3 += W.Y[(size_t)col * NB + t] * w;
-                s4 += Uel(A, N, p, t, col) * w;
-            } else if (t == j) {
-                s3 += yl[col - c0] * w;
+                for (int q = 0; q < 8; ++q) v[q] = (rr + q < r1) ? A[(size_t)(rr + q) * N + cc] : 0.f;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc[q] += v[q] * w;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[q] = wave_sum(acc[q]);
+            if (lane < 8 && rr + lane < r1) {
+                float av = acc[0];
+#pragma unroll
+                for (int q = 1; q < 8; ++q)
+                    if (lane == q) av = acc[q];
+                stc(W.g2p + (size_t)cb * M + rr + lane, av);
             }
         }
-        r3s[g][t] = s3;
-        r4s[g][t] = s4;
-        __syncthreads();
-        if (tid < NB) {
-            float a3 = 0.f, a4 = 0.f;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                a3 += r3s[q][tid];
-                a4 += r4s[q][tid];
+    } else {
+        for (int rr = rstart + tid; rr < r1; rr += GBT) stc(W.g2p + (size_t)cb * M + rr, 0.f);
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// grid synchronisation of the persistent panel launch
+// ---------------------------------------------------------------------------
+__device__ inline unsigned ld_acq(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned ld_rlx(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Marks the call as failed: later waits return at once, the problem's info becomes -2.
+__device__ inline void gb_abort(GridSync *gs, int *info)
+{
+    __hip_atomic_store(&gs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(info, -2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Registration: returns this workgroup's rank (or -1: registration already closed, the
+// workgroup leaves) and the number of participants in *parts.
+__device__ int gb_register(GridSync *gs, int *info, long long timeout, int *parts)
+{
+    __shared__ int s_rank, s_parts;
+    if (threadIdx.x == 0) {
+        int rank = -1, np = 0;
+        const unsigned old = __hip_atomic_fetch_add(&gs->reg, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (!(old & REG_CLOSED) && !ld_rlx(&gs->abort)) {
+            rank = (int)old;
+            const long long t0 = wall_clock64();
+            if (rank == 0) {
+                // close once every workgroup of the grid has joined, or after a short wait
+                // (about 20 us) when part of the chip is busy with other work
+                const unsigned G = gridDim.x;
+                while ((ld_rlx(&gs->reg) & ~REG_CLOSED) < G && wall_clock64() - t0 < timeout / 100000)
+                    __builtin_amdgcn_s_sleep(1);
+                const unsigned n =
+                    __hip_atomic_fetch_or(&gs->reg, REG_CLOSED, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) & ~REG_CLOSED;
+                np = (int)n;
+                __hip_atomic_store(&gs->nparts, n | NPARTS_READY, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                unsigned v;
+                while (!((v = ld_rlx(&gs->nparts)) & NPARTS_READY)) {
+                    if (wall_clock64() - t0 > timeout) {
+                        gb_abort(gs, info);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                np = (v & NPARTS_READY) ? (int)(v & ~NPARTS_READY) : 0;
+                // registered after the count was taken: not a participant
+                if (rank >= np) rank = -1;
             }
-            W.t3p[cb * NB + tid] = a3;
-            W.t4p[cb * NB + tid] = a4;
+        }
+        s_rank = rank;
+        s_parts = np;
+    }
+    __syncthreads();
+    *parts = s_parts;
+    return s_rank;
+}
+
+// Barrier b (0, 1, ... within the launch) over the np participants; false when the call
+// has been aborted.  Every participant stores b + 1 into its own flag (no contended
+// atomic); the workgroup of rank 0 polls all flags with one thread per flag, then
+// publishes the barrier count; the others wait for that count.  Data crossing the
+// barrier moves through device-coherent accesses, so only the stores' completion is
+// awaited before arriving.
+__device__ bool gb_barrier(GridSync *gs, unsigned *flags, int rank, unsigned np, unsigned b, int *info,
+                           long long timeout)
+{
+    __shared__ int s_ok, s_done;
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_ok = 1;
+        __hip_atomic_store(flags + (size_t)rank * FLAG_STRIDE, b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (rank == 0) {
+        const long long t0 = wall_clock64();
+        for (;;) {
+            if (threadIdx.x == 0) s_done = 1;
+            __syncthreads();
+            for (unsigned q = threadIdx.x; q < np; q += GBT)
+                if (ld_rlx(flags + (size_t)q * FLAG_STRIDE) != b + 1) s_done = 0;
+            __syncthreads();
+            if (s_done) break;
+            if (wall_clock64() - t0 > timeout || ld_rlx(&gs->abort)) {
+                if (threadIdx.x == 0) {
+                    gb_abort(gs, info);
+                    s_ok = 0;
+                }
+                break;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && s_ok)
+            __hip_atomic_store(&gs->gen, b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (threadIdx.x == 0) {
+        const long long t0 = wall_clock64();
+        while (ld_rlx(&gs->gen) != b + 1) {
+            if (ld_rlx(&gs->abort)) {
+                s_ok = 0;
+                break;
+            }
+            if (wall_clock64() - t0 > timeout) {
+                gb_abort(gs, info);
+                s_ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
         }
     }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// Resets the synchronisation words and the arrival flags before a panel launch (first:
+// also the abort flag, at the start of the call).  A fill kernel, not a memset, so that
+// the launch sequence can be captured in a graph.
+__global__ void gb_sync_reset_kernel(GridSync *gs, unsigned *flags, int first)
+{
+    for (int q = threadIdx.x; q < MAX_PARTS; q += blockDim.x) flags[(size_t)q * FLAG_STRIDE] = 0;
+    if (threadIdx.x == 0) {
+        gs->reg = 0;
+        gs->nparts = 0;
+        gs->count = 0;
+        gs->gen = 0;
+        if (first) gs->abort = 0;
+    }
+}
+
+
+// One panel: both phases of its NB columns, then the finish of the last column.  With as
+// many participants as tiles, each participant keeps its tile's panel state (registers,
+// LDS) for the whole launch.
+__global__ __launch_bounds__(GBT) void gb_panel_kernel(PanelArgs a)
+{
+    extern __shared__ float dyn[];
+    __shared__ PhaseLds S;
+    GridSync *gs = a.W.sync;
+    int np;
+    const int rank = gb_register(gs, a.P.info, a.timeout, &np);
+    if (rank < 0) return;
+    const int ntiles = a.gr * a.gc;
+    const bool resident = np == ntiles;
+    RowState R;
+    row_state_zero(R);
+    for (size_t q = threadIdx.x; q < 2 * (size_t)NB * a.tw; q += GBT) dyn[q] = 0.f;
+    __syncthreads();
+    unsigned b = 0;
+    for (int j = 0; j < NB; ++j) {
+        for (int t = rank; t < ntiles; t += np) gb_col_phase(a, j, t / a.gc, t % a.gc, dyn, S, R, resident);
+        if (!gb_barrier(gs, a.W.flags, rank, (unsigned)np, b++, a.P.info, a.timeout)) return;
+        for (int t = rank; t < ntiles; t += np) gb_row_phase(a, j, t / a.gc, t % a.gc, dyn, S, R, resident);
+        if (!gb_barrier(gs, a.W.flags, rank, (unsigned)np, b++, a.P.info, a.timeout)) return;
+    }
+    for (int t = rank; t < ntiles; t += np) gb_col_phase(a, NB, t / a.gc, t % a.gc, dyn, S, R, resident);
+}
+
+// The same phases as one launch each (two launches per column), one tile per workgroup;
+// PACO_GRID_PERSIST=0 selects it (comparisons of the two schedules).
+__global__ __launch_bounds__(GBT) void gb_col_kernel(PanelArgs a, int j)
+{
+    extern __shared__ float dyn[];
+    __shared__ PhaseLds S;
+    if (a.W.sync->abort) return;
+    RowState R;
+    row_state_zero(R);
+    gb_col_phase(a, j, blockIdx.x / a.gc, blockIdx.x % a.gc, dyn, S, R, false);
+}
+__global__ __launch_bounds__(GBT) void gb_row_kernel(PanelArgs a, int j)
+{
+    extern __shared__ float dyn[];
+    __shared__ PhaseLds S;
+    if (a.W.sync->abort) return;
+    RowState R;
+    row_state_zero(R);
+    gb_row_phase(a, j, blockIdx.x / a.gc, blockIdx.x % a.gc, dyn, S, R, false);
 }
 
 __global__ void gb_restore_kernel(float *A, int N, int q, const float *e)
@@ -454,12 +861,20 @@ static void launch_trail(const ProbDesc &P, const GridWs &W, int q, int p, hipSt
     hipLaunchKernelGGL(gb_trail_kernel<MODE>, dim3(tiles), dim3(GT), 0, s, t, tn);
 }
 
-// first column of the one-workgroup finish: panels of NB columns while more than SWITCH
-// columns remain
+// columns left to the one-workgroup kernel: PACO_GRID_SWITCH overrides (NB .. 512)
+static int grid_switch()
+{
+    const char *e = getenv("PACO_GRID_SWITCH");
+    const int v = e ? atoi(e) : SWITCH_DEFAULT;
+    return v < NB ? NB : v > 512 ? 512 : v;
+}
+
+// first column of the one-workgroup finish: panels of NB columns while more than the
+// switch count of columns remain
 static int grid_tail_start(int N)
 {
     int p = 0;
-    while (N - p > SWITCH) p += NB;
+    while (N - p > grid_switch()) p += NB;
     return p;
 }
 
@@ -474,13 +889,34 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s)
     const int M = P.M, N = P.N;
     GridWs W;
     carve(M, N, static_cast<char *>(ws), &W);
-    // tile grid: PACO_GRID overrides (8 .. GMAX)
+    const int p_tail = grid_tail_start(N);
+    if (bidiag_wg_lds_bytes(M - p_tail, N - p_tail, 1024) > bidiag_wg_lds_limit()) return hipErrorInvalidValue;
+    // tile grid: PACO_GRID overrides (4 .. GMAX); PACO_GRID_PERSIST=0: two launches per column
     const char *genv = getenv("PACO_GRID");
     int G = genv ? atoi(genv) : 16;
-    G = G < 8 ? 8 : G > GMAX ? GMAX : G;
-    const int GR = G, GC = G;
+    G = G < 4 ? 4 : G > GMAX ? GMAX : G;
+    const char *penv = getenv("PACO_GRID_PERSIST");
+    const bool persist = !(penv && penv[0] == '0');
+    static int cus = -1;
+    static long long timeout = 0;
+    static size_t maxdyn = 0;
+    if (cus < 0) {
+        int dev = 0, khz = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            khz = 100000;
+        timeout = 2000LL * khz; // 2 s per barrier wait
+        maxdyn = allow_dynamic_lds((const void *)gb_panel_kernel);
+        maxdyn = std::min(maxdyn, allow_dynamic_lds((const void *)gb_col_kernel));
+        maxdyn = std::min(maxdyn, allow_dynamic_lds((const void *)gb_row_kernel));
+    }
+    // at most GBT rows and TW_MAX columns per tile (one row / column per thread)
+    const int GR = std::min(GMAX, std::max(G, (M + GBT - 1) / GBT));
+    const int GC = std::min(GMAX, std::max(G, (N + TW_MAX - 1) / TW_MAX));
+    hipLaunchKernelGGL(gb_sync_reset_kernel, dim3(1), dim3(256), 0, s, W.sync, W.flags, 1);
     int p = 0;
-    while (N - p > SWITCH) {
+    while (p < p_tail) {
         PanelArgs a;
         a.P = P;
         a.W = W;
@@ -489,23 +925,29 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s)
         a.gc = GC;
         a.th = (M - p + GR - 1) / GR;
         a.tw = (N - p + GC - 1) / GC;
-        const size_t lds_col = sizeof(float) * 2 * (size_t)a.th;
-        const size_t lds_row = sizeof(float) * 2 * (size_t)a.tw;
-        static size_t maxc = allow_dynamic_lds((const void *)gb_col_kernel);
-        static size_t maxr = allow_dynamic_lds((const void *)gb_row_kernel);
-        if (lds_col > maxc || lds_row > maxr) return hipErrorInvalidValue;
-        for (int j = 0; j < NB; ++j) {
-            a.j = j;
-            a.finish_only = 0;
-            hipLaunchKernelGGL(gb_col_kernel, dim3(GR * GC), dim3(GBT), lds_col, s, a);
-            hipLaunchKernelGGL(gb_row_kernel, dim3(GR * GC), dim3(GBT), lds_row, s, a);
+        a.timeout = timeout;
+        if (a.th > GBT || a.tw > TW_MAX) return hipErrorInvalidValue;
+        const size_t lds = sizeof(float) * panel_dyn_floats_host(a.th, a.tw);
+        if (lds > maxdyn) return hipErrorInvalidValue;
+        if (persist) {
+            if (p > 0) hipLaunchKernelGGL(gb_sync_reset_kernel, dim3(1), dim3(256), 0, s, W.sync, W.flags, 0);
+            // PACO_GRID_PARTS caps the workgroups (tests of the several-tiles-per-workgroup path)
+            const char *penv2 = getenv("PACO_GRID_PARTS");
+            const int cap = penv2 ? std::max(1, atoi(penv2)) : MAX_PARTS;
+            const int grid = std::min(std::min(std::min(GR * GC, std::max(cus, 1)), MAX_PARTS), cap);
+            hipLaunchKernelGGL(gb_panel_kernel, dim3(grid), dim3(GBT), lds, s, a);
+        } else {
+            for (int j = 0; j < NB; ++j) {
+                hipLaunchKernelGGL(gb_col_kernel, dim3(GR * GC), dim3(GBT), lds, s, a, j);
+                hipLaunchKernelGGL(gb_row_kernel, dim3(GR * GC), dim3(GBT), lds, s, a, j);
+            }
+            hipLaunchKernelGGL(gb_col_kernel, dim3(GR * GC), dim3(GBT), lds, s, a, NB);
         }
-        a.j = NB;
-        a.finish_only = 1;
-        hipLaunchKernelGGL(gb_col_kernel, dim3(GR * GC), dim3(GBT), lds_col, s, a);
         const int mode = gemm_get_mode();
         if (mode == 2)
             launch_trail<2>(P, W, p + NB, p, s);
+        else if (mode == 1)
+            launch_trail<1>(P, W, p + NB, p, s);
         else
             launch_trail<0>(P, W, p + NB, p, s);
         hipLaunchKernelGGL(gb_restore_kernel, dim3(1), dim3(64), 0, s, P.a, N, p + NB, (const float *)P.e);
@@ -526,7 +968,6 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s)
     pl.single.e = P.e + p;
     pl.single.tauq = P.tauq + p;
     pl.single.taup = P.taup + p;
-    if (bidiag_wg_lds_bytes(M - p, N - p, 1024) > bidiag_wg_lds_limit()) return hipErrorInvalidValue;
     return launch_bidiag_wg(pl, nullptr, 1, bidiag_wg_lds_limit(), s, 1024);
 }
 

@@ -460,3 +460,50 @@ def test_server_alone_vs_mix_and_admission(torch_dev):
                 assert torch.equal(x, y), i
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("shape", [(1100, 1000), (2048, 700), (1500, 1500)])
+def test_grid_persistent_matches_launch_per_phase(torch_dev, shape):
+    """The persistent panel launch (grid barriers) and the launch-per-phase schedule run the
+    same tile code with the same reduction order: entry point (a) gives bitwise equal
+    outputs; a smaller switch to the one-workgroup finish stays within the parity bars."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(shape[0] + 7 * shape[1])
+    A = rng.standard_normal(shape).astype(np.float32)
+    outs = {}
+    for persist in ("1", "0"):
+        say("grid", shape, "persistent", persist)
+        os.environ["PACO_GRID_PERSIST"] = persist
+        try:
+            r = pacosvd.upper_bidiagonalization(torch.from_numpy(A.copy()).cuda())
+            torch.cuda.synchronize()
+        finally:
+            del os.environ["PACO_GRID_PERSIST"]
+        assert pacosvd.entry_info(r["ws"], *shape) == 0
+        outs[persist] = r
+    for k in ("a", "u", "vt", "d", "e", "tauq", "taup"):
+        assert torch.equal(outs["1"][k], outs["0"][k]), k
+    # several tiles per workgroup (fewer participants than tiles), with a workspace full of
+    # NaN bytes: only entries the panel defined may be read back
+    say("grid", shape, "7 participants, NaN-filled workspace")
+    ws = pacosvd.workspace(*shape)
+    ws.fill_(255)
+    os.environ["PACO_GRID_PARTS"] = "7"
+    try:
+        r = pacosvd.upper_bidiagonalization(torch.from_numpy(A.copy()).cuda(), ws=ws)
+        torch.cuda.synchronize()
+    finally:
+        del os.environ["PACO_GRID_PARTS"]
+    for k in ("a", "u", "vt", "d", "e", "tauq", "taup"):
+        assert torch.equal(outs["1"][k], r[k]), k
+    say("grid", shape, "switch 64")
+    os.environ["PACO_GRID_SWITCH"] = "64"
+    try:
+        _, s, U, Vt = run_entry_points(torch, A)
+    finally:
+        del os.environ["PACO_GRID_SWITCH"]
+    m = measures(A, s, U, Vt)
+    say("measures", m)
+    assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL
+    assert m["resid_ratio"] <= RESID_TOL
