@@ -47,6 +47,7 @@ extern "C" {
 #define PACO_EINVAL (-1)   /* bad argument (M < N, N < 1, null pointer, size over the limits) */
 #define PACO_ENOMEM (-2)   /* an internal buffer could not be allocated */
 #define PACO_EINJECTED (-3) /* serving loop: failure injected by PACO_SERVE_FAIL_EVERY (tests) */
+#define PACO_ETOOBIG (-4)   /* paco_svd_plain_noalloc: a job's scratch exceeds the fixed array */
 #define PACO_MAX_N 7680    /* D&C deflation stage keeps one merge in LDS (20 bytes per row) */
 
 /* Entry point (a): Golub-Kahan upper bidiagonalization, A = Q * B * P^T.
@@ -102,6 +103,20 @@ typedef struct {
 /* Plain path: every job runs through the two entry points, one matrix per launch
  * sequence (the baseline the batched path is measured against). */
 int paco_svd_plain(const paco_svd_job *jobs, int count, void *stream);
+
+/* Allocation-free variant of paco_svd_plain.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, hosts that forbid hipMalloc).  The jobs' scratch and info words live in a fixed
+ * device array of 64 MiB that the library holds on every GPU it runs on until the
+ * process ends; the call allocates nothing (no hipMalloc / hipFree).  Limits: at most 4096
+ * jobs per call, and each job's scratch (its a, e, tau, D&C bases and the entry points'
+ * workspace) within 64 MiB - 16 KiB: e.g. 1400 x 1400 needs 58.6 MiB, 2048 x 1024 40.1 MiB,
+ * 4096 x 256 19.9 MiB (square matrices up to about 1450 x 1450); larger jobs return
+ * PACO_ETOOBIG before anything is launched.  Every
+ * call on one GPU shares that GPU's array, which keeps the last call's contents: calls
+ * that overlap on one GPU (several streams or threads) race.  Results are bitwise those of
+ * paco_svd_plain; paco_last_info reports the jobs' info words. */
+int paco_svd_plain_noalloc(const paco_svd_job *jobs, int count, void *stream);
 
 /* Batched path: all jobs in one sequence of launches, each phase one launch over
  * every matrix (tile shapes per size bucket from the autotune table).  jobs is a host

@@ -1015,17 +1015,12 @@ int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s)
 
 // Runs jobs through the two entry points, one after the other on stream s (the plain
 // path); job i reports its info word into info[idx ? idx[i] : i].
-static int run_plain(const paco_svd_job *jobs, int count, hipStream_t s, int *info, const int *idx)
+static size_t plain_job_bytes(int M, int N) { return scratch_layout(M, N).total + ws_layout(M, N).total; }
+
+// The plain sequence of jobs on stream s with one job's scratch at buf (reused by every
+// job, in stream order).
+static int run_plain_in(const paco_svd_job *jobs, int count, hipStream_t s, int *info, const int *idx, char *buf)
 {
-    size_t need = 0;
-    for (int i = 0; i < count; ++i)
-        need = std::max(need, scratch_layout(jobs[i].M, jobs[i].N).total + ws_layout(jobs[i].M, jobs[i].N).total);
-    char *buf;
-    {
-        std::lock_guard<std::mutex> lk(g_mu);
-        buf = static_cast<char *>(stream_res(s)->plain.get(need));
-    }
-    if (!buf) return PACO_ENOMEM;
     for (int i = 0; i < count; ++i) {
         const int M = jobs[i].M, N = jobs[i].N;
         const BatchScratch L = scratch_layout(M, N);
@@ -1046,6 +1041,19 @@ static int run_plain(const paco_svd_job *jobs, int count, hipStream_t s, int *in
     return (int)hipGetLastError();
 }
 
+static int run_plain(const paco_svd_job *jobs, int count, hipStream_t s, int *info, const int *idx)
+{
+    size_t need = 0;
+    for (int i = 0; i < count; ++i) need = std::max(need, plain_job_bytes(jobs[i].M, jobs[i].N));
+    char *buf;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        buf = static_cast<char *>(stream_res(s)->plain.get(need));
+    }
+    if (!buf) return PACO_ENOMEM;
+    return run_plain_in(jobs, count, s, info, idx, buf);
+}
+
 // A zeroed per-job info array of the stream (count entries).
 static int *stream_info(hipStream_t s, int count)
 {
@@ -1056,6 +1064,60 @@ static int *stream_info(hipStream_t s, int count)
     }
     if (info) hipLaunchKernelGGL(zero_ints_kernel, dim3(cdiv(count, 256)), dim3(256), 0, s, info, count);
     return info;
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free plain path (paco_svd_plain_noalloc): the scratch of the jobs and their
+// info words live in a fixed device array of the library, so a call allocates nothing.
+// Every call on one device shares that device's array: calls that overlap on one device
+// race.  Layout: [info words of up to NOALLOC_MAX_JOBS jobs | one job's scratch].
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int NOALLOC_MAX_JOBS = 4096;
+constexpr size_t NOALLOC_INFO_BYTES = sizeof(int) * NOALLOC_MAX_JOBS;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static char *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of the current device's scratch array: looked up once per device (the
+// array's name is a host-side stand-in, never a device address).
+static int noalloc_scratch(char **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return PACO_EINVAL;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (!p) return PACO_ENOMEM;
+        g_noalloc_ptr[dev] = static_cast<char *>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return 0;
+}
+
+static int svd_plain_noalloc(const paco_svd_job *jobs, int count, hipStream_t s)
+{
+    if (count > NOALLOC_MAX_JOBS) return PACO_ETOOBIG;
+    for (int i = 0; i < count; ++i) {
+        if (!job_ok(jobs[i])) return PACO_EINVAL;
+        if (plain_job_bytes(jobs[i].M, jobs[i].N) > NOALLOC_SCRATCH_BYTES - NOALLOC_INFO_BYTES) return PACO_ETOOBIG;
+    }
+    char *base = nullptr;
+    const int rc0 = noalloc_scratch(&base);
+    if (rc0) return rc0;
+    int *info = reinterpret_cast<int *>(base);
+    // the array keeps the previous call's contents: the info words start from zero (a
+    // fill kernel, so the call can be captured); the rest is written before it is read
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(cdiv(count, 256)), dim3(256), 0, s, info, count);
+    const int rc = run_plain_in(jobs, count, s, info, nullptr, base + NOALLOC_INFO_BYTES);
+    if (rc) return rc;
+    set_last_info(info, count);
+    return 0;
 }
 
 } // namespace paco
@@ -1093,6 +1155,12 @@ int paco_set_precision(int mode)
     const int old = gemm_get_mode();
     if (mode >= 0 && mode <= 2) gemm_set_mode(mode);
     return old;
+}
+
+int paco_svd_plain_noalloc(const paco_svd_job *jobs, int count, void *stream)
+{
+    if (count <= 0) return 0;
+    return svd_plain_noalloc(jobs, count, static_cast<hipStream_t>(stream));
 }
 
 int paco_svd_plain(const paco_svd_job *jobs, int count, void *stream)
