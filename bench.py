@@ -29,8 +29,8 @@ HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFS = 157.3
 
 
-def make_workload(torch, count=10000, seed=3):
-    host, offs, Ms, Ns = workloads.c4_host(count, seed)
+def make_workload(torch, count=10000, seed=3, host_data=None):
+    host, offs, Ms, Ns = host_data if host_data is not None else workloads.c4_host(count, seed)
     dev = torch.from_numpy(host).cuda()
     As = [dev[offs[i]:offs[i + 1]].view(int(Ms[i]), int(Ns[i])) for i in range(count)]
     return host, offs, Ms, Ns, As
@@ -72,8 +72,10 @@ def _pool_dgesdd(A):
 
 
 def _pool_init():
+    from scipy.linalg import lapack
     from threadpoolctl import threadpool_limits
     threadpool_limits(1)
+    lapack.dgesdd(np.eye(4), full_matrices=1)  # imports and first-call set-up outside the timing
 
 
 def cpu_baseline(host, offs, Ms, Ns, budget_s=12.0):
@@ -93,6 +95,7 @@ def cpu_baseline(host, offs, Ms, Ns, budget_s=12.0):
         oracle.svd(A)
         done += 1
     dt = time.perf_counter() - t0
+    print("[bench] cpu: oracle %d matrices in %.1f s" % (done, dt), flush=True)
     sample = [host[offs[i]:offs[i + 1]].reshape(int(Ms[i]), int(Ns[i])) for i in range(done)]
     cores = min(len(os.sched_getaffinity(0)), 16)
 
@@ -105,13 +108,16 @@ def cpu_baseline(host, offs, Ms, Ns, budget_s=12.0):
     with threadpool_limits(cores):
         d_all = timed(lambda A: lapack.dgesdd(A.astype(np.float64), full_matrices=1))
         s_all = timed(lambda A: lapack.sgesdd(A, full_matrices=1))
+    print("[bench] cpu: LAPACK on %d cores done" % cores, flush=True)
     with threadpool_limits(1):
         d_one = timed(lambda A: lapack.dgesdd(A.astype(np.float64), full_matrices=1))
+    print("[bench] cpu: LAPACK on 1 thread done", flush=True)
     ctx = mp.get_context("spawn")
     with ctx.Pool(cores, initializer=_pool_init) as pool:
-        pool.map(_pool_dgesdd, [sample[0].astype(np.float64)] * cores)  # start-up
+        pool.map(_pool_dgesdd, [sample[0].astype(np.float64)] * cores, chunksize=1)  # start-up
+        work = [A.astype(np.float64) for A in sample]
         t = time.perf_counter()
-        pool.map(_pool_dgesdd, [A.astype(np.float64) for A in sample], chunksize=4)
+        pool.map(_pool_dgesdd, work, chunksize=1)
         d_pool = done / (time.perf_counter() - t)
     return dict(value=done / dt, unit="SVDs/s", cores=1, kind="port",
                 sample="first %d matrices of the batched workload (oracle FP64 bidiag + D&C, 1 thread, %.1f s)"
@@ -149,9 +155,7 @@ def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0):
     As = workloads.c5_matrices(torch, reqs)
     outs = pacosvd.alloc_outputs(As)
     torch.cuda.synchronize()
-    ex = ThreadPoolExecutor(2)
-    refs = [ex.submit(workloads.lapack_sigma, A.cpu().numpy()) for A in As]
-    srv = pacosvd.Server(int(budget_gb * (1 << 30)), max_batch=512, batch_lanes=2, wide_lanes=2,
+    srv = pacosvd.Server(int(budget_gb * (1 << 30)), max_batch=512, batch_lanes=2, wide_lanes=1,
                          max_M=4096, max_N=4096)
     try:
         print("[bench] serving: submitting", flush=True)
@@ -179,12 +183,15 @@ def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0):
     t_plain = time.perf_counter() - t0
     del plain_outs
     print("[bench] serving: plain path %.3f s for the same %d requests" % (t_plain, count), flush=True)
+    # accuracy of every request: LAPACK sigma on the host (threads), the rest on the GPU
+    print("[bench] serving: LAPACK references", flush=True)
+    with ThreadPoolExecutor(4) as ex:
+        refs = list(ex.map(workloads.lapack_sigma, [A.cpu().numpy() for A in As]))
     worst = {}
     for i in range(count):
-        m = measures_gpu(torch, As[i], *outs[i], refs[i].result())
+        m = measures_gpu(torch, As[i], *outs[i], refs[i])
         for k, v in m.items():
             worst[k] = max(worst.get(k, 0.0), v)
-    ex.shutdown()
     worst["matrices_checked"] = count
     flops = sum(svd_flops(r["M"], r["N"]) for r in reqs)
     served = st.get("svds_per_s", 0.0)
@@ -302,11 +309,18 @@ def main():
               "collective (see DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    print("[bench] generating workload", flush=True)
+    host_data = workloads.c4_host(args.count)
+    # the CPU baseline runs before anything touches the GPU (its process pool starts new
+    # interpreters, which a GPU-initialised process must not do)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(*host_data, args.cpu_budget)
+        print("[bench] cpu %s" % cpu, flush=True)
     import torch
     import pacosvd
     pacosvd.lib()
-    print("[bench] generating workload", flush=True)
-    host, offs, Ms, Ns, As = make_workload(torch, args.count)
+    host, offs, Ms, Ns, As = make_workload(torch, args.count, host_data=host_data)
     outs = alloc_outputs(torch, Ms, Ns)
     torch.cuda.synchronize()
 
@@ -346,6 +360,22 @@ def main():
 
     acc = accuracy_sample(torch, host, offs, Ms, Ns, outs)
     print("[bench] accuracy %s" % acc, flush=True)
+
+    # ---- singular values only (jobz = 'N') on the same batch, replayed from a hipGraph
+    S_vals = [o[0].clone() for o in outs]
+    pacosvd.svd_values(As, S=S_vals, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pacosvd.svd_values(As, S=S_vals, graph=True)
+    torch.cuda.synchronize()
+    dt_v = time.perf_counter() - t0
+    vidx = np.linspace(0, len(Ms) - 1, 40).astype(int)
+    verr = max(float(np.max(np.abs(S_vals[i].cpu().numpy() - workloads.lapack_sigma(
+        host[offs[i]:offs[i + 1]].reshape(int(Ms[i]), int(Ns[i]))))) / max(float(outs[i][0][0]), 1e-30)) for i in vidx)
+    values_only = dict(svds_per_s=round(len(As) * args.steps / dt_v, 1), ms_per_step=round(1000 * dt_v / args.steps, 3),
+                       sigma_err_vs_fp64_lapack=verr, matrices_checked=len(vidx))
+    print("[bench] values only %s" % values_only, flush=True)
 
     # ---- plain path on the same stream (first matrices, bounded time)
     plain_outs = alloc_outputs(torch, Ms[:1], Ns[:1])
@@ -395,11 +425,6 @@ def main():
                                "calibrated with the copy kernel)" if traffic else None)
     flops = sum(svd_flops(int(m), int(n)) for m, n in zip(Ms, Ns))
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(host, offs, Ms, Ns, args.cpu_budget)
-        print("[bench] cpu %s" % cpu, flush=True)
-
     best = max(svds_s, svds_s_graph)
     line = {
         "metric": "SVDs/sec + p50/p99 latency, served/batched vs plain, at sigma rel err <=1e-5",
@@ -426,6 +451,7 @@ def main():
                        {"p50": round(ms_step, 2), "p99": round(ms_step, 2),
                         "source": "batched step: every SVD completes with the step"}),
         "serving": serving,
+        "values_only": values_only,
         "plain_4096x4096": plain_large,
         "precision_8192x4096": precision,
         "phase_ms": {k: round(v, 3) for k, v in ph.items()},

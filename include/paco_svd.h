@@ -127,6 +127,14 @@ int paco_svd_batched(const paco_svd_job *jobs, int count, void *stream);
  * of a batch signature (shapes and pointers) and replayed afterwards. */
 int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream);
 
+/* Singular values only (LAPACK jobz = 'N'), batched: S of every job, U and Vt ignored
+ * (may be NULL).  Q and P^T are not formed, the divide and conquer keeps no left vectors
+ * and only the first and last columns of each node's right vectors (all a merge reads),
+ * and there is no back-transform.  graph != 0 records / replays a hipGraph per batch
+ * signature as paco_svd_batched_graph does.  No counterpart in the reference (svd_DC always
+ * forms the vectors, svd.cpp:1911-1981). */
+int paco_svd_values_batched(const paco_svd_job *jobs, int count, int graph, void *stream);
+
 /* Device bytes the batched path needs for these shapes (scratch + plan). */
 size_t paco_batch_workspace_size(const int *M, const int *N, int count);
 

@@ -201,6 +201,44 @@ __device__ __attribute__((always_inline)) inline float get_at(const float (&a)[N
     return v;
 }
 
+// Reload of the panel state of one row (V[row][t], X[row][t], t < nv / t < nx) and one
+// column (Y[col][t], U[t][col], t < ny / t < nu) from the global copies: every load is
+// issued unconditionally from a valid address and the entries the panel has not defined
+// are selected away (V is 1 / 0 at and above its head, X exists below row p + t, Y right
+// of column p + t, U is 1 / 0 at and left of its head), so all loads are in flight at once.
+__device__ __attribute__((always_inline)) inline void reload_row(const float *A, const GridWs &W, int N, int p, int row,
+                                                                int nv, int nx, RowState &R)
+{
+    float v[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) v[t] = ldc(A + (size_t)row * N + p + t);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+        const int c = p + t;
+        R.v[t] = t < nv ? (row > c ? v[t] : (row == c ? 1.f : 0.f)) : 0.f;
+    }
+#pragma unroll
+    for (int t = 0; t < NB; ++t) v[t] = ldc(W.X + (size_t)row * NB + t);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) R.x[t] = (t < nx && row > p + t) ? v[t] : 0.f;
+}
+__device__ __attribute__((always_inline)) inline void reload_col(const float *A, const GridWs &W, int N, int p, int col,
+                                                                int ny, int nu, float *Yc, float *Uc, int tw)
+{
+    float y[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) y[t] = ldc(W.Y + (size_t)col * NB + t);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) Yc[t * tw] = (t < ny && col > p + t) ? y[t] : 0.f;
+#pragma unroll
+    for (int t = 0; t < NB; ++t) y[t] = ldc(A + (size_t)(p + t) * N + col);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+        const int h = p + t + 1; // head of row reflector t
+        Uc[t * tw] = t < nu ? (col > h ? y[t] : (col == h ? 1.f : 0.f)) : 0.f;
+    }
+}
+
 // Column phase of tile (rb, cb) for panel column j (j == NB: only the finish of x of
 // column NB - 1 and the row reflector of row p + NB - 1).
 __device__ __attribute__((always_inline)) inline void gb_col_phase(const PanelArgs &a, int j, int rb, int cb, float *dyn, PhaseLds &S, RowState &R,
@@ -245,19 +283,10 @@ __device__ __attribute__((always_inline)) inline void gb_col_phase(const PanelAr
         else ari = A[(size_t)row * N + ip + 1];
     }
     if (!resident) {
-        if (rowv)
-#pragma unroll
-            for (int t = 0; t < NB; ++t) {
-                // X[row][t] exists for rows below p + t, Y[col][t] right of p + t (the
-                // workspace holds stale values elsewhere): the same zeros as the resident state
-                R.v[t] = t < j ? Vel(A, N, p, t, row) : 0.f;
-                R.x[t] = (t < jp && row > p + t) ? ldc(W.X + (size_t)row * NB + t) : 0.f;
-            }
+        // (the workspace holds stale values where the panel has not written: selected away)
+        if (rowv) reload_row(A, W, N, p, row, j, jp, R);
         if (colv) {
-            for (int t = 0; t < NB; ++t) {
-                Yl[t * tw + tid] = (t < j && col > p + t) ? ldc(W.Y + (size_t)col * NB + t) : 0.f;
-                Ul[t * tw + tid] = t < jp ? Uel(A, N, p, t, col) : 0.f;
-            }
+            reload_col(A, W, N, p, col, j, jp, Yl + tid, Ul + tid, tw);
             R.w = (j > 0 && col > ip) ? ldc(W.wrow + col) : 0.f;
         }
     }
@@ -452,19 +481,10 @@ __device__ __attribute__((always_inline)) inline void gb_row_phase(const PanelAr
     }
     if (!resident) {
         if (rowv) {
-#pragma unroll
-            for (int t = 0; t < NB; ++t) {
-                R.v[t] = t < j ? Vel(A, N, p, t, row) : 0.f;
-                R.x[t] = (t < j && row > p + t) ? ldc(W.X + (size_t)row * NB + t) : 0.f;
-            }
+            reload_row(A, W, N, p, row, j, j, R);
             R.c = row >= i ? ldc(W.wcol + row) : 0.f;
         }
-        if (colv) {
-            for (int t = 0; t < NB; ++t) {
-                Yl[t * tw + tid] = (t < j && col > p + t) ? ldc(W.Y + (size_t)col * NB + t) : 0.f;
-                Ul[t * tw + tid] = t < j ? Uel(A, N, p, t, col) : 0.f;
-            }
-        }
+        if (colv) reload_col(A, W, N, p, col, j, j, Yl + tid, Ul + tid, tw);
     }
     if (tid == GBT - 1) S.ta[NB] = x1;
     __syncthreads();

@@ -144,17 +144,20 @@ __device__ inline MergeCtx merge_ctx(const ProbList &pl, const LevelArgs &la, in
 __host__ __device__ inline int strips_per_task(const LevelArgs &la) { return (la.stride + DT - 1) / DT; }
 
 // Column-parallel merge kernels (gather, finalize): a merge takes TPM = the power of two
-// >= stride threads when stride <= DT (DT / TPM merges per workgroup), else strips of DT
-// columns.  Thread -> (merge t, first column x, column step).
+// >= its column count threads when that is <= DT (DT / TPM merges per workgroup), else
+// strips of DT columns.  Thread -> (merge t, first column x, column step).  With singular
+// values only, a merge has two columns: the first and the last of its right vectors.
+__host__ __device__ inline int col_count(const LevelArgs &la) { return la.vonly ? 2 : la.stride; }
 __host__ __device__ inline int merge_tpm(const LevelArgs &la)
 {
     int t = 1;
-    while (t < la.stride) t <<= 1;
+    while (t < col_count(la)) t <<= 1;
     return t < DT ? t : DT;
 }
+__host__ __device__ inline int strips_of(const LevelArgs &la) { return (col_count(la) + DT - 1) / DT; }
 __host__ inline int column_blocks(const LevelArgs &la)
 {
-    return la.stride <= DT ? (la.ntask + DT / merge_tpm(la) - 1) / (DT / merge_tpm(la)) : la.ntask * strips_per_task(la);
+    return col_count(la) <= DT ? (la.ntask + DT / merge_tpm(la) - 1) / (DT / merge_tpm(la)) : la.ntask * strips_of(la);
 }
 struct ColumnSlot {
     int t, x, step, first; // first: this slot's threads hold columns 0..
@@ -162,14 +165,14 @@ struct ColumnSlot {
 __device__ inline ColumnSlot column_slot(const LevelArgs &la)
 {
     ColumnSlot c;
-    if (la.stride <= DT) {
+    if (col_count(la) <= DT) {
         const int tpm = merge_tpm(la);
         c.t = blockIdx.x * (DT / tpm) + (int)threadIdx.x / tpm;
         c.x = threadIdx.x % tpm;
         c.step = tpm;
         c.first = 1;
     } else {
-        const int spt = strips_per_task(la), strip = blockIdx.x % spt;
+        const int spt = strips_of(la), strip = blockIdx.x % spt;
         c.t = blockIdx.x / spt;
         c.x = strip * DT + threadIdx.x;
         c.step = DT;
@@ -354,8 +357,10 @@ __global__ __launch_bounds__(DT) void dc_gather_kernel(ProbList pl, LevelArgs la
     const MergeState S = *T.S;
     const int N = P.N, c0 = T.c0, c1 = T.c1;
     const int n = S.n, m = S.m, n1 = S.n1, m1 = S.m1, n2 = S.n2, m2 = S.m2;
-    const int x = cs.x;
+    // singular values only: slot 0 takes the first column, slot 1 the last
+    const int x = la.vonly ? (cs.x == 0 ? 0 : (cs.x == 1 && m > 1 ? m - 1 : m)) : cs.x;
     if (x >= m) return;
+    const bool vonly = la.vonly != 0;
     const Arena Ar = make_arena(P.idx, c0, n);
     const float *qt = P.qt, *wt = P.wt;
     float *st = P.qt + (size_t)N * N, *gt = P.wt + (size_t)N * N;
@@ -370,7 +375,7 @@ __global__ __launch_bounds__(DT) void dc_gather_kernel(ProbList pl, LevelArgs la
         const int q = -1 - s;
         const bool qok = !pole && (left ? x < n1 : (x > n1 && x < n));
         const bool wok = !pole && (left ? x < m1 : x >= m1);
-        qv = qok ? qt[left ? (size_t)(c0 + s) * N + c0 + x : (size_t)(c1 + q) * N + c1 + (x - n1 - 1)] : 0.f;
+        qv = (qok && !vonly) ? qt[left ? (size_t)(c0 + s) * N + c0 + x : (size_t)(c1 + q) * N + c1 + (x - n1 - 1)] : 0.f;
         wv = wok ? wt[left ? (size_t)(c0 + s) * N + c0 + x : (size_t)(c1 + q) * N + c1 + (x - m1)] : 0.f;
         if (pole) {
             qv = (x == n1) ? 1.f : 0.f;
@@ -378,7 +383,7 @@ __global__ __launch_bounds__(DT) void dc_gather_kernel(ProbList pl, LevelArgs la
         }
     };
     auto put = [&](int t, float qv, float wv) {
-        if (x < n) st[(size_t)(c0 + t) * N + c0 + x] = qv;
+        if (x < n && !vonly) st[(size_t)(c0 + t) * N + c0 + x] = qv;
         gt[(size_t)(c0 + t) * N + c0 + x] = wv;
     };
     int t = 0;
@@ -398,7 +403,7 @@ __global__ __launch_bounds__(DT) void dc_gather_kernel(ProbList pl, LevelArgs la
     for (int r = 0; r < S.nrot; ++r) {
         const int pr = Ar.rpair[r], jp = pr & 0xffff, j = pr >> 16;
         const float c = Ar.rc[r], s = Ar.rs[r];
-        if (x < n) {
+        if (x < n && !vonly) {
             float *a = st + (size_t)(c0 + jp) * N + c0 + x, *b = st + (size_t)(c0 + j) * N + c0 + x;
             const float av = *a, bv = *b;
             *a = c * av - s * bv;
@@ -726,9 +731,10 @@ __global__ __launch_bounds__(DT) void dc_vectors_kernel(ProbList pl, LevelArgs l
     const double *d = Ar.dk, *z = Ar.zk;
     float *um = P.a + (size_t)(c0 + i) * N + c0;
     float *vm = P.vmt + (size_t)(c0 + i) * N + c0;
+    const bool left = !la.vonly;
     if (k == 1 && z[0] == 0.0) {
         if (lane == 0) {
-            um[0] = 1.f;
+            if (left) um[0] = 1.f;
             vm[0] = 1.f;
         }
         return;
@@ -748,7 +754,7 @@ __global__ __launch_bounds__(DT) void dc_vectors_kernel(ProbList pl, LevelArgs l
     for (int j = lane; j < k; j += G) {
         const double df = (d[j] - dK) * (d[j] + dK) - mu;
         const double v = z[j] / df, u = (j == 0) ? -1.0 : d[j] * z[j] / df;
-        um[j] = (float)(u * nu);
+        if (left) um[j] = (float)(u * nu);
         vm[j] = (float)(v * nv);
     }
 }
@@ -766,11 +772,12 @@ __global__ __launch_bounds__(DT) void dc_finalize_kernel(ProbList pl, LevelArgs 
     const int N = P.N, c0 = T.c0, n = S.n, m = S.m, k = S.k;
     const Arena Ar = make_arena(P.idx, c0, n);
     const float *st = P.qt + (size_t)N * N, *gt = P.wt + (size_t)N * N;
-    const int x = cs.x;
+    const bool vonly = la.vonly != 0;
+    const int x = vonly ? (cs.x == 0 ? 0 : (cs.x == 1 && m > 1 ? m - 1 : m)) : cs.x;
     if (x < m) {
         for (int t = k; t < n; ++t) {
             const int src = Ar.order[t], dst = Ar.pos[t];
-            if (x < n) P.qt[(size_t)(c0 + dst) * N + c0 + x] = st[(size_t)(c0 + src) * N + c0 + x];
+            if (x < n && !vonly) P.qt[(size_t)(c0 + dst) * N + c0 + x] = st[(size_t)(c0 + src) * N + c0 + x];
             P.wt[(size_t)(c0 + dst) * N + c0 + x] = gt[(size_t)(c0 + src) * N + c0 + x];
         }
         if (S.sq) P.wt[(size_t)(c0 + n) * N + c0 + x] = gt[(size_t)(c0 + n) * N + c0 + x];
@@ -823,6 +830,29 @@ __global__ __launch_bounds__(GT) void dc_merge_gemm_kernel(ProbList pl, LevelArg
     t.n0 = (tile % tpt) * 64;
     if (t.m0 >= t.m || t.n0 >= t.n) return;
     tile_gemm<MODE, TKV>(t);
+}
+
+// Right-vector product of a merge with singular values only: the first and last columns
+// of rows pos[i] of the node block <- sum_j VMt[i][j] * gathered[order[j]] (i, j < k), one
+// thread per output entry, FP32 in the order j = 0, 1, ...
+__global__ __launch_bounds__(DT) void dc_merge_cols_kernel(ProbList pl, LevelArgs la)
+{
+    const int g = blockIdx.x * DT + threadIdx.x;
+    const int per = 2 * la.stride;
+    if (g >= la.ntask * per) return;
+    const int w = g % per, i = w >> 1, side = w & 1;
+    const MergeCtx T = merge_ctx(pl, la, g / per);
+    const MergeState &S = *T.S;
+    const int k = S.k, m = S.m;
+    if (i >= k || (side == 1 && m == 1)) return;
+    const ProbDesc P = pl.get(T.prob);
+    const int N = P.N, c0 = T.c0, x = side ? m - 1 : 0;
+    const Arena Ar = make_arena(P.idx, c0, S.n);
+    const float *vm = P.vmt + (size_t)(c0 + i) * N + c0;
+    const float *gt = P.wt + (size_t)N * N + (size_t)c0 * N + c0 + x;
+    float acc = 0.f;
+    for (int j = 0; j < k; ++j) acc += vm[j] * gt[(size_t)Ar.order[j] * N];
+    P.wt[(size_t)(c0 + Ar.pos[i]) * N + c0 + x] = acc;
 }
 
 // back-transforms (updateUVt, svd.cpp:1911-1981):
@@ -942,7 +972,10 @@ hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s)
     else
         hipLaunchKernelGGL(dc_vectors_kernel<64>, dim3(vblocks), dim3(DT), 0, s, pl, la);
     const int mode = gemm_get_mode();
-    if (mode == 1)
+    if (la.vonly)
+        hipLaunchKernelGGL(dc_merge_cols_kernel, dim3((int)(((size_t)la.ntask * 2 * la.stride + DT - 1) / DT)), dim3(DT),
+                           0, s, pl, la);
+    else if (mode == 1)
         launch_merge_gemms<1>(pl, la, s);
     else if (mode == 2)
         launch_merge_gemms<2>(pl, la, s);

@@ -243,8 +243,9 @@ static hipError_t form_q_pt(ProbList pl, const int *idq, int nidq, const int *id
     return launch_transpose_vt(pl, tpair, ntp, s);
 }
 
+// vonly: singular values only (Q and P^T are not formed; u, vt are not touched)
 static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, float *e, float *tauq, float *taup,
-                         void *workspace, hipStream_t s)
+                         void *workspace, hipStream_t s, bool vonly = false)
 {
     const WsLayout L = ws_layout(M, N);
     char *ws = static_cast<char *>(workspace ? workspace : internal_ws(s, L.total));
@@ -270,6 +271,10 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     }
     if (err != hipSuccess) return (int)err;
     tmark(s, 2, tm);
+    if (vonly) {
+        tmark(s, 3, tm);
+        return 0;
+    }
     const int nt = cdiv(N, 32);
     if (large_matrix(M, N)) {
         // blocks of 32 reflectors applied to the whole trailing part of Q, P as GEMMs
@@ -300,6 +305,7 @@ struct DcOffsets {
     std::vector<int> ntask, maxn;
     std::vector<MergeState *> states;
     const int *bt_tu = nullptr, *bt_tv = nullptr, *bt_ru = nullptr, *bt_rv = nullptr;
+    bool vonly = false; // singular values only (no left vectors, two right-vector columns per node)
     int ntu = 0, ntv = 0, nru = 0, nrv = 0;
 };
 
@@ -319,6 +325,7 @@ static hipError_t run_dc_levels(ProbList pl, const DcOffsets &o, const uint32_t 
         la.maxn = o.maxn[r - 1];
         la.stride = std::max(1, std::min(3 << r, la.maxn));
         la.maxit = g_secular_maxit;
+        la.vonly = o.vonly ? 1 : 0;
         if ((e = launch_dc_level(pl, la, s)) != hipSuccess) return e;
     }
     return hipSuccess;
@@ -337,7 +344,7 @@ static hipError_t run_dc(ProbList pl, const DcOffsets &o, const uint32_t *blob_l
 }
 
 static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, float *e, float *qt, float *wt,
-                     uint32_t *idx, void *workspace, const void *tiling_dev, hipStream_t s)
+                     uint32_t *idx, void *workspace, const void *tiling_dev, hipStream_t s, bool vonly = false)
 {
     const WsLayout L = ws_layout(M, N);
     char *ws = static_cast<char *>(workspace ? workspace : internal_ws(s, L.total));
@@ -352,6 +359,7 @@ static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, floa
     P.idx = idx;
     hipLaunchKernelGGL(dc_guard_kernel, dim3(1), dim3(256), 0, s, pl);
     DcOffsets o;
+    o.vonly = vonly;
     o.nleaf = leaf_count(N);
     MergeState *st = reinterpret_cast<MergeState *>(ws + L.states);
     const int R = rounds_of(N);
@@ -376,7 +384,7 @@ static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, floa
     hipError_t err = run_dc_levels(pl, o, blob, s);
     if (err != hipSuccess) return (int)err;
     tmark(s, 4, tm);
-    err = run_dc_backtransform(pl, o, s);
+    if (!vonly) err = run_dc_backtransform(pl, o, s);
     tmark(s, 5, tm);
     return (int)err;
 }
@@ -455,6 +463,7 @@ struct BatchPlan {
     std::vector<size_t> o_qext[2], o_qg1[2], o_qg3[2];
     std::vector<int> n_qext[2], n_qg1[2], n_qg3[2];
     bool fits = true;
+    bool vonly = false; // singular values only: no Q, P^T, back-transforms or left vectors
 };
 
 template <class T>
@@ -726,10 +735,11 @@ struct Prepared {
 // blob in pinned memory.  Allocates (outside any stream capture).  With info_ext, job i
 // reports into info_ext[info_idx[i]] instead of the plan's own array.
 static hipError_t prepare_batch(const paco_svd_job *jobs, int count, BatchPlan &bp, BatchRes &res, Prepared &pr,
-                                int *info_ext = nullptr, const int *info_idx = nullptr)
+                                int *info_ext = nullptr, const int *info_idx = nullptr, bool vonly = false)
 {
     std::vector<size_t> scr_off;
     build_plan(jobs, count, bp, scr_off);
+    bp.vonly = vonly;
     if (!bp.fits) return hipErrorInvalidValue;
     size_t nstates = 0;
     for (int v : bp.ntask) nstates += v;
@@ -822,7 +832,9 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     if (n_side > 0 && !serial && (e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
     tmark(s, 2, tm);
     const char *qenv = getenv("PACO_QP_BATCH");
-    if (qenv && qenv[0] == '0') { // strips of 64 columns, FMA
+    if (bp.vonly) {
+        // singular values only: Q and P^T are not formed
+    } else if (qenv && qenv[0] == '0') { // strips of 64 columns, FMA
         if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
                            I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)
             return e;
@@ -849,6 +861,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     }
     tmark(s, 3, tm);
     DcOffsets o;
+    o.vonly = bp.vonly;
     o.leaf_off = I(bp.o_leaf);
     o.nleaf = bp.nleaf;
     MergeState *st = reinterpret_cast<MergeState *>(dev + pr.o_states);
@@ -870,7 +883,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     hipLaunchKernelGGL(dc_guard_kernel, dim3(count), dim3(256), 0, s, pl);
     if ((e = run_dc_levels(pl, o, nullptr, s)) != hipSuccess) return e;
     tmark(s, 4, tm);
-    e = run_dc_backtransform(pl, o, s);
+    if (!bp.vonly) e = run_dc_backtransform(pl, o, s);
     tmark(s, 5, tm);
     return e;
 }
@@ -937,9 +950,10 @@ static uint64_t batch_signature(const paco_svd_job *jobs, int count)
     return h;
 }
 
-static bool job_ok(const paco_svd_job &j)
+static bool job_ok(const paco_svd_job &j, bool vonly = false)
 {
-    return j.M >= j.N && j.N >= 1 && j.A && j.S && j.U && j.Vt && j.N <= PACO_MAX_N && bidiag_fits(j.M, j.N);
+    return j.M >= j.N && j.N >= 1 && j.A && j.S && (vonly || (j.U && j.Vt)) && j.N <= PACO_MAX_N &&
+           bidiag_fits(j.M, j.N);
 }
 
 // ---------------------------------------------------------------------------
@@ -1019,7 +1033,8 @@ static size_t plain_job_bytes(int M, int N) { return scratch_layout(M, N).total 
 
 // The plain sequence of jobs on stream s with one job's scratch at buf (reused by every
 // job, in stream order).
-static int run_plain_in(const paco_svd_job *jobs, int count, hipStream_t s, int *info, const int *idx, char *buf)
+static int run_plain_in(const paco_svd_job *jobs, int count, hipStream_t s, int *info, const int *idx, char *buf,
+                        bool vonly = false)
 {
     for (int i = 0; i < count; ++i) {
         const int M = jobs[i].M, N = jobs[i].N;
@@ -1029,11 +1044,12 @@ static int run_plain_in(const paco_svd_job *jobs, int count, hipStream_t s, int 
         hipError_t e = hipMemcpyAsync(a, jobs[i].A, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return (int)e;
         int rc = bidiag_single(M, N, a, jobs[i].U, jobs[i].Vt, jobs[i].S, reinterpret_cast<float *>(buf + L.e),
-                               reinterpret_cast<float *>(buf + L.tauq), reinterpret_cast<float *>(buf + L.taup), ws, s);
+                               reinterpret_cast<float *>(buf + L.tauq), reinterpret_cast<float *>(buf + L.taup), ws, s,
+                               vonly);
         if (rc) return rc;
         rc = dc_single(M, N, a, jobs[i].U, jobs[i].Vt, jobs[i].S, reinterpret_cast<float *>(buf + L.e),
                        reinterpret_cast<float *>(buf + L.qt), reinterpret_cast<float *>(buf + L.wt),
-                       reinterpret_cast<uint32_t *>(buf + L.idx), ws, nullptr, s);
+                       reinterpret_cast<uint32_t *>(buf + L.idx), ws, nullptr, s, vonly);
         if (rc) return rc;
         hipLaunchKernelGGL(copy_int_kernel, dim3(1), dim3(64), 0, s, info + (idx ? idx[i] : i),
                            reinterpret_cast<const int *>(ws + ws_layout(M, N).info));
@@ -1041,7 +1057,8 @@ static int run_plain_in(const paco_svd_job *jobs, int count, hipStream_t s, int 
     return (int)hipGetLastError();
 }
 
-static int run_plain(const paco_svd_job *jobs, int count, hipStream_t s, int *info, const int *idx)
+static int run_plain(const paco_svd_job *jobs, int count, hipStream_t s, int *info, const int *idx,
+                     bool vonly = false)
 {
     size_t need = 0;
     for (int i = 0; i < count; ++i) need = std::max(need, plain_job_bytes(jobs[i].M, jobs[i].N));
@@ -1051,7 +1068,7 @@ static int run_plain(const paco_svd_job *jobs, int count, hipStream_t s, int *in
         buf = static_cast<char *>(stream_res(s)->plain.get(need));
     }
     if (!buf) return PACO_ENOMEM;
-    return run_plain_in(jobs, count, s, info, idx, buf);
+    return run_plain_in(jobs, count, s, info, idx, buf, vonly);
 }
 
 // A zeroed per-job info array of the stream (count entries).
@@ -1188,41 +1205,19 @@ static bool split_wide(const paco_svd_job *jobs, int count, std::vector<int> &na
 // The batched launch sequence of jobs on stream s.  With info_ext, job i reports into
 // info_ext[info_idx[i]]; otherwise into the plan's own array, which becomes the thread's
 // last info.
-static int batched_narrow(const paco_svd_job *jobs, int count, hipStream_t s, int *info_ext, const int *info_idx)
+static int batched_narrow(const paco_svd_job *jobs, int count, hipStream_t s, int *info_ext, const int *info_idx,
+                          bool vonly)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     BatchRes &res = stream_res(s)->batch;
     BatchPlan bp;
     Prepared pr;
-    hipError_t e = prepare_batch(jobs, count, bp, res, pr, info_ext, info_idx);
+    hipError_t e = prepare_batch(jobs, count, bp, res, pr, info_ext, info_idx, vonly);
     if (e == hipErrorInvalidValue && !bp.fits) return PACO_EINVAL;
     if (e != hipSuccess) return (int)e;
     e = enqueue_batch(bp, res, pr, s, true);
     if (e != hipSuccess) return (int)e;
     if (!info_ext) set_last_info(pr.info, count);
-    return 0;
-}
-
-int paco_svd_batched(const paco_svd_job *jobs, int count, void *stream)
-{
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    if (count <= 0) return 0;
-    for (int i = 0; i < count; ++i)
-        if (!job_ok(jobs[i])) return PACO_EINVAL;
-    std::vector<int> ni, wi;
-    if (!split_wide(jobs, count, ni, wi)) return batched_narrow(jobs, count, s, nullptr, nullptr);
-    // mixed: one info array in call order; the narrow jobs as one batch, the wide ones
-    // through the entry points
-    int *info = stream_info(s, count);
-    if (!info) return PACO_ENOMEM;
-    std::vector<paco_svd_job> nj, wj;
-    for (int i : ni) nj.push_back(jobs[i]);
-    for (int i : wi) wj.push_back(jobs[i]);
-    int rc = nj.empty() ? 0 : batched_narrow(nj.data(), (int)nj.size(), s, info, ni.data());
-    if (rc) return rc;
-    rc = run_plain(wj.data(), (int)wj.size(), s, info, wi.data());
-    if (rc) return rc;
-    set_last_info(info, count);
     return 0;
 }
 
@@ -1233,18 +1228,12 @@ static void graph_entry_destroy(GraphEntry *g)
     delete g;
 }
 
-int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
+// The narrow jobs recorded into a hipGraph per batch signature (shapes, pointers,
+// precision, iteration limit, values-only) on first use, replayed afterwards.
+static int batched_graph(const paco_svd_job *jobs, int count, hipStream_t s, bool vonly)
 {
-    if (count <= 0) return 0;
-    for (int i = 0; i < count; ++i)
-        if (!job_ok(jobs[i])) return PACO_EINVAL;
-    {
-        std::vector<int> ni, wi;
-        if (split_wide(jobs, count, ni, wi)) return paco_svd_batched(jobs, count, stream);
-    }
-    hipStream_t s = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(g_mu);
-    const uint64_t sig = batch_signature(jobs, count);
+    const uint64_t sig = batch_signature(jobs, count) ^ (vonly ? 0x9e3779b97f4a7c15ull : 0ull);
     auto it = g_graphs.find(sig);
     GraphEntry *g = nullptr;
     if (it == g_graphs.end()) {
@@ -1257,7 +1246,7 @@ int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
         }
         g = new GraphEntry();
         Prepared pr;
-        hipError_t e = prepare_batch(jobs, count, g->plan, g->res, pr);
+        hipError_t e = prepare_batch(jobs, count, g->plan, g->res, pr, nullptr, nullptr, vonly);
         if (e != hipSuccess) {
             graph_entry_destroy(g);
             return (e == hipErrorInvalidValue) ? PACO_EINVAL : (int)e;
@@ -1297,6 +1286,46 @@ int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
     if (e != hipSuccess) return (int)e;
     set_last_info(g->info, count);
     return 0;
+}
+
+// Whole SVDs (vonly: singular values only) of a batch: narrow jobs as one launch sequence
+// (eager or a replayed graph), wide jobs through the entry points; info in call order.
+static int batched_impl(const paco_svd_job *jobs, int count, hipStream_t s, bool graph, bool vonly)
+{
+    if (count <= 0) return 0;
+    for (int i = 0; i < count; ++i)
+        if (!job_ok(jobs[i], vonly)) return PACO_EINVAL;
+    std::vector<int> ni, wi;
+    if (!split_wide(jobs, count, ni, wi))
+        return graph ? batched_graph(jobs, count, s, vonly) : batched_narrow(jobs, count, s, nullptr, nullptr, vonly);
+    // mixed: one info array in call order; the narrow jobs as one batch, the wide ones
+    // through the entry points
+    int *info = stream_info(s, count);
+    if (!info) return PACO_ENOMEM;
+    std::vector<paco_svd_job> nj, wj;
+    for (int i : ni) nj.push_back(jobs[i]);
+    for (int i : wi) wj.push_back(jobs[i]);
+    int rc = nj.empty() ? 0 : batched_narrow(nj.data(), (int)nj.size(), s, info, ni.data(), vonly);
+    if (rc) return rc;
+    rc = run_plain(wj.data(), (int)wj.size(), s, info, wi.data(), vonly);
+    if (rc) return rc;
+    set_last_info(info, count);
+    return 0;
+}
+
+int paco_svd_batched(const paco_svd_job *jobs, int count, void *stream)
+{
+    return batched_impl(jobs, count, static_cast<hipStream_t>(stream), false, false);
+}
+
+int paco_svd_batched_graph(const paco_svd_job *jobs, int count, void *stream)
+{
+    return batched_impl(jobs, count, static_cast<hipStream_t>(stream), true, false);
+}
+
+int paco_svd_values_batched(const paco_svd_job *jobs, int count, int graph, void *stream)
+{
+    return batched_impl(jobs, count, static_cast<hipStream_t>(stream), graph != 0, true);
 }
 
 size_t paco_batch_workspace_size(const int *M, const int *N, int count)

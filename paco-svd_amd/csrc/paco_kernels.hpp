@@ -25,6 +25,8 @@ struct LevelArgs {
     int ntask, r, stride;
     int maxn; // largest merged row count of the level (sizes the deflation LDS)
     int maxit; // iteration limit of the secular solver per root (a root that reaches it adds 1 to info)
+    int vonly; // singular values only: nodes keep the first and last columns of their right
+               // vectors (all a merge reads), no left vectors
 };
 
 struct CopyDesc {

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libpacosvd.so")
 EXPORTS = [
     "paco_upper_bidiagonalization", "paco_svd_DC", "getTilingSize", "GenerateTiling",
     "paco_workspace_size", "paco_set_precision", "paco_svd_plain", "paco_svd_batched",
-    "paco_svd_batched_graph", "paco_svd_plain_noalloc", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
+    "paco_svd_batched_graph", "paco_svd_values_batched", "paco_svd_plain_noalloc", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
     "paco_timing_enable", "paco_timing_read", "paco_info_offset", "paco_set_secular_maxiter",
     "paco_server_create", "paco_server_submit", "paco_server_poll", "paco_server_drain", "paco_server_stats",
     "paco_server_destroy",
@@ -48,6 +48,7 @@ def lib():
     L.paco_svd_plain.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, _vp]
     L.paco_svd_batched.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, _vp]
     L.paco_svd_plain_noalloc.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, _vp]
+    L.paco_svd_values_batched.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, ctypes.c_int, _vp]
     L.paco_svd_batched_graph.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, _vp]
     L.paco_batch_workspace_size.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     L.paco_batch_workspace_size.restype = ctypes.c_size_t
@@ -176,6 +177,21 @@ def svd_plain_noalloc(As, outs=None):
     outs = outs if outs is not None else alloc_outputs(As)
     rc = lib().paco_svd_plain_noalloc(_jobs(As, outs), len(As), _stream())
     return rc, outs
+
+
+def svd_values(As, S=None, graph=False):
+    """Singular values only of every matrix (batched): returns [S]."""
+    import torch
+    S = S if S is not None else [torch.zeros(A.shape[1], dtype=torch.float32, device=A.device) for A in As]
+    jobs = (SvdJob * len(As))()
+    for i, (A, s) in enumerate(zip(As, S)):
+        jobs[i].M, jobs[i].N = A.shape
+        jobs[i].A = A.data_ptr()
+        jobs[i].S = s.data_ptr()
+        jobs[i].U = None
+        jobs[i].Vt = None
+    _check(lib().paco_svd_values_batched(jobs, len(As), 1 if graph else 0, _stream()), "paco_svd_values_batched")
+    return S
 
 
 def svd_batched(As, outs=None, graph=False):
