@@ -104,6 +104,18 @@ typedef struct {
  * sequence (the baseline the batched path is measured against). */
 int paco_svd_plain(const paco_svd_job *jobs, int count, void *stream);
 
+/* LAPACK-style front end (the natural caller of the two entry points): the SVD of one
+ * column-major m x n matrix A (leading dimension lda >= m), as LAPACK sgesdd with device
+ * pointers: S (min(m, n) values, descending); jobz 'A': U (m x m, ldu >= m) and VT
+ * (n x n, ldvt >= n); 'S': the first min(m, n) columns of U and rows of VT (ldvt >=
+ * min(m, n)); 'N': S only (U, VT unused).  Any m, n: the reference rejects m < n
+ * (main.cpp:25-29); here m < n runs on A^T, whose row-major form is the input itself.  A is
+ * not modified ('O' is not offered).  Scratch is the stream's library-owned buffer (see
+ * Scratch above); the info word goes to paco_last_info.  Returns 0, PACO_EINVAL for a bad
+ * argument, or a hipError_t code.  No counterpart in the reference. */
+int paco_sgesdd(char jobz, int m, int n, const float *A, int lda, float *S, float *U, int ldu, float *VT, int ldvt,
+                void *stream);
+
 /* Allocation-free variant of paco_svd_plain.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under hipGraph capture, tight serving
  * loops, hosts that forbid hipMalloc).  The jobs' scratch and info words live in a fixed

@@ -1084,6 +1084,112 @@ static int *stream_info(hipStream_t s, int count)
 }
 
 // ---------------------------------------------------------------------------
+// LAPACK-style front end (paco_sgesdd): column-major input and outputs with leading
+// dimensions, jobz = 'A' | 'S' | 'N', any M, N.
+// ---------------------------------------------------------------------------
+// dst[r * ldd + c] = src[r + c * lds] for r < rows, c < cols (a column-major matrix into a
+// row-major one, or back): 32 x 32 tiles through LDS, both sides coalesced
+__global__ __launch_bounds__(256) void transpose_copy_kernel(const float *src, int lds, float *dst, int ldd, int rows,
+                                                              int cols)
+{
+    __shared__ float tile[32][33];
+    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8) { // read: consecutive r along src's columns
+        const int r = r0 + tx, c = c0 + k;
+        if (r < rows && c < cols) tile[k][tx] = src[(size_t)c * lds + r];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) { // write: consecutive c along dst's rows
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) dst[(size_t)r * ldd + c] = tile[tx][k];
+    }
+}
+
+// dst[r * ldd + c] = src[r * lds + c] for r < rows, c < cols
+__global__ __launch_bounds__(256) void strided_copy_kernel(const float *src, int lds, float *dst, int ldd, int rows,
+                                                            int cols)
+{
+    const int r = blockIdx.y;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < cols; c += 256 * gridDim.x)
+        if (r < rows) dst[(size_t)r * ldd + c] = src[(size_t)r * lds + c];
+}
+
+static hipError_t transpose_copy(const float *src, int lds, float *dst, int ldd, int rows, int cols, hipStream_t s)
+{
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(transpose_copy_kernel, dim3(cdiv(cols, 32), cdiv(rows, 32)), dim3(256), 0, s, src, lds, dst, ldd,
+                       rows, cols);
+    return hipGetLastError();
+}
+static hipError_t strided_copy(const float *src, int lds, float *dst, int ldd, int rows, int cols, hipStream_t s)
+{
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(strided_copy_kernel, dim3(std::min(cdiv(cols, 256), 16), rows), dim3(256), 0, s, src, lds, dst,
+                       ldd, rows, cols);
+    return hipGetLastError();
+}
+
+static int sgesdd_impl(char jobz, int m, int n, const float *A, int lda, float *S, float *U, int ldu, float *VT,
+                       int ldvt, hipStream_t s)
+{
+    const bool all = jobz == 'A' || jobz == 'a', some = jobz == 'S' || jobz == 's', none = jobz == 'N' || jobz == 'n';
+    if (!all && !some && !none) return PACO_EINVAL;
+    if (m < 1 || n < 1 || !A || !S || lda < m) return PACO_EINVAL;
+    const int k = std::min(m, n);
+    if (!none) {
+        if (!U || !VT || ldu < m || ldvt < (all ? n : k)) return PACO_EINVAL;
+    }
+    // the engine's problem: row-major Mr x Nr with Mr >= Nr.  m >= n: B = A (a transpose
+    // of the column-major input); m < n: B = A^T, whose row-major form is the input itself.
+    const bool tall = m >= n;
+    const int Mr = tall ? m : n, Nr = tall ? n : m;
+    if (Nr > PACO_MAX_N || !bidiag_fits(Mr, Nr)) return PACO_EINVAL;
+    const BatchScratch L = scratch_layout(Mr, Nr);
+    const size_t o_u = align_up(plain_job_bytes(Mr, Nr), 256);
+    const size_t o_vt = align_up(o_u + (none ? 0 : sizeof(float) * (size_t)Mr * Mr), 256);
+    const size_t need = align_up(o_vt + (none ? 0 : sizeof(float) * (size_t)Nr * Nr), 256);
+    char *buf;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        buf = static_cast<char *>(stream_res(s)->plain.get(need));
+    }
+    if (!buf) return PACO_ENOMEM;
+    char *ws = buf + L.total;
+    float *a = reinterpret_cast<float *>(buf + L.a);
+    float *u = none ? nullptr : reinterpret_cast<float *>(buf + o_u);
+    float *vt = none ? nullptr : reinterpret_cast<float *>(buf + o_vt);
+    hipError_t e = tall ? transpose_copy(A, lda, a, n, m, n, s) : strided_copy(A, lda, a, m, n, m, s);
+    if (e != hipSuccess) return (int)e;
+    int rc = bidiag_single(Mr, Nr, a, u, vt, S, reinterpret_cast<float *>(buf + L.e),
+                           reinterpret_cast<float *>(buf + L.tauq), reinterpret_cast<float *>(buf + L.taup), ws, s, none);
+    if (rc) return rc;
+    rc = dc_single(Mr, Nr, a, u, vt, S, reinterpret_cast<float *>(buf + L.e), reinterpret_cast<float *>(buf + L.qt),
+                   reinterpret_cast<float *>(buf + L.wt), reinterpret_cast<uint32_t *>(buf + L.idx), ws, nullptr, s,
+                   none);
+    if (rc) return rc;
+    int *info = stream_info(s, 1);
+    if (!info) return PACO_ENOMEM;
+    hipLaunchKernelGGL(copy_int_kernel, dim3(1), dim3(64), 0, s, info,
+                       reinterpret_cast<const int *>(ws + ws_layout(Mr, Nr).info));
+    if (!none) {
+        const int ucols = all ? m : k, vrows = all ? n : k;
+        if (tall) {
+            // U (m x ucols) = u[:, :ucols], VT (n x n) = vt: column-major copies of row-major
+            if ((e = transpose_copy(u, Mr, U, ldu, ucols, m, s)) != hipSuccess) return (int)e;
+            if ((e = transpose_copy(vt, Nr, VT, ldvt, n, n, s)) != hipSuccess) return (int)e;
+        } else {
+            // A = V_B S U_B^T: U = V_B, whose column-major form is vt row by row; VT = U_B^T,
+            // whose column-major form is u row by row (its first vrows entries)
+            if ((e = strided_copy(vt, Nr, U, ldu, m, m, s)) != hipSuccess) return (int)e;
+            if ((e = strided_copy(u, Mr, VT, ldvt, n, vrows, s)) != hipSuccess) return (int)e;
+        }
+    }
+    set_last_info(info, 1);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // allocation-free plain path (paco_svd_plain_noalloc): the scratch of the jobs and their
 // info words live in a fixed device array of the library, so a call allocates nothing.
 // Every call on one device shares that device's array: calls that overlap on one device
@@ -1172,6 +1278,12 @@ int paco_set_precision(int mode)
     const int old = gemm_get_mode();
     if (mode >= 0 && mode <= 2) gemm_set_mode(mode);
     return old;
+}
+
+int paco_sgesdd(char jobz, int m, int n, const float *A, int lda, float *S, float *U, int ldu, float *VT, int ldvt,
+                void *stream)
+{
+    return sgesdd_impl(jobz, m, n, A, lda, S, U, ldu, VT, ldvt, static_cast<hipStream_t>(stream));
 }
 
 int paco_svd_plain_noalloc(const paco_svd_job *jobs, int count, void *stream)

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libpacosvd.so")
 EXPORTS = [
     "paco_upper_bidiagonalization", "paco_svd_DC", "getTilingSize", "GenerateTiling",
     "paco_workspace_size", "paco_set_precision", "paco_svd_plain", "paco_svd_batched",
-    "paco_svd_batched_graph", "paco_svd_values_batched", "paco_svd_plain_noalloc", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
+    "paco_svd_batched_graph", "paco_svd_values_batched", "paco_sgesdd", "paco_svd_plain_noalloc", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
     "paco_timing_enable", "paco_timing_read", "paco_info_offset", "paco_set_secular_maxiter",
     "paco_server_create", "paco_server_submit", "paco_server_poll", "paco_server_drain", "paco_server_stats",
     "paco_server_destroy",
@@ -49,6 +49,8 @@ def lib():
     L.paco_svd_batched.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, _vp]
     L.paco_svd_plain_noalloc.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, _vp]
     L.paco_svd_values_batched.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, ctypes.c_int, _vp]
+    L.paco_sgesdd.argtypes = [ctypes.c_char, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp,
+                              ctypes.c_int, _vp]
     L.paco_svd_batched_graph.argtypes = [ctypes.POINTER(SvdJob), ctypes.c_int, _vp]
     L.paco_batch_workspace_size.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     L.paco_batch_workspace_size.restype = ctypes.c_size_t
@@ -177,6 +179,13 @@ def svd_plain_noalloc(As, outs=None):
     outs = outs if outs is not None else alloc_outputs(As)
     rc = lib().paco_svd_plain_noalloc(_jobs(As, outs), len(As), _stream())
     return rc, outs
+
+
+def sgesdd(jobz, m, n, A, lda, S, U=None, ldu=1, VT=None, ldvt=1):
+    """LAPACK-style SVD of a column-major m x n matrix held in device tensor A (flat, leading
+    dimension lda); S, U, VT are flat device tensors (column-major).  Returns the code."""
+    return lib().paco_sgesdd(jobz.encode() if isinstance(jobz, str) else jobz, m, n, _ptr(A), lda, _ptr(S), _ptr(U),
+                             ldu, _ptr(VT), ldvt, _stream())
 
 
 def svd_values(As, S=None, graph=False):
