@@ -39,6 +39,8 @@ def lib():
         L = ctypes.CDLL(_LIB_PATH)
         L.orc_house.argtypes = [ctypes.c_int, _dp, ctypes.c_long, _dp, _dp]
         L.orc_bidiag.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 7
+        _fp = ctypes.POINTER(ctypes.c_float)
+        L.orc_bidiag_f32.argtypes = [ctypes.c_int, ctypes.c_int] + [_fp] * 5
         L.orc_leaf_partition.argtypes = [ctypes.c_int, _ip, _ip]
         L.orc_leaf_partition.restype = ctypes.c_int
         L.orc_leaf_svd.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
@@ -63,6 +65,21 @@ def house(x):
     tau = ctypes.c_double()
     lib().orc_house(len(x), _p(x), 1, ctypes.byref(beta), ctypes.byref(tau))
     return x, beta.value, tau.value
+
+
+def bidiag_f32(A):
+    """The bidiagonalization in FP32 (the yardstick of FP32 forward error): dict(a, d, e,
+    tauq, taup) in the reference layout."""
+    a = np.ascontiguousarray(A, dtype=np.float32).copy()
+    M, N = a.shape
+    d = np.zeros(N, np.float32)
+    e = np.zeros(max(N - 1, 1), np.float32)
+    tq = np.zeros(N, np.float32)
+    tp = np.zeros(max(N - 1, 1), np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib().orc_bidiag_f32(M, N, a.ctypes.data_as(fp), d.ctypes.data_as(fp), e.ctypes.data_as(fp), tq.ctypes.data_as(fp),
+                         tp.ctypes.data_as(fp))
+    return dict(a=a, d=d, e=e[: N - 1], tauq=tq, taup=tp[: N - 1])
 
 
 def bidiag(A):

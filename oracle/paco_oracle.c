@@ -15,6 +15,8 @@
  *   orc_bidiag           invariants vs FP64 LAPACK: U, Vt orthogonal; U B Vt = A;
  *                        sigma(B) = dgesdd(A); d_i, e_i >= 0 (uniqueness of Golub-Kahan
  *                        with this sign convention); brute-force 2x2 / 3x2 cases
+ *   orc_bidiag_f32       the FP32 yardstick: within FP32 forward error of orc_bidiag, tau in
+ *                        [0, 2], tau = 2 / |v|^2 for every stored vector
  *   orc_leaf_svd         brute force vs numpy.linalg.svd on every 1x1/1x2/2x2/2x3 case,
  *                        including rank-0/1 and zero-pattern branches
  *   orc_secular_root     scipy.linalg.lapack.dlasd4 per root (LAPACK's own solver)
@@ -108,6 +110,61 @@ ORC_EXPORT void orc_bidiag(int M, int N, double *a, double *u, double *vt, doubl
         if (taup[i] == 0.0) continue;
         for (int j = i; j < N; j++)
             apply_reflector(N - i - 1, a + (long)i * N + i + 1, 1, taup[i], vt + (long)j * N + i + 1, 1);
+    }
+}
+
+/* The same bidiagonalization in FP32 (vectors, reflector applications, stored outputs),
+ * with each reflector's sum of squares and its parameters in FP64 as the GPU forms them.
+ * It is the yardstick of the FP32 forward error: Golub-Kahan coefficients are
+ * forward-sensitive, so the GPU's outputs are compared with the FP64 oracle within a
+ * margin of what an FP32 run of the same algorithm deviates.  Only the reflectors, d, e,
+ * tauq, taup are formed (no u, vt). */
+static void house_f32(int n, float *x, long inc, float *beta, float *tau)
+{
+    if (n <= 1) {
+        *tau = 0.f;
+        *beta = x[0];
+        return;
+    }
+    const double x0 = x[0];
+    double sigma = 0.0;
+    for (int k = 1; k < n; k++) sigma += (double)x[k * inc] * x[k * inc];
+    if (sigma == 0.0) {
+        *tau = 0.f;
+        *beta = (float)x0;
+        return;
+    }
+    const double mu = sqrt(x0 * x0 + sigma);
+    const double v0 = (x0 <= 0.0) ? x0 - mu : -sigma / (x0 + mu);
+    *tau = (float)(2.0 * v0 * v0 / (sigma + v0 * v0));
+    const float inv = (float)(1.0 / v0);
+    for (int k = 1; k < n; k++) x[k * inc] *= inv;
+    x[0] = (float)mu;
+    *beta = (float)mu;
+}
+
+static void apply_reflector_f32(int n, const float *hv, long hinc, float tau, float *y, long yinc)
+{
+    float c = y[0];
+    for (int k = 1; k < n; k++) c += hv[k * hinc] * y[k * yinc];
+    c *= tau;
+    y[0] -= c;
+    for (int k = 1; k < n; k++) y[k * yinc] -= c * hv[k * hinc];
+}
+
+ORC_EXPORT void orc_bidiag_f32(int M, int N, float *a, float *d, float *e, float *tauq, float *taup)
+{
+    for (int i = 0; i < N; i++) {
+        house_f32(M - i, a + (long)i * N + i, N, d + i, tauq + i);
+        if (tauq[i] != 0.f)
+            for (int j = i + 1; j < N; j++)
+                apply_reflector_f32(M - i, a + (long)i * N + i, N, tauq[i], a + (long)i * N + j, N);
+        if (i < N - 1) {
+            house_f32(N - i - 1, a + (long)i * N + i + 1, 1, e + i, taup + i);
+            if (taup[i] != 0.f)
+                for (int r = i + 1; r < M; r++)
+                    apply_reflector_f32(N - i - 1, a + (long)i * N + i + 1, 1, taup[i], a + (long)r * N + i + 1, 1);
+        }
     }
 }
 

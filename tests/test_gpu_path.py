@@ -507,3 +507,35 @@ def test_grid_persistent_matches_launch_per_phase(torch_dev, shape):
     say("measures", m)
     assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL
     assert m["resid_ratio"] <= RESID_TOL
+
+
+@pytest.mark.parametrize("shape", [(128, 96), (300, 200), (512, 512), (90, 1), (1100, 1000), (2048, 700)])
+def test_entry_a_outputs_vs_oracle(torch_dev, shape):
+    """Every output of entry point (a) element-wise against the FP64 oracle: d, e, tauq,
+    taup and the whole of a (d / e on the band, the column reflectors below it, the row
+    reflectors right of it; upper_bidiagonalization.cpp:204,271-282).  Golub-Kahan
+    coefficients are forward-sensitive, so the bar is derived from the same algorithm run in
+    FP32 on the CPU: max|gpu - fp64| <= 4 max|fp32 - fp64| + 16 eps32 scale, per output
+    (scale: ||A||_2 for d, e; 1 for tau in [0, 2]; max|a| for a)."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(shape[0] * 3 + shape[1])
+    A = (rng.uniform(-1, 1, shape) if shape == (128, 96) else rng.standard_normal(shape)).astype(np.float32)
+    if shape == (128, 96):
+        A = np.load(os.path.join(GOLD, "c1.npz"))["A"]
+    say("entry (a) outputs", shape)
+    r = pacosvd.upper_bidiagonalization(torch.from_numpy(A.copy()).cuda())
+    torch.cuda.synchronize()
+    g = {k: np_of(v).astype(np.float64) for k, v in r.items() if k in ("a", "d", "e", "tauq", "taup")}
+    o = oracle.bidiag(A.astype(np.float64))
+    f = oracle.bidiag_f32(A)
+    an = np.linalg.norm(A.astype(np.float64), 2)
+    eps = float(np.finfo(np.float32).eps)
+    for k, scale in (("d", an), ("e", an), ("tauq", 1.0), ("taup", 1.0), ("a", max(1.0, np.max(np.abs(o["a"]))))):
+        if g[k].size == 0 or (k in ("e", "taup") and shape[1] < 2):
+            continue
+        err = float(np.max(np.abs(g[k] - o[k])))
+        yard = float(np.max(np.abs(f[k].astype(np.float64) - o[k])))
+        bar = 4.0 * yard + 16.0 * eps * scale
+        say(k, "gpu err %.3e fp32 err %.3e bar %.3e" % (err, yard, bar))
+        assert err <= bar, (k, err, yard)

@@ -250,3 +250,29 @@ def test_svd_DC_matches_bdc_backtransform():
     assert m["sigma_err"] < 1e-13 and m["orth_u"] < 1e-12 and m["orth_v"] < 1e-12
     # columns N..M-1 of U keep Q's columns (svd.cpp:1925-1953 touches only the first N)
     assert np.array_equal(U[:, 45:], r["u"][:, 45:])
+
+
+@pytest.mark.parametrize("shape", [(128, 96), (57, 13), (200, 200), (300, 7)])
+def test_bidiag_f32_yardstick(shape):
+    """The FP32 run of the bidiagonalization (the yardstick of FP32 forward error used by
+    the GPU parity test) stays within FP32 forward error of the FP64 oracle, and its own
+    reflectors are consistent (tau in [0, 2], tau (1 + |v_tail|^2) = 2)."""
+    rng = np.random.default_rng(shape[0] + shape[1])
+    A = rng.uniform(-1, 1, shape).astype(np.float32)
+    M, N = shape
+    f = oracle.bidiag_f32(A)
+    o = oracle.bidiag(A.astype(np.float64))
+    an = np.linalg.norm(A.astype(np.float64), 2)
+    assert np.max(np.abs(f["d"] - o["d"])) <= 1e-4 * an
+    assert np.max(np.abs(f["e"] - o["e"])) <= 1e-4 * an
+    assert np.max(np.abs(f["tauq"] - o["tauq"])) <= 1e-3
+    assert np.all((f["tauq"] >= 0) & (f["tauq"] <= 2)) and np.all((f["taup"] >= 0) & (f["taup"] <= 2))
+    for i in range(N):
+        v = f["a"][i + 1:, i].astype(np.float64)
+        if f["tauq"][i] != 0:
+            assert abs(f["tauq"][i] * (1 + v @ v) - 2.0) <= 1e-5 * (1 + M)
+        if i < N - 1 and f["taup"][i] != 0:
+            w = f["a"][i, i + 2:].astype(np.float64)
+            assert abs(f["taup"][i] * (1 + w @ w) - 2.0) <= 1e-5 * (1 + N)
+    # and the stored reflectors themselves within FP32 forward error of the FP64 ones
+    assert np.max(np.abs(f["a"] - o["a"])) <= 1e-3 * max(1.0, np.max(np.abs(o["a"])))
