@@ -142,7 +142,7 @@ def accuracy_sample(torch, host, offs, Ms, Ns, outs, n=40):
     return worst
 
 
-def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0):
+def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0, wide_lanes=4):
     """The serving configuration: Poisson arrivals through the serving loop (continuous
     batching), then the same requests through the plain path (one matrix per launch
     sequence, one after the other) for the served / plain ratio.  Accuracy of every
@@ -155,7 +155,7 @@ def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0):
     As = workloads.c5_matrices(torch, reqs)
     outs = pacosvd.alloc_outputs(As)
     torch.cuda.synchronize()
-    srv = pacosvd.Server(int(budget_gb * (1 << 30)), max_batch=512, batch_lanes=2, wide_lanes=1,
+    srv = pacosvd.Server(int(budget_gb * (1 << 30)), max_batch=512, batch_lanes=2, wide_lanes=wide_lanes,
                          max_M=4096, max_N=4096)
     try:
         print("[bench] serving: submitting", flush=True)
@@ -299,6 +299,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-wide-lanes", type=int, default=4, help="serving: lanes for wide requests")
     ap.add_argument("--no-plain-large", action="store_true", help="skip the 4096^2 plain-path configuration")
     ap.add_argument("--no-precision", action="store_true", help="skip the 8192x4096 precision configuration")
     ap.add_argument("--profile-batched", action="store_true",
@@ -395,7 +396,8 @@ def main():
 
     serving = None
     if args.serve_requests > 0:
-        serving = serving_bench(torch, pacosvd, args.serve_requests, args.serve_rate)
+        serving = serving_bench(torch, pacosvd, args.serve_requests, args.serve_rate,
+                                wide_lanes=args.serve_wide_lanes)
         print("[bench] serving %s" % serving, flush=True)
 
     plain_large = None

@@ -904,7 +904,7 @@ bool bidiag_grid_supported(int M, int N)
     return bidiag_wg_lds_bytes(M - p, N - p, 1024) <= bidiag_wg_lds_limit();
 }
 
-hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s)
+hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int grid)
 {
     const int M = P.M, N = P.N;
     GridWs W;
@@ -913,7 +913,7 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s)
     if (bidiag_wg_lds_bytes(M - p_tail, N - p_tail, 1024) > bidiag_wg_lds_limit()) return hipErrorInvalidValue;
     // tile grid: PACO_GRID overrides (4 .. GMAX); PACO_GRID_PERSIST=0: two launches per column
     const char *genv = getenv("PACO_GRID");
-    int G = genv ? atoi(genv) : 16;
+    int G = grid > 0 ? grid : genv ? atoi(genv) : 16;
     G = G < 4 ? 4 : G > GMAX ? GMAX : G;
     const char *penv = getenv("PACO_GRID_PERSIST");
     const bool persist = !(penv && penv[0] == '0');

@@ -244,8 +244,9 @@ static hipError_t form_q_pt(ProbList pl, const int *idq, int nidq, const int *id
 }
 
 // vonly: singular values only (Q and P^T are not formed; u, vt are not touched)
+// grid: tile grid of the chip-wide kernel (0: default)
 static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, float *e, float *tauq, float *taup,
-                         void *workspace, hipStream_t s, bool vonly = false)
+                         void *workspace, hipStream_t s, bool vonly = false, int grid = 0)
 {
     const WsLayout L = ws_layout(M, N);
     char *ws = static_cast<char *>(workspace ? workspace : internal_ws(s, L.total));
@@ -264,7 +265,7 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
         tmark(s, 1, tm);
     }
     if (use_grid_bidiag(M, N)) {
-        err = launch_bidiag_grid(pl.single, ws + L.grid, s);
+        err = launch_bidiag_grid(pl.single, ws + L.grid, s, grid);
     } else {
         const BidiagChoice c = bidiag_choice(M, N);
         err = launch_bidiag_wg(pl, nullptr, 1, c.lds, s, c.nt);
@@ -1010,7 +1011,7 @@ size_t wide_scratch_bytes(int M, int N) { return scratch_layout(M, N).total + ws
 
 bool job_supported(int M, int N) { return M >= N && N >= 1 && N <= PACO_MAX_N && bidiag_fits(M, N); }
 
-int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s)
+int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s, int grid)
 {
     const int M = job.M, N = job.N;
     const BatchScratch L = scratch_layout(M, N);
@@ -1020,7 +1021,8 @@ int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s)
     hipError_t e = hipMemcpyAsync(a, job.A, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return (int)e;
     int rc = bidiag_single(M, N, a, job.U, job.Vt, job.S, reinterpret_cast<float *>(base + L.e),
-                           reinterpret_cast<float *>(base + L.tauq), reinterpret_cast<float *>(base + L.taup), ws, s);
+                           reinterpret_cast<float *>(base + L.tauq), reinterpret_cast<float *>(base + L.taup), ws, s,
+                           false, grid);
     if (rc) return rc;
     return dc_single(M, N, a, job.U, job.Vt, job.S, reinterpret_cast<float *>(base + L.e),
                      reinterpret_cast<float *>(base + L.qt), reinterpret_cast<float *>(base + L.wt),

@@ -25,6 +25,8 @@ int lane_enqueue(Lane *l, const paco_svd_job *jobs, int count, hipStream_t s);
 bool job_supported(int M, int N);
 // Scratch for one wide job through the two entry points.
 size_t wide_scratch_bytes(int M, int N);
-int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s);
+// grid: tile grid of the chip-wide bidiagonalization (0: default); several wide lanes
+// running at once each take a smaller grid
+int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s, int grid);
 
 } // namespace paco

@@ -96,7 +96,8 @@ hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hip
 size_t bidiag_grid_ws_bytes(int M, int N);
 // whether the LDS of the chip-wide path's kernels holds an M x N problem
 bool bidiag_grid_supported(int M, int N);
-hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s);
+// grid: tiles per side of the panel kernel's tile grid (0: the default, 16, or PACO_GRID)
+hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int grid = 0);
 
 // dc.hip
 hipError_t launch_dc_leaves(ProbList pl, const int *leaf_off, const uint32_t *blob_leaves, int nleaf,

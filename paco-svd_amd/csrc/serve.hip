@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstdlib>
 #include <cstring>
@@ -59,6 +60,7 @@ struct ServeLane {
 struct Server {
     size_t budget = 0, used = 0;
     int max_batch = 0;
+    int wide_grid = 0; // tile grid of the wide lanes' chip-wide bidiagonalization
     std::vector<ServeLane> lanes;
     std::deque<Req> pending;
     std::vector<std::pair<uint64_t, double>> finished; // not yet polled
@@ -113,7 +115,7 @@ bool Server::admit_locked()
         if (fail_every > 0 && admissions % fail_every == 0) {
             rc = PACO_EINJECTED;
         } else if (L.wide) {
-            rc = wide_enqueue(take[0].job, L.scratch, L.s);
+            rc = wide_enqueue(take[0].job, L.scratch, L.s, wide_grid);
             ++wide_jobs;
         } else {
             std::vector<paco_svd_job> jobs;
@@ -210,6 +212,11 @@ void *paco_server_create(size_t hbm_budget_bytes, int max_batch, int batch_lanes
     Server *S = new Server();
     S->budget = hbm_budget_bytes;
     S->max_batch = max_batch;
+    // wide solves are bound by their per-column latency, not by bandwidth: several lanes
+    // run at once, each on a share of the chip, round(16 / sqrt(lanes)) tiles per side (16
+    // alone, 11 for two lanes, 8 for four, 6 for eight); a server's results depend on its
+    // configuration only
+    S->wide_grid = wide_lanes <= 1 ? 16 : std::max(4, (int)(16.0 / std::sqrt((double)wide_lanes) + 0.5));
     if (const char *fe = getenv("PACO_SERVE_FAIL_EVERY")) S->fail_every = atol(fe);
     const size_t cap = hbm_budget_bytes / (size_t)batch_lanes;
     for (int i = 0; i < batch_lanes + wide_lanes; ++i) {

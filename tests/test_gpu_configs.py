@@ -156,7 +156,7 @@ def test_c5_serving_stream(torch_dev):
     with ThreadPoolExecutor(2) as ex:
         refs = [ex.submit(workloads.lapack_sigma, h) for h in host]
         outs = pacosvd.alloc_outputs(As)
-        srv = pacosvd.Server(64 << 30, max_batch=512, batch_lanes=2, wide_lanes=2, max_M=4096, max_N=4096)
+        srv = pacosvd.Server(64 << 30, max_batch=512, batch_lanes=2, wide_lanes=4, max_M=4096, max_N=4096)
         try:
             t0 = time.perf_counter()
             for i, r in enumerate(reqs):
@@ -178,11 +178,12 @@ def test_c5_serving_stream(torch_dev):
             for k, v in m.items():
                 worst[k] = max(worst.get(k, 0.0), v)
         check(worst, "C5 worst over %d requests" % count)
-    # alone: a fresh server, one request at a time: bitwise the same as in the mix
+    # alone: a fresh server of the same configuration (its wide lanes use the same tile
+    # grid), one request at a time: bitwise the same as in the mix
     sub = [i for i in range(count) if reqs[i]["clustered"]][:3]
     sub += [i for i in range(count) if reqs[i]["M"] >= 2048 and i not in sub][:3]
     sub += [i for i in range(count) if i not in sub][:4]
-    srv = pacosvd.Server(64 << 30, max_batch=512, batch_lanes=1, wide_lanes=1, max_M=4096, max_N=4096)
+    srv = pacosvd.Server(64 << 30, max_batch=512, batch_lanes=2, wide_lanes=4, max_M=4096, max_N=4096)
     try:
         for i in sub:
             alone = pacosvd.alloc_outputs([As[i]])[0]
