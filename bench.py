@@ -302,6 +302,7 @@ def main():
     ap.add_argument("--serve-wide-lanes", type=int, default=4, help="serving: lanes for wide requests")
     ap.add_argument("--no-plain-large", action="store_true", help="skip the 4096^2 plain-path configuration")
     ap.add_argument("--no-precision", action="store_true", help="skip the 8192x4096 precision configuration")
+    ap.add_argument("--leaf-size", type=int, default=0, help="columns per D&C leaf (0: the library's default)")
     ap.add_argument("--profile-batched", action="store_true",
                     help="only run the batched step (warmup + steps), for counter collection")
     args = ap.parse_args()
@@ -321,6 +322,8 @@ def main():
     import torch
     import pacosvd
     pacosvd.lib()
+    if args.leaf_size:
+        pacosvd.set_leaf_size(args.leaf_size)
     host, offs, Ms, Ns, As = make_workload(torch, args.count, host_data=host_data)
     outs = alloc_outputs(torch, Ms, Ns)
     torch.cuda.synchronize()

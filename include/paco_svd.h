@@ -25,7 +25,9 @@
  * Status.  Each problem has a LAPACK-style info word: 0 ok, -1 a NaN or infinite entry
  * in the bidiagonal handed to the divide and conquer (the problem is then solved as the
  * zero matrix), -2 a grid barrier of the chip-wide bidiagonalization timed out (results
- * invalid), > 0 the number of secular roots that reached the iteration limit.  The entry
+ * invalid), -3 the tiling blob given to paco_svd_DC does not describe the leaf partition
+ * of N at the current leaf size (it was ignored and that partition used: the results are
+ * valid), > 0 the number of secular roots that reached the iteration limit.  The entry
  * points keep it in their workspace at paco_info_offset(M, N): entry point (a) resets
  * it, (b) adds to it.  The whole-SVD paths report it through paco_last_info.
  *
@@ -49,6 +51,7 @@ extern "C" {
 #define PACO_EINJECTED (-3) /* serving loop: failure injected by PACO_SERVE_FAIL_EVERY (tests) */
 #define PACO_ETOOBIG (-4)   /* paco_svd_plain_noalloc: a job's scratch exceeds the fixed array */
 #define PACO_MAX_N 7680    /* D&C deflation stage keeps one merge in LDS (20 bytes per row) */
+#define PACO_LEAF_DEFAULT 6 /* columns per D&C leaf unless paco_set_leaf_size says otherwise */
 
 /* Entry point (a): Golub-Kahan upper bidiagonalization, A = Q * B * P^T.
  * Replaces the kernel upper_bidiagonalization (upper_bidiagonalization.cpp:967-999),
@@ -68,14 +71,17 @@ int paco_upper_bidiagonalization(unsigned blockDim, void *stream, int M, int N, 
  * Tiling).  In: the outputs of entry point (a).  Out: d = sigma (descending),
  * u = left singular vectors (columns N..M-1 keep Q's), vt = right singular vectors
  * (rows).  a, qt, wt and idx are scratch.  tiling_dev is the device copy of the blob
- * written by GenerateTiling (its leaf list is read on the device); NULL derives the
- * same partition from N. */
+ * written by GenerateTiling or NULL.  The leaves are always the partition of N at the
+ * current leaf size (paco_set_leaf_size); a blob is checked against it on the device and
+ * info = -3 reports one that differs (see Status). */
 int paco_svd_DC(unsigned blockDim, void *stream, int M, int N, float *a, float *u, float *vt, float *d, float *e,
                 float *qt, float *wt, uint32_t *idx, void *workspace, const void *tiling_dev);
 
 /* Host tiling (svd_tiling.cpp:67-80 getTilingSize, :83-130 GenerateTiling): size in
- * bytes of the D&C tiling blob for N columns, and the blob itself (header + leaf list
- * of the reference's 3-column partition). */
+ * bytes of the D&C tiling blob for N columns, and the blob itself: a 64-byte header
+ * (uint32 magic "PACO", version 2, N, leaf count, leaf offset 64, svdBlockNum, leaf size)
+ * and {uint32 start_col, end_col} per leaf, for the current leaf size (with leaf size 3,
+ * the reference's partition). */
 int getTilingSize(int N, int svdBlockNum);
 void GenerateTiling(int N, int svdBlockNum, uint8_t *tiling_host);
 
@@ -159,6 +165,17 @@ int paco_last_info(int *info_host, int count, void *stream);
 /* Byte offset of the int info word inside a workspace of paco_workspace_size(M, N)
  * bytes (written by the two entry points).  No counterpart in the reference. */
 size_t paco_info_offset(int M, int N);
+
+/* Columns per leaf of the divide and conquer, 3..32 (default PACO_LEAF_DEFAULT).  Leaves
+ * of 3 columns are the reference's partition (svd_tiling.cpp:83-130, solved in closed form
+ * at svd.cpp:1596-1864); larger leaves are solved on chip by FP64 one-sided Jacobi (one
+ * thread per leaf up to 6 columns, one wavefront per leaf above) and remove about
+ * log2(ls / 3) merge rounds.  Applies to calls made after it
+ * (and to GenerateTiling / getTilingSize, whose blob paco_svd_DC checks against it).
+ * Returns the previous size; values outside 3..32 only query.  No counterpart in the
+ * reference. */
+int paco_set_leaf_size(int ls);
+int paco_leaf_size(void);
 
 /* Iteration limit of the secular solver per root (default 200; a root that reaches it
  * adds 1 to its problem's info).  Returns the previous limit; values < 1 only query.

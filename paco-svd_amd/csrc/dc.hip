@@ -5,7 +5,8 @@
 // :1596-1864).  The reference runs every merge serially on one vector core.  Here the
 // merge tree is processed level by level; every phase of a level is one launch over
 // all merges of that level (and, in the batched path, of every matrix):
-//   leaves    one thread per leaf, FP64 one-sided Jacobi
+//   leaves    FP64 one-sided Jacobi: one thread per leaf of <= 3 columns, one
+//             wavefront per larger leaf (up to LEAF_MAX columns)
 //   deflate   one workgroup per merge: parallel merge-path ranks, scan, compaction
 //   gather    one thread per basis column: gathered bases + deflation rotations
 //   secular   one wavefront per root, FP64 safeguarded rational iteration
@@ -94,24 +95,16 @@ __device__ void leaf_svd(int n, int m, const double *dd, const double *ee, doubl
     }
 }
 
-__global__ __launch_bounds__(DT) void dc_leaf_kernel(ProbList pl, const int *leaf_off, const uint32_t *blob_leaves,
-                                                     int nleaf)
+// one thread per leaf of at most 3 columns
+__global__ __launch_bounds__(DT) void dc_leaf_kernel(ProbList pl, const int *leaf_off, int nleaf, int ls)
 {
     const int l = blockIdx.x * DT + threadIdx.x;
     if (l >= nleaf) return;
-    int prob, c0, c1;
-    if (blob_leaves) {
-        prob = 0;
-        c0 = (int)blob_leaves[2 * l];
-        c1 = (int)blob_leaves[2 * l + 1];
-    } else {
-        prob = find_task(leaf_off, pl.count, l);
-        const int ll = l - off_at(leaf_off, prob), NN = pl.get(prob).N;
-        c0 = leaf_start(NN, ll);
-        c1 = leaf_start(NN, ll + 1);
-    }
+    const int prob = find_task(leaf_off, pl.count, l);
+    const int ll = l - off_at(leaf_off, prob);
     const ProbDesc P = pl.get(prob);
-    const int N = P.N, m = c1 - c0, n = (c1 == N) ? m : m - 1;
+    const int N = P.N, c0 = leaf_start(N, ll, ls), c1 = leaf_start(N, ll + 1, ls);
+    const int m = c1 - c0, n = (c1 == N) ? m : m - 1;
     if (m < 1 || m > 3 || n < 1) return;
     double dd[2] = {0, 0}, ee[2] = {0, 0}, sig[2], U[4], V[9];
     for (int r = 0; r < n; r++) {
@@ -127,6 +120,306 @@ __global__ __launch_bounds__(DT) void dc_leaf_kernel(ProbList pl, const int *lea
         for (int r = 0; r < m; r++) P.wt[(size_t)(c0 + i) * N + c0 + r] = (float)V[r * m + i];
 }
 
+// ---------------------------------------------------------------------------
+// Leaves of up to LEAF_THREAD_MAX columns: one thread per leaf, the block W and the
+// rotations V in registers (FP64, compile-time indices), cyclic-by-rows one-sided Jacobi
+// with the same rotation, threshold and ordering rules as the wavefront kernel below.
+// ---------------------------------------------------------------------------
+constexpr int LEAF_THREAD_MAX = 6;
+// Jacobi of the leaves: sweep limit, rotation threshold |w_p . w_q| / (||w_p|| ||w_q||),
+// sigma below which a left vector is rebuilt (relative), column norm treated as zero
+// (relative to ||B||_F), LDS column stride of the wavefront kernel (doubles)
+constexpr int LEAF_SWEEPS = 30;
+constexpr double LEAF_TOL = 1e-14;
+constexpr double LEAF_SMALL = 1e-8;
+constexpr double LEAF_ZERO = 1e-15;
+constexpr int LEAF_LP = LEAF_MAX + 1;
+
+template <int MP>
+__global__ __launch_bounds__(DT) void dc_leaf_thread_kernel(ProbList pl, const int *leaf_off, int nleaf, int ls)
+{
+    const int l = blockIdx.x * DT + threadIdx.x;
+    if (l >= nleaf) return;
+    const int prob = find_task(leaf_off, pl.count, l);
+    const int ll = l - off_at(leaf_off, prob);
+    const ProbDesc P = pl.get(prob);
+    const int N = P.N, c0 = leaf_start(N, ll, ls), c1 = leaf_start(N, ll + 1, ls);
+    const int m = c1 - c0, n = (c1 == N) ? m : m - 1;
+    if (m < 1 || m > MP || n < 1) return;
+    double W[MP][MP], V[MP][MP]; // W[c][r]: column c; V[c][r]
+    double fro2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < MP; ++c)
+#pragma unroll
+        for (int r = 0; r < MP; ++r) {
+            V[c][r] = (c == r && c < m) ? 1.0 : 0.0;
+            double w = 0.0;
+            if (r < n && c == r) w = P.d[c0 + r];
+            if (r < n && c == r + 1 && c < m) w = P.e[c0 + r];
+            W[c][r] = w;
+            fro2 += w * w;
+        }
+    const double zero2 = LEAF_ZERO * LEAF_ZERO * fro2;
+    for (int sweep = 0; sweep < LEAF_SWEEPS; ++sweep) {
+        bool rotated = false;
+#pragma unroll
+        for (int p = 0; p < MP - 1; ++p)
+#pragma unroll
+            for (int q = p + 1; q < MP; ++q) {
+                double app = 0.0, aqq = 0.0, apq = 0.0;
+#pragma unroll
+                for (int r = 0; r < MP; ++r) {
+                    app += W[p][r] * W[p][r];
+                    aqq += W[q][r] * W[q][r];
+                    apq += W[p][r] * W[q][r];
+                }
+                if (app > zero2 && aqq > zero2 && apq != 0.0 && fabs(apq) > LEAF_TOL * sqrt(app) * sqrt(aqq)) {
+                    const double zeta = (aqq - app) / (2.0 * apq);
+                    const double t = fabs(zeta) > 1e150
+                                         ? 0.5 / zeta
+                                         : (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+#pragma unroll
+                    for (int r = 0; r < MP; ++r) {
+                        const double wp = W[p][r], wq = W[q][r], vp = V[p][r], vq = V[q][r];
+                        W[p][r] = cs * wp - sn * wq;
+                        W[q][r] = sn * wp + cs * wq;
+                        V[p][r] = cs * vp - sn * vq;
+                        V[q][r] = sn * vp + cs * vq;
+                    }
+                    rotated = true;
+                }
+            }
+        if (!rotated) break;
+    }
+    // sigma_c = ||w_c||; rank of each column (descending, ties by column)
+    double sg[MP];
+    int rk[MP];
+#pragma unroll
+    for (int c = 0; c < MP; ++c) {
+        double s2 = 0.0;
+#pragma unroll
+        for (int r = 0; r < MP; ++r) s2 += W[c][r] * W[c][r];
+        sg[c] = c < m ? sqrt(s2) : -1.0; // padding columns rank last
+    }
+    double smax = 0.0;
+#pragma unroll
+    for (int c = 0; c < MP; ++c) {
+        int k = 0;
+#pragma unroll
+        for (int c2 = 0; c2 < MP; ++c2) k += (sg[c2] > sg[c]) || (sg[c2] == sg[c] && c2 < c);
+        rk[c] = k;
+        smax = fmax(smax, sg[c]);
+    }
+    const double small = LEAF_SMALL * smax;
+    // left vectors: w_c / sigma_c; the small ones (in rank order) orthogonalised against
+    // the others, or replaced by the first unit vector that is not in their span
+#pragma unroll
+    for (int c = 0; c < MP; ++c)
+        if (sg[c] > small)
+#pragma unroll
+            for (int r = 0; r < MP; ++r) W[c][r] /= sg[c];
+    for (int i = 0; i < n; ++i) {
+#pragma unroll
+        for (int c = 0; c < MP; ++c) {
+            if (rk[c] != i || sg[c] > small) continue;
+            double x[MP];
+#pragma unroll
+            for (int r = 0; r < MP; ++r) x[r] = sg[c] > 0.0 ? W[c][r] / sg[c] : (r == i ? 1.0 : 0.0);
+            for (int a = 0; a <= n; ++a) {
+                if (a > 0)
+#pragma unroll
+                    for (int r = 0; r < MP; ++r) x[r] = (r == a - 1) ? 1.0 : 0.0;
+#pragma unroll
+                for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
+                    for (int c2 = 0; c2 < MP; ++c2) {
+                        if (c2 == c || rk[c2] >= n || (rk[c2] > i && sg[c2] <= small)) continue;
+                        double dot = 0.0;
+#pragma unroll
+                        for (int r = 0; r < MP; ++r) dot += x[r] * W[c2][r];
+#pragma unroll
+                        for (int r = 0; r < MP; ++r) x[r] -= dot * W[c2][r];
+                    }
+                double nr = 0.0;
+#pragma unroll
+                for (int r = 0; r < MP; ++r) nr += x[r] * x[r];
+                nr = sqrt(nr);
+                if (nr > 1e-3) {
+#pragma unroll
+                    for (int r = 0; r < MP; ++r) x[r] /= nr;
+                    break;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < MP; ++r) W[c][r] = x[r];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < MP; ++c) {
+        if (c >= m) continue;
+        const int i = rk[c];
+        if (i < n) {
+            P.d[c0 + i] = (float)sg[c];
+#pragma unroll
+            for (int r = 0; r < MP; ++r)
+                if (r < n) P.qt[(size_t)(c0 + i) * N + c0 + r] = (float)W[c][r];
+        }
+#pragma unroll
+        for (int r = 0; r < MP; ++r)
+            if (r < m) P.wt[(size_t)(c0 + i) * N + c0 + r] = (float)V[c][r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Leaves of up to LEAF_MAX columns: one wavefront per leaf runs FP64 one-sided (Hestenes)
+// Jacobi on the n x m bidiagonal block W (n = m - 1 rows, m for the last leaf).  The
+// m2 = m rounded up to even columns are paired by the round-robin (circle) schedule,
+// m2 / 2 disjoint pairs per step and every pair once in a sweep of m2 - 1 steps; pair k
+// takes lanes k, k + 16, k + 32, k + 48, each summing and rotating every 4th row.  A
+// pair rotates (Rutishauser's formulas) when |w_p . w_q| > tol ||w_p|| ||w_q|| and both
+// columns are above 1e-15 ||B||_F (a column below that is rounding noise, whose direction
+// never settles: an n x (n + 1) leaf has one); sweeps repeat until no pair rotates.  Then sigma_c = ||w_c||, in descending order; the right
+// vectors are the accumulated rotations V; the left vectors are w_c / sigma_c, except
+// where sigma_c <= 1e-8 sigma_max: those are orthogonalised (twice, in order) against
+// the others and, when nothing is left, replaced by unit vectors that are, so the left
+// basis is orthonormal also for rank-deficient leaves.
+// LDS: W and V column-major, column c at c * LP (LP = LEAF_MAX + 1 doubles against bank
+// conflicts between the pairs' columns).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(64) void dc_leaf_wave_kernel(ProbList pl, const int *leaf_off, int ls)
+{
+    __shared__ double W[LEAF_MAX * LEAF_LP], V[LEAF_MAX * LEAF_LP];
+    __shared__ double sg[LEAF_MAX];
+    __shared__ int col_of[LEAF_MAX];
+    const int l = blockIdx.x, lane = threadIdx.x;
+    const int prob = find_task(leaf_off, pl.count, l);
+    const int ll = l - off_at(leaf_off, prob);
+    const ProbDesc P = pl.get(prob);
+    const int N = P.N, c0 = leaf_start(N, ll, ls), c1 = leaf_start(N, ll + 1, ls);
+    const int m = c1 - c0, n = (c1 == N) ? m : m - 1;
+    if (m < 1 || m > LEAF_MAX || n < 1) return; // uniform over the workgroup
+    for (int i = lane; i < LEAF_MAX * LEAF_LP; i += 64) {
+        W[i] = 0.0;
+        V[i] = 0.0;
+    }
+    __syncthreads();
+    if (lane < m) V[lane * LEAF_LP + lane] = 1.0;
+    if (lane < n) {
+        W[lane * LEAF_LP + lane] = (double)P.d[c0 + lane];
+        if (lane + 1 < m) W[(lane + 1) * LEAF_LP + lane] = (double)P.e[c0 + lane];
+    }
+    __syncthreads();
+    const int m2 = (m + 1) & ~1, np = m2 >> 1, k = lane & 15, sub = lane >> 4;
+    double fro2 = 0.0; // ||B||_F^2, the same in every lane
+    for (int r = 0; r < n; ++r) {
+        const double a = P.d[c0 + r], b = r + 1 < m ? P.e[c0 + r] : 0.f;
+        fro2 += a * a + b * b;
+    }
+    const double zero2 = LEAF_ZERO * LEAF_ZERO * fro2;
+    for (int sweep = 0; sweep < LEAF_SWEEPS; ++sweep) {
+        int rotated = 0;
+        for (int st = 0; st < m2 - 1; ++st) {
+            const int p = k == 0 ? st : (st + k) % (m2 - 1);
+            const int q = k == 0 ? m2 - 1 : (st - k + m2 - 1) % (m2 - 1);
+            const bool act = k < np && p < m && q < m;
+            double wp[LEAF_MAX / 4], wq[LEAF_MAX / 4], app = 0.0, aqq = 0.0, apq = 0.0;
+#pragma unroll
+            for (int j = 0; j < LEAF_MAX / 4; ++j) {
+                const int r = sub + 4 * j;
+                wp[j] = (act && r < n) ? W[p * LEAF_LP + r] : 0.0;
+                wq[j] = (act && r < n) ? W[q * LEAF_LP + r] : 0.0;
+                app += wp[j] * wp[j];
+                aqq += wq[j] * wq[j];
+                apq += wp[j] * wq[j];
+            }
+            // the four lanes of the pair: the same sums in every one of them
+            app += __shfl_xor(app, 16);
+            aqq += __shfl_xor(aqq, 16);
+            apq += __shfl_xor(apq, 16);
+            app += __shfl_xor(app, 32);
+            aqq += __shfl_xor(aqq, 32);
+            apq += __shfl_xor(apq, 32);
+            if (act && app > zero2 && aqq > zero2 && apq != 0.0 && fabs(apq) > LEAF_TOL * sqrt(app) * sqrt(aqq)) {
+                const double zeta = (aqq - app) / (2.0 * apq);
+                const double t = fabs(zeta) > 1e150 ? 0.5 / zeta
+                                                    : (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+#pragma unroll
+                for (int j = 0; j < LEAF_MAX / 4; ++j) {
+                    const int r = sub + 4 * j;
+                    if (r < n) {
+                        W[p * LEAF_LP + r] = c * wp[j] - s * wq[j];
+                        W[q * LEAF_LP + r] = s * wp[j] + c * wq[j];
+                    }
+                    if (r < m) {
+                        const double vp = V[p * LEAF_LP + r], vq = V[q * LEAF_LP + r];
+                        V[p * LEAF_LP + r] = c * vp - s * vq;
+                        V[q * LEAF_LP + r] = s * vp + c * vq;
+                    }
+                }
+                rotated = 1;
+            }
+            __syncthreads();
+        }
+        if (!__syncthreads_or(rotated)) break;
+    }
+    // sigma_c = ||w_c||; col_of[i]: the column of rank i (descending, ties by column)
+    if (lane < m) {
+        double s2 = 0.0;
+        for (int r = 0; r < n; ++r) s2 += W[lane * LEAF_LP + r] * W[lane * LEAF_LP + r];
+        sg[lane] = sqrt(s2);
+    }
+    __syncthreads();
+    if (lane < m) {
+        int rk = 0;
+        for (int c = 0; c < m; ++c) rk += (sg[c] > sg[lane]) || (sg[c] == sg[lane] && c < lane);
+        col_of[rk] = lane;
+    }
+    __syncthreads();
+    const double small = LEAF_SMALL * sg[col_of[0]];
+    for (int x = lane; x < m * m; x += 64) {
+        const int i = x / m, r = x - i * m;
+        P.wt[(size_t)(c0 + i) * N + c0 + r] = (float)V[col_of[i] * LEAF_LP + r];
+    }
+    if (lane < n) P.d[c0 + lane] = (float)sg[col_of[lane]];
+    for (int x = lane; x < n * n; x += 64) {
+        const int i = x / n, r = x - i * n, c = col_of[i];
+        if (sg[c] > small) W[c * LEAF_LP + r] /= sg[c];
+    }
+    __syncthreads();
+    for (int i = 0; i < n; ++i) {
+        const int c = col_of[i];
+        if (sg[c] > small) continue; // uniform: sg and col_of are in LDS
+        double x = 0.0;
+        if (lane < n) x = sg[c] > 0.0 ? W[c * LEAF_LP + lane] / sg[c] : (lane == i ? 1.0 : 0.0);
+        // attempt 0: the column itself; attempt a > 0: the unit vector e_{a-1}
+        for (int a = 0; a <= n; ++a) {
+            if (a > 0) x = (lane == a - 1) ? 1.0 : 0.0;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int j = 0; j < n; ++j) {
+                    const int cj = col_of[j];
+                    if (j == i || (j > i && sg[cj] <= small)) continue; // not (yet) in the basis
+                    const double u = lane < n ? W[cj * LEAF_LP + lane] : 0.0;
+                    x -= wave_sum_dpp_d(x * u) * u;
+                }
+            const double nr = sqrt(wave_sum_dpp_d(x * x));
+            if (nr > 1e-3) {
+                x /= nr;
+                break;
+            }
+        }
+        __syncthreads();
+        if (lane < n) W[c * LEAF_LP + lane] = x;
+        __syncthreads();
+    }
+    for (int x = lane; x < n * n; x += 64) {
+        const int i = x / n, r = x - i * n;
+        P.qt[(size_t)(c0 + i) * N + c0 + r] = (float)W[col_of[i] * LEAF_LP + r];
+    }
+}
+
 // Merge t of a level: owning problem and columns (device-derived schedule).
 struct MergeCtx {
     int prob, c0, c1, c2;
@@ -137,7 +430,7 @@ __device__ inline MergeCtx merge_ctx(const ProbList &pl, const LevelArgs &la, in
     MergeCtx c;
     c.prob = la.task_prob ? la.task_prob[t] : find_task(la.task_off, pl.count, t);
     const int j = t - off_at(la.task_off, c.prob);
-    merge_cols(pl.get(c.prob).N, la.r, j, c.c0, c.c1, c.c2);
+    merge_cols(pl.get(c.prob).N, la.r, j, la.ls, c.c0, c.c1, c.c2);
     c.S = la.states + t;
     return c;
 }
@@ -534,7 +827,7 @@ __global__ __launch_bounds__(DT) void dc_secular_kernel(ProbList pl, LevelArgs l
     if (lane == 0) {
         Ar.Kx[i] = K;
         Ar.mu[i] = mu;
-        if (it > la.maxit) atomicAdd(P.info, 1);
+        if (it > la.maxit) info_count_failure(P.info);
     }
 }
 
@@ -643,7 +936,7 @@ __global__ __launch_bounds__(DT) void dc_secular_thread_kernel(ProbList pl, Leve
     const int it = secular_serial(k, Ar.dk, Ar.zk, i, la.maxit, K, mu);
     Ar.Kx[i] = K;
     Ar.mu[i] = mu;
-    if (it > la.maxit) atomicAdd(P.info, 1);
+    if (it > la.maxit) info_count_failure(P.info);
 }
 
 // ---------------------------------------------------------------------------
@@ -919,10 +1212,17 @@ __global__ __launch_bounds__(DT) void dc_bt_copy_kernel(ProbList pl, const int *
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_dc_leaves(ProbList pl, const int *leaf_off, const uint32_t *blob_leaves, int nleaf, hipStream_t s)
+hipError_t launch_dc_leaves(ProbList pl, const int *leaf_off, int nleaf, int ls, hipStream_t s)
 {
     if (nleaf <= 0) return hipSuccess;
-    hipLaunchKernelGGL(dc_leaf_kernel, dim3((nleaf + DT - 1) / DT), dim3(DT), 0, s, pl, leaf_off, blob_leaves, nleaf);
+    if (ls < LEAF_MIN || ls > LEAF_MAX) return hipErrorInvalidValue;
+    if (ls <= 3)
+        hipLaunchKernelGGL(dc_leaf_kernel, dim3((nleaf + DT - 1) / DT), dim3(DT), 0, s, pl, leaf_off, nleaf, ls);
+    else if (ls <= LEAF_THREAD_MAX)
+        hipLaunchKernelGGL(dc_leaf_thread_kernel<LEAF_THREAD_MAX>, dim3((nleaf + DT - 1) / DT), dim3(DT), 0, s, pl,
+                           leaf_off, nleaf, ls);
+    else
+        hipLaunchKernelGGL(dc_leaf_wave_kernel, dim3(nleaf), dim3(64), 0, s, pl, leaf_off, ls);
     return hipGetLastError();
 }
 

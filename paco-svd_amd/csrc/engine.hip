@@ -7,6 +7,7 @@
 // path uploads one small plan (problem descriptors + per-problem prefix arrays) and
 // runs each phase as one launch over every matrix.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -41,19 +42,42 @@ __global__ void copy_int_kernel(int *dst, const int *src)
 // NaN or infinite entry (non-finite input, or overflow in the bidiagonalization) sets
 // info = -1 and is replaced by zeros, so the deflation's sorting and compaction only
 // ever see finite poles.  The reference checks nothing (svd.cpp:134-195).
-__global__ __launch_bounds__(256) void dc_guard_kernel(ProbList pl)
+// Before the D&C of each problem: a NaN / infinite entry of d or e sets info = -1 and the
+// problem is solved as the zero matrix.  With a tiling blob (entry point (b), one problem),
+// the blob must describe the leaf partition of N with leaves of ls columns (header: magic
+// "PACO", N, leaf count, leaf offset 64, leaf size; then {start, end} per leaf); otherwise
+// info = -3.  The leaves are always taken from that partition, never from the blob, so a
+// blob made for another N or by another tool cannot steer writes.
+__global__ __launch_bounds__(256) void dc_guard_kernel(ProbList pl, const uint32_t *blob, int ls)
 {
     const ProbDesc P = pl.get(blockIdx.x);
     const int N = P.N;
-    __shared__ int bad;
-    if (threadIdx.x == 0) bad = 0;
+    __shared__ int bad, blob_bad;
+    if (threadIdx.x == 0) {
+        bad = 0;
+        blob_bad = 0;
+        if (blob) {
+            const int L = leaf_count(N, ls);
+            blob_bad = blob[0] != 0x4F434150u || blob[2] != (uint32_t)N || blob[3] != (uint32_t)L || blob[4] != 64u ||
+                       blob[6] != (uint32_t)ls;
+        }
+    }
     __syncthreads();
+    if (blob && !blob_bad) { // the header vouches for leaf_count(N, ls) entries
+        int bb = 0;
+        for (int l = threadIdx.x; l < leaf_count(N, ls); l += 256)
+            bb |= blob[16 + 2 * l] != (uint32_t)leaf_start(N, l, ls) || blob[17 + 2 * l] != (uint32_t)leaf_start(N, l + 1, ls);
+        if (bb) blob_bad = 1;
+    }
     int b = 0;
     for (int i = threadIdx.x; i < N; i += 256) b |= !isfinite(P.d[i]);
     for (int i = threadIdx.x; i < N - 1; i += 256) b |= !isfinite(P.e[i]);
     if (b) bad = 1;
     __syncthreads();
-    if (!bad) return;
+    if (!bad) {
+        if (threadIdx.x == 0 && blob_bad) *P.info = -3;
+        return;
+    }
     for (int i = threadIdx.x; i < N; i += 256) P.d[i] = 0.f;
     for (int i = threadIdx.x; i < N - 1; i += 256) P.e[i] = 0.f;
     if (threadIdx.x == 0) *P.info = -1;
@@ -82,7 +106,8 @@ struct WsLayout {
 // matrices that take the chip-wide bidiagonalization and the GEMM formation of Q, P
 static bool large_matrix(int M, int N) { return N > BIDIAG_WG_MAXN || (size_t)M * N >= (size_t)1024 * 1024; }
 
-static int total_merges(int N) { return leaf_count(N) - 1; }
+// merges of the D&C tree: most with the smallest leaves (scratch is sized for those)
+static int total_merges(int N) { return leaf_count(N, LEAF_MIN) - 1; }
 
 static WsLayout ws_layout(int M, int N)
 {
@@ -146,6 +171,8 @@ static void *internal_ws(hipStream_t s, size_t bytes);
 
 // iteration limit of the secular solver (paco_set_secular_maxiter)
 static int g_secular_maxit = 200;
+// columns per D&C leaf (paco_set_leaf_size); GenerateTiling describes this partition
+static std::atomic<int> g_leaf_size{PACO_LEAF_DEFAULT};
 
 static bool use_grid_bidiag(int M, int N)
 {
@@ -301,6 +328,7 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
 struct DcOffsets {
     const int *leaf_off = nullptr;
     int nleaf = 0;
+    int ls = LEAF_MIN; // columns per leaf
     std::vector<const int *> task_off; // per round
     std::vector<const int *> task_prob; // per round (empty: search task_off)
     std::vector<int> ntask, maxn;
@@ -310,9 +338,9 @@ struct DcOffsets {
     int ntu = 0, ntv = 0, nru = 0, nrv = 0;
 };
 
-static hipError_t run_dc_levels(ProbList pl, const DcOffsets &o, const uint32_t *blob_leaves, hipStream_t s)
+static hipError_t run_dc_levels(ProbList pl, const DcOffsets &o, hipStream_t s)
 {
-    hipError_t e = launch_dc_leaves(pl, o.leaf_off, blob_leaves, o.nleaf, s);
+    hipError_t e = launch_dc_leaves(pl, o.leaf_off, o.nleaf, o.ls, s);
     if (e != hipSuccess) return e;
     for (size_t r = 1; r <= o.ntask.size(); ++r) {
         LevelArgs la;
@@ -321,10 +349,11 @@ static hipError_t run_dc_levels(ProbList pl, const DcOffsets &o, const uint32_t 
         la.states = o.states[r - 1];
         la.ntask = o.ntask[r - 1];
         la.r = (int)r;
-        // every merge of the round has at most min(3 2^r, N) rows and columns: the
+        // every merge of the round has at most min(ls 2^r, N) rows and columns: the
         // launches give each merge that many threads / tiles
         la.maxn = o.maxn[r - 1];
-        la.stride = std::max(1, std::min(3 << r, la.maxn));
+        la.stride = std::max(1, std::min(o.ls << r, la.maxn));
+        la.ls = o.ls;
         la.maxit = g_secular_maxit;
         la.vonly = o.vonly ? 1 : 0;
         if ((e = launch_dc_level(pl, la, s)) != hipSuccess) return e;
@@ -337,9 +366,9 @@ static hipError_t run_dc_backtransform(ProbList pl, const DcOffsets &o, hipStrea
     return launch_dc_backtransform(pl, o.bt_tu, o.ntu, o.bt_tv, o.ntv, o.bt_ru, o.nru, o.bt_rv, o.nrv, s);
 }
 
-static hipError_t run_dc(ProbList pl, const DcOffsets &o, const uint32_t *blob_leaves, hipStream_t s)
+static hipError_t run_dc(ProbList pl, const DcOffsets &o, hipStream_t s)
 {
-    hipError_t e = run_dc_levels(pl, o, blob_leaves, s);
+    hipError_t e = run_dc_levels(pl, o, s);
     if (e != hipSuccess) return e;
     return run_dc_backtransform(pl, o, s);
 }
@@ -358,31 +387,31 @@ static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, floa
     P.qt = qt;
     P.wt = wt;
     P.idx = idx;
-    hipLaunchKernelGGL(dc_guard_kernel, dim3(1), dim3(256), 0, s, pl);
+    const int ls = g_leaf_size.load();
+    hipLaunchKernelGGL(dc_guard_kernel, dim3(1), dim3(256), 0, s, pl, static_cast<const uint32_t *>(tiling_dev), ls);
     DcOffsets o;
     o.vonly = vonly;
-    o.nleaf = leaf_count(N);
+    o.ls = ls;
+    o.nleaf = leaf_count(N, ls);
     MergeState *st = reinterpret_cast<MergeState *>(ws + L.states);
-    const int R = rounds_of(N);
+    const int R = rounds_of(N, ls);
     for (int r = 1; r <= R; ++r) {
         o.task_off.push_back(nullptr);
-        o.ntask.push_back(merges_in_round(N, r));
-        o.maxn.push_back(std::min(3 << r, N));
+        o.ntask.push_back(merges_in_round(N, r, ls));
+        o.maxn.push_back(std::min(ls << r, N));
         o.states.push_back(st);
-        st += merges_in_round(N, r);
+        st += merges_in_round(N, r, ls);
     }
     o.ntu = cdiv(M, 64) * cdiv(N, 64);
     o.ntv = cdiv(N, 64) * cdiv(N, 64);
     o.nru = cdiv(M, ROW_GROUP);
     o.nrv = cdiv(N, ROW_GROUP);
-    const uint32_t *blob = nullptr;
-    if (tiling_dev) blob = reinterpret_cast<const uint32_t *>(static_cast<const char *>(tiling_dev) + 64);
     const bool tm = g_timing;
     if (tm) {
         g_tev_recorded = true;
         tmark(s, 3, tm);
     }
-    hipError_t err = run_dc_levels(pl, o, blob, s);
+    hipError_t err = run_dc_levels(pl, o, s);
     if (err != hipSuccess) return (int)err;
     tmark(s, 4, tm);
     if (!vonly) err = run_dc_backtransform(pl, o, s);
@@ -465,6 +494,7 @@ struct BatchPlan {
     std::vector<int> n_qext[2], n_qg1[2], n_qg3[2];
     bool fits = true;
     bool vonly = false; // singular values only: no Q, P^T, back-transforms or left vectors
+    int ls = LEAF_MIN;  // columns per D&C leaf
 };
 
 template <class T>
@@ -506,6 +536,7 @@ static double svd_cost(int M, int N) { return 4.0 * M * M * N + 2.0 * M * N * N 
 
 static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::vector<size_t> &scr_off)
 {
+    bp.ls = g_leaf_size.load();
     bp.count = count;
     scr_off.resize(count);
     size_t so = 0;
@@ -641,9 +672,10 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         const int nt = cdiv(jobs[i].N, 32);
         return nt * (nt + 1) / 2;
     }, bp.ntp));
-    bp.o_leaf = blob_put(b, prefix(count, [&](int i) { return leaf_count(jobs[i].N); }, bp.nleaf));
+    const int ls = bp.ls;
+    bp.o_leaf = blob_put(b, prefix(count, [&](int i) { return leaf_count(jobs[i].N, ls); }, bp.nleaf));
     int R = 0;
-    for (int i = 0; i < count; ++i) R = std::max(R, rounds_of(jobs[i].N));
+    for (int i = 0; i < count; ++i) R = std::max(R, rounds_of(jobs[i].N, ls));
     bp.o_task.clear();
     bp.o_tprob.clear();
     bp.ntask.clear();
@@ -652,16 +684,16 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     size_t sacc = 0;
     for (int r = 1; r <= R; ++r) {
         int nt = 0;
-        bp.o_task.push_back(blob_put(b, prefix(count, [&](int i) { return merges_in_round(jobs[i].N, r); }, nt)));
+        bp.o_task.push_back(blob_put(b, prefix(count, [&](int i) { return merges_in_round(jobs[i].N, r, ls); }, nt)));
         std::vector<int> tprob;
         tprob.reserve(nt);
         for (int i = 0; i < count; ++i)
-            for (int q = merges_in_round(jobs[i].N, r); q > 0; --q) tprob.push_back(i);
+            for (int q = merges_in_round(jobs[i].N, r, ls); q > 0; --q) tprob.push_back(i);
         bp.o_tprob.push_back(blob_put(b, tprob));
         bp.ntask.push_back(nt);
         int mx = 0;
         for (int i = 0; i < count; ++i)
-            if (merges_in_round(jobs[i].N, r) > 0) mx = std::max(mx, std::min(3 << r, jobs[i].N));
+            if (merges_in_round(jobs[i].N, r, ls) > 0) mx = std::max(mx, std::min(ls << r, jobs[i].N));
         bp.maxn.push_back(mx);
         bp.state_off.push_back(sacc);
         sacc += nt;
@@ -863,6 +895,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     tmark(s, 3, tm);
     DcOffsets o;
     o.vonly = bp.vonly;
+    o.ls = bp.ls;
     o.leaf_off = I(bp.o_leaf);
     o.nleaf = bp.nleaf;
     MergeState *st = reinterpret_cast<MergeState *>(dev + pr.o_states);
@@ -881,8 +914,8 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     o.nru = bp.nru;
     o.bt_rv = I(bp.o_rv);
     o.nrv = bp.nrv;
-    hipLaunchKernelGGL(dc_guard_kernel, dim3(count), dim3(256), 0, s, pl);
-    if ((e = run_dc_levels(pl, o, nullptr, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(dc_guard_kernel, dim3(count), dim3(256), 0, s, pl, nullptr, bp.ls);
+    if ((e = run_dc_levels(pl, o, s)) != hipSuccess) return e;
     tmark(s, 4, tm);
     if (!bp.vonly) e = run_dc_backtransform(pl, o, s);
     tmark(s, 5, tm);
@@ -948,6 +981,7 @@ static uint64_t batch_signature(const paco_svd_job *jobs, int count)
     }
     mix((uint64_t)gemm_get_mode());
     mix((uint64_t)g_secular_maxit);
+    mix((uint64_t)g_leaf_size.load());
     return h;
 }
 
@@ -1541,6 +1575,14 @@ void paco_release(void)
     g_streams.clear();
     ++g_release_gen;
 }
+
+int paco_set_leaf_size(int ls)
+{
+    if (ls >= LEAF_MIN && ls <= LEAF_MAX) return g_leaf_size.exchange(ls);
+    return g_leaf_size.load();
+}
+
+int paco_leaf_size(void) { return g_leaf_size.load(); }
 
 int paco_set_secular_maxiter(int maxit)
 {

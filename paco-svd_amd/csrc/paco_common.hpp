@@ -259,6 +259,13 @@ __host__ __device__ inline int find_task(const int *offs, int ntask, int x)
 
 __host__ __device__ inline int off_at(const int *offs, int i) { return offs ? offs[i] : 0; }
 
+// Count one non-converged secular root in a problem's status word unless the word already
+// holds a negative code (bad input), which is kept as it is.
+__device__ inline void info_count_failure(int *info)
+{
+    if (atomicAdd(info, 1) < 0) atomicSub(info, 1);
+}
+
 // Problem of task x for offsets that the batch plan writes as count + 1 prefix entries
 // followed by one entry per task (its problem): one load instead of a binary search.  A
 // single problem needs neither (offs may be null).
@@ -267,39 +274,43 @@ __host__ __device__ inline int task_of(const int *offs, int count, int x)
     return (count <= 1 || !offs) ? 0 : offs[count + 1 + x];
 }
 
-// Leaf partition of N columns (svd_tiling.cpp:83-130): L = N/3 + 1 leaves for N >= 3,
-// leaf l starts at 3l; the last starts one column early when N % 3 == 0.  N <= 2 is
-// one leaf.  Leaf l covers [leaf_start(l), leaf_start(l+1)) with leaf_start(L) = N.
-__host__ __device__ inline int leaf_count(int N) { return N <= 2 ? 1 : N / 3 + 1; }
-__host__ __device__ inline int leaf_start(int N, int l)
+// Leaf partition of N columns with leaves of ls columns (svd_tiling.cpp:83-130 is the
+// case ls = 3): L = N/ls + 1 leaves for N >= ls, leaf l starts at ls l; the last starts
+// one column early when N % ls == 0.  N < ls is one leaf.  Leaf l covers
+// [leaf_start(l), leaf_start(l+1)) with leaf_start(L) = N.  Every leaf but the last is
+// an (m - 1) x m upper bidiagonal block (m <= ls columns), the last is square (m < ls).
+constexpr int LEAF_MIN = 3, LEAF_MAX = 32;
+__host__ __device__ inline int leaf_count(int N, int ls) { return N < ls ? 1 : N / ls + 1; }
+__host__ __device__ inline int leaf_start(int N, int l, int ls)
 {
-    const int L = leaf_count(N);
+    const int L = leaf_count(N, ls);
     if (l >= L) return N;
     if (L == 1) return 0;
-    if (l <= L - 2) return 3 * l;
-    return 3 * l - (N % 3 == 0 ? 1 : 0);
+    if (l <= L - 2) return ls * l;
+    return ls * l - (N % ls == 0 ? 1 : 0);
 }
 // Merges of round r >= 1 (pairs of nodes of round r-1, left to right, odd tail carried;
 // svd.cpp:168-189): node j of round r covers leaves [j 2^r, min((j+1) 2^r, L)); it is
-// a merge when its right half is not empty.
-__host__ __device__ inline int merges_in_round(int N, int r)
+// a merge when its right half is not empty.  A node of round r spans at most ls 2^r
+// columns.
+__host__ __device__ inline int merges_in_round(int N, int r, int ls)
 {
-    const int L = leaf_count(N), span = 1 << r, half = span >> 1;
+    const int L = leaf_count(N, ls), span = 1 << r, half = span >> 1;
     return L > half ? (L - half + span - 1) / span : 0;
 }
-__host__ __device__ inline int rounds_of(int N)
+__host__ __device__ inline int rounds_of(int N, int ls)
 {
-    int L = leaf_count(N), r = 0;
+    int L = leaf_count(N, ls), r = 0;
     while ((1 << r) < L) ++r;
     return r;
 }
-__host__ __device__ inline void merge_cols(int N, int r, int j, int &c0, int &c1, int &c2)
+__host__ __device__ inline void merge_cols(int N, int r, int j, int ls, int &c0, int &c1, int &c2)
 {
-    const int L = leaf_count(N), span = 1 << r, half = span >> 1;
+    const int L = leaf_count(N, ls), span = 1 << r, half = span >> 1;
     const int l0 = j * span, lm = l0 + half, l1 = (l0 + span < L) ? l0 + span : L;
-    c0 = leaf_start(N, l0);
-    c1 = leaf_start(N, lm);
-    c2 = leaf_start(N, l1);
+    c0 = leaf_start(N, l0, ls);
+    c1 = leaf_start(N, lm, ls);
+    c2 = leaf_start(N, l1, ls);
 }
 
 } // namespace paco

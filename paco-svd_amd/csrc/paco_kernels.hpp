@@ -27,6 +27,7 @@ struct LevelArgs {
     int maxit; // iteration limit of the secular solver per root (a root that reaches it adds 1 to info)
     int vonly; // singular values only: nodes keep the first and last columns of their right
                // vectors (all a merge reads), no left vectors
+    int ls;    // leaf size of the partition (columns per leaf, LEAF_MIN..LEAF_MAX)
 };
 
 struct CopyDesc {
@@ -100,8 +101,8 @@ bool bidiag_grid_supported(int M, int N);
 hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int grid = 0);
 
 // dc.hip
-hipError_t launch_dc_leaves(ProbList pl, const int *leaf_off, const uint32_t *blob_leaves, int nleaf,
-                            hipStream_t s);
+// leaves of ls columns (leaf_off: prefix of the leaf counts over the problems, or null)
+hipError_t launch_dc_leaves(ProbList pl, const int *leaf_off, int nleaf, int ls, hipStream_t s);
 hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s);
 // back-transform: U[:, :N] <- U[:, :N] qt^T and Vt <- wt Vt for every problem
 hipError_t launch_dc_backtransform(ProbList pl, const int *tile_off_u, int ntiles_u, const int *tile_off_v,

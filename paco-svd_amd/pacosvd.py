@@ -15,6 +15,7 @@ EXPORTS = [
     "paco_workspace_size", "paco_set_precision", "paco_svd_plain", "paco_svd_batched",
     "paco_svd_batched_graph", "paco_svd_values_batched", "paco_sgesdd", "paco_svd_plain_noalloc", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
     "paco_timing_enable", "paco_timing_read", "paco_info_offset", "paco_set_secular_maxiter",
+    "paco_set_leaf_size", "paco_leaf_size",
     "paco_server_create", "paco_server_submit", "paco_server_poll", "paco_server_drain", "paco_server_stats",
     "paco_server_destroy",
 ]
@@ -58,6 +59,8 @@ def lib():
     L.paco_info_offset.argtypes = [ctypes.c_int, ctypes.c_int]
     L.paco_info_offset.restype = ctypes.c_size_t
     L.paco_set_secular_maxiter.argtypes = [ctypes.c_int]
+    L.paco_set_leaf_size.argtypes = [ctypes.c_int]
+    L.paco_leaf_size.argtypes = []
     L.paco_gemm_f32.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_float, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, _vp, ctypes.c_int]
     L.paco_timing_enable.argtypes = [ctypes.c_int]
@@ -123,8 +126,9 @@ def upper_bidiagonalization(a, ws=None):
     return dict(a=a, u=u, vt=vt, d=d, e=e[: N - 1], tauq=tq, taup=tp[: N - 1], ws=ws)
 
 
-def svd_DC(M, N, a, u, vt, d, e, ws=None, use_tiling=True):
-    """Entry point (b): d <- sigma, u, vt <- singular vectors (in place)."""
+def svd_DC(M, N, a, u, vt, d, e, ws=None, use_tiling=True, blob=None):
+    """Entry point (b): d <- sigma, u, vt <- singular vectors (in place).  The tiling blob
+    is GenerateTiling's for N unless blob (bytes) is given."""
     import torch
     dev = u.device
     qt = torch.zeros(2 * N * N, dtype=torch.float32, device=dev)
@@ -134,7 +138,7 @@ def svd_DC(M, N, a, u, vt, d, e, ws=None, use_tiling=True):
         ws = workspace(M, N, dev)
     tiling = None
     if use_tiling:
-        blob = tiling_blob(N)
+        blob = tiling_blob(N) if blob is None else blob
         tiling = torch.frombuffer(bytearray(blob), dtype=torch.uint8).to(dev)
     if e.numel() < max(N - 1, 1):
         e = torch.zeros(1, dtype=torch.float32, device=dev)
@@ -234,6 +238,12 @@ def gemm(A, B, C=None, transA=False, transB=False, alpha=1.0, beta=0.0):
 
 def set_precision(mode):
     return lib().paco_set_precision(mode)
+
+
+def set_leaf_size(ls):
+    """Columns per divide-and-conquer leaf (3..32) for later calls; returns the previous
+    size (values outside 3..32 only query)."""
+    return lib().paco_set_leaf_size(ls)
 
 
 def set_secular_maxiter(maxit):

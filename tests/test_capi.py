@@ -33,13 +33,50 @@ def test_library_exports_every_declared_symbol():
 
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8, 96, 97, 98, 4096])
 def test_tiling_blob_matches_reference_partition(N):
-    blob = pacosvd.tiling_blob(N)
-    assert len(blob) == pacosvd.lib().getTilingSize(N, 1)
-    magic, version, n, cnt, off, nb = struct.unpack("<6I", blob[:24])
-    assert magic == 0x4F434150 and version == 1 and n == N and off == 64 and nb == 1
+    old = pacosvd.set_leaf_size(3)  # leaves of 3 columns: the reference's partition
+    try:
+        assert blob_leaves(N, 1, 3) == oracle.leaf_partition(N)
+    finally:
+        pacosvd.set_leaf_size(old)
+
+
+def test_default_leaf_size():
+    assert pacosvd.set_leaf_size(0) == 6
+
+
+def blob_leaves(N, nb, ls):
+    blob = pacosvd.tiling_blob(N, nb)
+    assert len(blob) == pacosvd.lib().getTilingSize(N, nb)
+    magic, version, n, cnt, off, nbl, lsz = struct.unpack("<7I", blob[:28])
+    assert magic == 0x4F434150 and version == 2 and n == N and off == 64 and nbl == nb and lsz == ls
     leaves = struct.unpack("<%dI" % (2 * cnt), blob[off:off + 8 * cnt])
-    got = [(leaves[2 * i], leaves[2 * i + 1]) for i in range(cnt)]
-    assert got == oracle.leaf_partition(N)
+    return [(leaves[2 * i], leaves[2 * i + 1]) for i in range(cnt)]
+
+
+@pytest.mark.parametrize("ls", [4, 5, 8, 16, 31, 32])
+def test_tiling_blob_larger_leaves(ls):
+    """Leaves of ls columns: they tile [0, N) in order; every leaf but the last has ls
+    columns (ls - 1 for the one before a last leaf that starts a column early), so
+    N // ls + 1 leaves for N >= ls; the last, square, leaf has 1 .. ls - 1 columns."""
+    old = pacosvd.set_leaf_size(ls)
+    try:
+        assert pacosvd.set_leaf_size(0) == ls
+        for N in list(range(1, 3 * ls + 3)) + [96, 97, 511, 512, 4096, 4097]:
+            got = blob_leaves(N, 1, ls)
+            assert got[0][0] == 0 and got[-1][1] == N, (N, got)
+            assert all(a[1] == b[0] for a, b in zip(got, got[1:])), (N, got)
+            assert all(0 < b - a for a, b in got)
+            if N < ls:
+                assert got == [(0, N)]
+                continue
+            assert len(got) == N // ls + 1
+            assert all(b - a == ls for a, b in got[:-2])
+            assert got[-2][1] - got[-2][0] == (ls - 1 if N % ls == 0 else ls)
+            assert 1 <= got[-1][1] - got[-1][0] <= ls - 1
+    finally:
+        pacosvd.set_leaf_size(old)
+    # out-of-range sizes only query
+    assert pacosvd.set_leaf_size(2) == old and pacosvd.set_leaf_size(33) == old
 
 
 def test_workspace_size():
