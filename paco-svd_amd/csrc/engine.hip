@@ -223,6 +223,16 @@ struct BidiagChoice {
     size_t lds;
     bool resident;
 };
+// Streamed matrices at 1024 threads take the blocked kernel (bidiag_blk.hip) when
+// PACO_BIDIAG_BLOCKED=1.  It writes the trailing matrix once per panel instead of once per
+// column, but issues more vector instructions than the one-pass-per-column kernel, which
+// is bound by instruction issue rather than memory traffic, so the latter stays the default.
+static bool blocked_bidiag(int M, int N, int nt)
+{
+    const char *env = getenv("PACO_BIDIAG_BLOCKED");
+    return env && env[0] == '1' && nt == 1024 && bidiag_blk_supported(M, N);
+}
+
 static BidiagChoice bidiag_choice(int M, int N)
 {
     BidiagChoice c;
@@ -295,7 +305,10 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
         err = launch_bidiag_grid(pl.single, ws + L.grid, s, grid);
     } else {
         const BidiagChoice c = bidiag_choice(M, N);
-        err = launch_bidiag_wg(pl, nullptr, 1, c.lds, s, c.nt);
+        if (!c.resident && blocked_bidiag(M, N, c.nt))
+            err = launch_bidiag_blk(pl, nullptr, 1, std::max(c.lds, bidiag_blk_lds_bytes(M, N)), s);
+        else
+            err = launch_bidiag_wg(pl, nullptr, 1, c.lds, s, c.nt);
     }
     if (err != hipSuccess) return (int)err;
     tmark(s, 2, tm);
@@ -469,6 +482,10 @@ struct BatchPlan {
     // streamed bidiagonalization: one launch per workgroup size (256, 512, 1024)
     size_t o_order_nt[3] = {0, 0, 0}, lds_nt[3] = {0, 0, 0};
     int n_nt[3] = {0, 0, 0};
+    // streamed matrices of the blocked kernel (bidiag_blk.hip); LDS beyond its vectors
+    // caches bottom rows
+    size_t o_order_blk = 0, lds_blk = 0;
+    int n_blk = 0;
     // one wavefront per LDS-resident matrix: one launch per LDS class (the dynamic LDS
     // of a launch is its largest member's, which sets the occupancy of all)
     // and per chunk count C = ceil(N / 64) (the kernel is specialised on it)
@@ -565,7 +582,7 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         return svd_cost(jobs[x].M, jobs[x].N) > svd_cost(jobs[y].M, jobs[y].N);
     });
     // a matrix's launch shape (and so its arithmetic) depends on its shape only
-    std::vector<int> og[3], orr[BatchPlan::kWaveClasses], ow[BatchPlan::kWaveGroups];
+    std::vector<int> og[3], ob, orr[BatchPlan::kWaveClasses], ow[BatchPlan::kWaveGroups];
     // LDS class of a resident matrix: 0..3 for <= 1/8, 1/4, 1/2, all of the limit
     auto lds_class = [](size_t bytes, size_t lim) {
         int cl = 0;
@@ -615,6 +632,9 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
             const int g = nt >= 1024 ? 2 : nt >= 512 ? 1 : 0;
             og[g].push_back(i);
             bp.lds_nt[g] = std::max(bp.lds_nt[g], bidiag_wg_lds_limit());
+        } else if (blocked_bidiag(jobs[i].M, jobs[i].N, c.nt)) {
+            ob.push_back(i);
+            bp.lds_blk = std::max(bp.lds_blk, std::max(c.lds, bidiag_blk_lds_bytes(jobs[i].M, jobs[i].N)));
         } else {
             const int g = c.nt >= 1024 ? 2 : c.nt >= 512 ? 1 : 0;
             og[g].push_back(i);
@@ -625,6 +645,8 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         bp.n_nt[g] = (int)og[g].size();
         bp.o_order_nt[g] = blob_put(b, og[g]);
     }
+    bp.n_blk = (int)ob.size();
+    bp.o_order_blk = blob_put(b, ob);
     for (int cl = 0; cl < BatchPlan::kWaveClasses; ++cl) {
         bp.n_res[cl] = (int)orr[cl].size();
         bp.o_order_res[cl] = blob_put(b, orr[cl]);
@@ -859,6 +881,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
                 return e;
         if (!serial && (e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
     }
+    if ((e = launch_bidiag_blk(pl, I(bp.o_order_blk), bp.n_blk, bp.lds_blk, s)) != hipSuccess) return e;
     for (int g = 2; g >= 0; --g)
         if ((e = launch_bidiag_wg(pl, I(bp.o_order_nt[g]), bp.n_nt[g], bp.lds_nt[g], s, 256 << g)) != hipSuccess)
             return e;

@@ -63,6 +63,13 @@ size_t bidiag_wg_lds_resident_bytes(int M, int N, int nt = 256); // whole matrix
 size_t bidiag_wg_lds_limit();
 constexpr int BIDIAG_WG_MAXN = 512; // widest matrix of the one-workgroup kernel
 hipError_t launch_bidiag_wg(ProbList pl, const int *order, int count, size_t lds, hipStream_t s, int nt = 256);
+// bidiag_blk.hip: blocked one-workgroup bidiagonalization (panels of BK_NB columns, the
+// trailing matrix written once per panel) for streamed matrices, N <= 512, M <= 1024
+constexpr int BK_NB = 8;
+constexpr int BIDIAG_BLK_MAXM = 1024;
+size_t bidiag_blk_lds_bytes(int M, int N); // vectors only; the rest of the LDS caches rows
+bool bidiag_blk_supported(int M, int N);
+hipError_t launch_bidiag_blk(ProbList pl, const int *order, int count, size_t lds, hipStream_t s);
 // one wavefront per matrix, whole matrix in LDS (batched path), N <= 64 * BIDIAG_WAVE_MAXC
 constexpr int BIDIAG_WAVE_MAXC = 4;
 size_t bidiag_wave_lds(int M, int N);

@@ -306,7 +306,8 @@ def test_batched_graph_replays(torch_dev):
 @pytest.mark.parametrize("threads", ["256", "512", "1024"])
 def test_bidiag_variants_bitwise_equal(torch_dev, threads):
     """The LDS-resident and the streamed one-workgroup bidiagonalization run the same
-    arithmetic in the same order at one workgroup size: their outputs must be identical."""
+    arithmetic in the same order at one workgroup size: their outputs must be identical.
+    (The blocked kernel, another order of operations, is kept off for the streamed run.)"""
     torch = torch_dev
     import pacosvd
     rng = np.random.default_rng(21)
@@ -315,7 +316,7 @@ def test_bidiag_variants_bitwise_equal(torch_dev, threads):
     As = [torch.from_numpy(m).cuda() for m in mats]
     runs = []
     for env in ({"PACO_RES_THREADS": threads},
-                {"PACO_LDS_RESIDENT_MAX": "0", "PACO_BIDIAG_THREADS": threads}):
+                {"PACO_LDS_RESIDENT_MAX": "0", "PACO_BIDIAG_THREADS": threads, "PACO_BIDIAG_BLOCKED": "0"}):
         os.environ.update(env)
         try:
             runs.append(pacosvd.svd_batched(As))
@@ -363,12 +364,15 @@ def test_bidiag_one_wavefront(torch_dev):
         assert np.max(np.abs(s - s2)) <= SIG_TOL * max(float(s2[0]), 1e-30), shapes[i]
 
 
-@pytest.mark.parametrize("threads", ["512", "1024"])
+@pytest.mark.parametrize("threads", ["512", "1024", "blocked"])
 def test_bidiag_tiers_bitwise_equal(torch_dev, threads):
     """Rows streamed from global memory or cached in LDS take the same arithmetic in the
     same order: every split of a matrix between the two gives the same bits, including
     the re-packing of the cached rows as the trailing matrix narrows (all of the LDS and
-    64 KiB re-pack the larger shapes here; 0 bytes caches no row)."""
+    64 KiB re-pack the larger shapes here; 0 bytes caches no row).  "blocked": the panel
+    kernel (1024 threads, PACO_BIDIAG_BLOCKED=1), whose cached rows are not re-packed."""
+    blocked = threads == "blocked"
+    threads = "1024" if blocked else threads
     torch = torch_dev
     import pacosvd
     rng = np.random.default_rng(5)
@@ -383,13 +387,15 @@ def test_bidiag_tiers_bitwise_equal(torch_dev, threads):
     for lds in variants:
         os.environ["PACO_BIDIAG_THREADS"] = threads
         os.environ["PACO_LDS_RESIDENT_MAX"] = "0"
+        if blocked:
+            os.environ["PACO_BIDIAG_BLOCKED"] = "1"
         if lds is not None:
             os.environ["PACO_BIDIAG_LDS"] = lds
         try:
             results.append(pacosvd.svd_batched(As))
             torch.cuda.synchronize()
         finally:
-            for k in ("PACO_BIDIAG_THREADS", "PACO_LDS_RESIDENT_MAX", "PACO_BIDIAG_LDS"):
+            for k in ("PACO_BIDIAG_THREADS", "PACO_LDS_RESIDENT_MAX", "PACO_BIDIAG_LDS", "PACO_BIDIAG_BLOCKED"):
                 os.environ.pop(k, None)
     for i, A in enumerate(mats):
         m = measures(A, *(np_of(x) for x in results[0][i]))
@@ -509,8 +515,10 @@ def test_grid_persistent_matches_launch_per_phase(torch_dev, shape):
     assert m["resid_ratio"] <= RESID_TOL
 
 
-@pytest.mark.parametrize("shape", [(128, 96), (300, 200), (512, 512), (90, 1), (1100, 1000), (2048, 700)])
-def test_entry_a_outputs_vs_oracle(torch_dev, shape):
+@pytest.mark.parametrize("blocked", [False, True])
+@pytest.mark.parametrize("shape", [(128, 96), (300, 200), (512, 512), (90, 1), (1100, 1000), (2048, 700),
+                                   (1024, 450), (700, 512), (257, 240), (520, 17)])
+def test_entry_a_outputs_vs_oracle(torch_dev, shape, blocked):
     """Every output of entry point (a) element-wise against the FP64 oracle: d, e, tauq,
     taup and the whole of a (d / e on the band, the column reflectors below it, the row
     reflectors right of it; upper_bidiagonalization.cpp:204,271-282).  Golub-Kahan
@@ -523,9 +531,14 @@ def test_entry_a_outputs_vs_oracle(torch_dev, shape):
     A = (rng.uniform(-1, 1, shape) if shape == (128, 96) else rng.standard_normal(shape)).astype(np.float32)
     if shape == (128, 96):
         A = np.load(os.path.join(GOLD, "c1.npz"))["A"]
-    say("entry (a) outputs", shape)
-    r = pacosvd.upper_bidiagonalization(torch.from_numpy(A.copy()).cuda())
-    torch.cuda.synchronize()
+    say("entry (a) outputs", shape, "blocked" if blocked else "")
+    if blocked:  # the panel kernel for the shapes it takes (streamed, N <= 512, M <= 1024)
+        os.environ["PACO_BIDIAG_BLOCKED"] = "1"
+    try:
+        r = pacosvd.upper_bidiagonalization(torch.from_numpy(A.copy()).cuda())
+        torch.cuda.synchronize()
+    finally:
+        os.environ.pop("PACO_BIDIAG_BLOCKED", None)
     g = {k: np_of(v).astype(np.float64) for k, v in r.items() if k in ("a", "d", "e", "tauq", "taup")}
     o = oracle.bidiag(A.astype(np.float64))
     f = oracle.bidiag_f32(A)
