@@ -305,11 +305,23 @@ def main():
     ap.add_argument("--leaf-size", type=int, default=0, help="columns per D&C leaf (0: the library's default)")
     ap.add_argument("--profile-batched", action="store_true",
                     help="only run the batched step (warmup + steps), for counter collection")
+    ap.add_argument("--profile-plain-large", action="store_true",
+                    help="only run the 4096 x 4096 plain SVD (one warm-up, one measured), for profiles")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: independent SVDs shard by matrix across GPUs with no "
               "collective (see DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.profile_plain_large:
+        import torch
+        import pacosvd
+        pacosvd.lib()
+        A = torch.from_numpy(workloads.c2_matrix()).cuda()
+        single_svd(torch, pacosvd, A)
+        _, _, _, ph = single_svd(torch, pacosvd, A)
+        print(json.dumps({"profile_plain_4096_phase_ms": {k: round(v, 3) for k, v in ph.items()}}), flush=True)
+        return
 
     print("[bench] generating workload", flush=True)
     host_data = workloads.c4_host(args.count)
