@@ -881,26 +881,28 @@ static void launch_trail(const ProbDesc &P, const GridWs &W, int q, int p, hipSt
     hipLaunchKernelGGL(gb_trail_kernel<MODE>, dim3(tiles), dim3(GT), 0, s, t, tn);
 }
 
-// columns left to the one-workgroup kernel: PACO_GRID_SWITCH overrides (NB .. 512)
-static int grid_switch()
+// columns left to the one-workgroup kernel: PACO_GRID_SWITCH, else the autotuner's table
+// for the (M, N) bucket, else SWITCH_DEFAULT (clamped to NB .. 512)
+static int grid_switch(int M, int N)
 {
     const char *e = getenv("PACO_GRID_SWITCH");
-    const int v = e ? atoi(e) : SWITCH_DEFAULT;
+    const int v = e ? atoi(e) : tuned_grid_switch(M, N, SWITCH_DEFAULT);
     return v < NB ? NB : v > 512 ? 512 : v;
 }
 
 // first column of the one-workgroup finish: panels of NB columns while more than the
 // switch count of columns remain
-static int grid_tail_start(int N)
+static int grid_tail_start(int M, int N)
 {
+    const int sw = grid_switch(M, N);
     int p = 0;
-    while (N - p > grid_switch()) p += NB;
+    while (N - p > sw) p += NB;
     return p;
 }
 
 bool bidiag_grid_supported(int M, int N)
 {
-    const int p = grid_tail_start(N);
+    const int p = grid_tail_start(M, N);
     return bidiag_wg_lds_bytes(M - p, N - p, 1024) <= bidiag_wg_lds_limit();
 }
 
@@ -909,11 +911,12 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int gr
     const int M = P.M, N = P.N;
     GridWs W;
     carve(M, N, static_cast<char *>(ws), &W);
-    const int p_tail = grid_tail_start(N);
+    const int p_tail = grid_tail_start(M, N);
     if (bidiag_wg_lds_bytes(M - p_tail, N - p_tail, 1024) > bidiag_wg_lds_limit()) return hipErrorInvalidValue;
-    // tile grid: PACO_GRID overrides (4 .. GMAX); PACO_GRID_PERSIST=0: two launches per column
+    // tile grid: the caller's, else PACO_GRID, else the autotuner's table, else 16 (4 .. GMAX);
+    // PACO_GRID_PERSIST=0: two launches per column
     const char *genv = getenv("PACO_GRID");
-    int G = grid > 0 ? grid : genv ? atoi(genv) : 16;
+    int G = grid > 0 ? grid : genv ? atoi(genv) : tuned_grid_tiles(M, N, 16);
     G = G < 4 ? 4 : G > GMAX ? GMAX : G;
     const char *penv = getenv("PACO_GRID_PERSIST");
     const bool persist = !(penv && penv[0] == '0');

@@ -131,7 +131,6 @@ static WsLayout ws_layout(int M, int N)
     return L;
 }
 
-int tuned_bidiag_threads(int M, int N);
 
 static std::mutex g_mu;
 // bumped by paco_release: per-thread pointers into freed pools compare against it
@@ -171,8 +170,15 @@ static void *internal_ws(hipStream_t s, size_t bytes);
 
 // iteration limit of the secular solver (paco_set_secular_maxiter)
 static int g_secular_maxit = 200;
-// columns per D&C leaf (paco_set_leaf_size); GenerateTiling describes this partition
+// columns per D&C leaf (paco_set_leaf_size; at first use the autotuner's table, else
+// PACO_LEAF_DEFAULT); GenerateTiling describes this partition
 static std::atomic<int> g_leaf_size{PACO_LEAF_DEFAULT};
+static int leaf_size_now()
+{
+    static std::once_flag once;
+    std::call_once(once, [] { g_leaf_size = tuned_leaf_size(PACO_LEAF_DEFAULT); });
+    return g_leaf_size.load();
+}
 
 static bool use_grid_bidiag(int M, int N)
 {
@@ -400,7 +406,7 @@ static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, floa
     P.qt = qt;
     P.wt = wt;
     P.idx = idx;
-    const int ls = g_leaf_size.load();
+    const int ls = leaf_size_now();
     hipLaunchKernelGGL(dc_guard_kernel, dim3(1), dim3(256), 0, s, pl, static_cast<const uint32_t *>(tiling_dev), ls);
     DcOffsets o;
     o.vonly = vonly;
@@ -553,7 +559,7 @@ static double svd_cost(int M, int N) { return 4.0 * M * M * N + 2.0 * M * N * N 
 
 static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::vector<size_t> &scr_off)
 {
-    bp.ls = g_leaf_size.load();
+    bp.ls = leaf_size_now();
     bp.count = count;
     scr_off.resize(count);
     size_t so = 0;
@@ -599,7 +605,7 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     // classes or four comma-separated ones (classes <= 1/8, 1/4, 1/2, all of the LDS);
     // 512,512,1024,1024 measured best (bidiagonalization 59.3 ms against 60.1 at 1024 for
     // all classes; one value for all: 62.1 at 1024, 63.7 at 512, 66.7 at 256)
-    for (int cl = 0; cl < BatchPlan::kWaveClasses; ++cl) bp.res_threads[cl] = kResThreadsDefault[cl];
+    for (int cl = 0; cl < BatchPlan::kWaveClasses; ++cl) bp.res_threads[cl] = tuned_res_threads(cl, kResThreadsDefault[cl]);
     if (const char *tenv = getenv("PACO_RES_THREADS")) {
         const char *p = tenv;
         for (int cl = 0; cl < BatchPlan::kWaveClasses; ++cl) {
@@ -1004,7 +1010,7 @@ static uint64_t batch_signature(const paco_svd_job *jobs, int count)
     }
     mix((uint64_t)gemm_get_mode());
     mix((uint64_t)g_secular_maxit);
-    mix((uint64_t)g_leaf_size.load());
+    mix((uint64_t)leaf_size_now());
     return h;
 }
 
@@ -1601,11 +1607,12 @@ void paco_release(void)
 
 int paco_set_leaf_size(int ls)
 {
+    const int cur = leaf_size_now();
     if (ls >= LEAF_MIN && ls <= LEAF_MAX) return g_leaf_size.exchange(ls);
-    return g_leaf_size.load();
+    return cur;
 }
 
-int paco_leaf_size(void) { return g_leaf_size.load(); }
+int paco_leaf_size(void) { return leaf_size_now(); }
 
 int paco_set_secular_maxiter(int maxit)
 {

@@ -55,6 +55,13 @@ inline size_t allow_dynamic_lds(const void *func)
     return dyn;
 }
 
+// autotune.cpp: the autotuner's table (fallback when the table has no entry)
+int tuned_bidiag_threads(int M, int N);
+int tuned_res_threads(int cls, int fallback);
+int tuned_leaf_size(int fallback);
+int tuned_grid_tiles(int M, int N, int fallback);
+int tuned_grid_switch(int M, int N, int fallback);
+
 // bidiag.hip
 // one-workgroup bidiagonalization with nt threads; lds bytes of dynamic LDS hold the
 // vectors and as many bottom rows of each matrix as fit

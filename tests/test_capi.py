@@ -100,3 +100,23 @@ def test_argument_errors_without_gpu():
     assert L.paco_svd_batched(job, 1, z) == -1
     assert L.paco_svd_plain(job, 1, z) == -1
     assert L.paco_set_precision(0) in (0, 1, 2)
+
+
+def test_autotune_table_loader(tmp_path):
+    """The library reads the autotuner's table at first use: leaf size, grid and resident
+    thread entries are parsed (a fresh process, table named by PACO_AUTOTUNE_TABLE)."""
+    import json
+    import subprocess
+    import sys
+    t = tmp_path / "table.json"
+    t.write_text(json.dumps({"512x512": 512, "leaf_size": 12, "grid_tiles:4096x4096": 12,
+                             "grid_switch:8192x4096": 256, "res_threads:0": 256, "bogus": 7}))
+    code = ("import sys; sys.path.insert(0, %r); import pacosvd; print(pacosvd.lib().paco_leaf_size())"
+            % os.path.join(ROOT, "paco-svd_amd"))
+    env = dict(os.environ, PACO_AUTOTUNE_TABLE=str(t))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "12"
+    # the committed table keeps the built-in default leaf size
+    committed = json.load(open(os.path.join(ROOT, "paco-svd_amd", "autotune_table.json")))
+    assert committed.get("leaf_size", 6) == pacosvd.set_leaf_size(0)
