@@ -208,6 +208,25 @@ def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0, wide_lanes=4):
                 accuracy_vs_fp64_lapack=worst)
 
 
+def merge_figures(stats, ph):
+    """Realised deflation of the batched step's D&C merges (paco_merge_stats) per round and
+    in total: merges, mean merged size n, fraction of poles kept (k / n), kept poles from the
+    left and right children, and the FLOPs of the merge products, dense ([U1 0; 0 U2] as a
+    full matrix, what the merge GEMMs compute) and restricted to the nonzero blocks."""
+    rounds = []
+    for r, s in enumerate(stats):
+        m = max(s["merges"], 1.0)
+        rounds.append(dict(round=r + 1, merges=int(s["merges"]), mean_n=round(s["sum_n"] / m, 1),
+                           kept_frac=round(s["sum_k"] / max(s["sum_n"], 1.0), 4),
+                           mean_k_left=round(s["sum_k_left"] / m, 1), mean_k_right=round(s["sum_k_right"] / m, 1),
+                           dense_gflop=round(s["dense_flops"] / 1e9, 3), block_gflop=round(s["block_flops"] / 1e9, 3)))
+    dense = sum(s["dense_flops"] for s in stats)
+    block = sum(s["block_flops"] for s in stats)
+    return dict(rounds=rounds, dense_tflop=round(dense / 1e12, 4), block_tflop=round(block / 1e12, 4),
+                kept_frac=round(sum(s["sum_k"] for s in stats) / max(sum(s["sum_n"] for s in stats), 1.0), 4),
+                dense_tflops_over_dc_phase=round(dense / max(ph.get("dc", 0.0), 1e-6) / 1e9, 2))
+
+
 def single_svd(torch, pacosvd, A):
     """Both entry points on one device matrix, with per-phase hipEvent times."""
     M, N = A.shape
@@ -358,6 +377,8 @@ def main():
     svds_s = len(As) * args.steps / dt
     ph = {k: float(np.mean([p[k] for p in phase_ms])) for k in phase_ms[0]}
     print("[bench] batched %.1f ms/step, phases %s" % (ms_step, ph), flush=True)
+    merges = merge_figures(pacosvd.merge_stats(), ph)
+    print("[bench] merges %s" % merges, flush=True)
 
     if args.profile_batched:
         print(json.dumps({"profile_batched_ms_per_step": ms_step, "phase_ms": ph}), flush=True)
@@ -472,6 +493,7 @@ def main():
         "plain_4096x4096": plain_large,
         "precision_8192x4096": precision,
         "phase_ms": {k: round(v, 3) for k, v in ph.items()},
+        "dc_merges": merges,
         "achieved_tflops": round(flops / (1000.0 * len(As) / best / 1000.0) / 1e12 / 1.0, 3),
         "accuracy_vs_fp64_lapack": acc,
         "info_nonzero": int(sum(1 for x in info if x != 0)),

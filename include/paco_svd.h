@@ -166,6 +166,18 @@ int paco_last_info(int *info_host, int count, void *stream);
  * bytes (written by the two entry points).  No counterpart in the reference. */
 size_t paco_info_offset(int M, int N);
 
+/* Merges of the last paco_svd_batched / paco_svd_batched_graph call on this stream,
+ * summed per round (merge level) into out[PACO_MERGE_STATS * r + q], r < the returned
+ * count (at most max_rounds): q = 0 merges, 1 sum of n (merged rows), 2 sum of k (poles
+ * kept by deflation), 3 / 4 of them from the left / right child, 5 deflation rotations
+ * that mixed a left and a right pole, 6 FLOPs of the two merge products in dense form
+ * 2 k^2 (n + m), 7 FLOPs of the same products restricted to their nonzero blocks
+ * ([U1 0; 0 U2] structure).  Synchronises the stream.  Returns the rounds written (0
+ * when no batched call is on record), PACO_EINVAL for bad arguments.  No counterpart in
+ * the reference (it logs nothing). */
+#define PACO_MERGE_STATS 8
+int paco_merge_stats(double *out, int max_rounds, void *stream);
+
 /* Columns per leaf of the divide and conquer, 3..32 (default PACO_LEAF_DEFAULT).  Leaves
  * of 3 columns are the reference's partition (svd_tiling.cpp:83-130, solved in closed form
  * at svd.cpp:1596-1864); larger leaves are solved on chip by FP64 one-sided Jacobi (one

@@ -634,6 +634,20 @@ __global__ __launch_bounds__(DT) void dc_deflate_kernel(ProbList pl, LevelArgs l
         if (k > 1 && Ar.dk[1] <= 0.5 * s_tol) Ar.dk[1] = 0.5 * s_tol;
         T.S->k = k;
         T.S->nrot = s_nrot;
+        // where the kept poles come from (the merge products' block structure)
+        int kl = 0, kr = 0, km = 0;
+        for (int t = 0; t < k; ++t) {
+            const int s = Ar.src[Ar.order[t]];
+            kl += s != SRC_POLE0 && s >= 0;
+            kr += s < 0;
+        }
+        for (int r = 0; r < s_nrot; ++r) {
+            const int pr = Ar.rpair[r], sa = Ar.src[pr & 0xffff], sb = Ar.src[pr >> 16];
+            km += sa != SRC_POLE0 && sb != SRC_POLE0 && ((sa >= 0) != (sb >= 0));
+        }
+        T.S->kl = kl;
+        T.S->kr = kr;
+        T.S->km = km;
     }
 }
 

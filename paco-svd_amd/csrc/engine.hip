@@ -964,11 +964,21 @@ static void set_last_info(int *info, int count)
     t_last_gen = g_release_gen;
 }
 
+// Merge states of the last batched call on a stream (paco_merge_stats): device address,
+// merges per round and their offsets; valid while the release generation is unchanged.
+struct LastMerges {
+    const MergeState *dev = nullptr;
+    std::vector<int> ntask;
+    std::vector<size_t> off;
+    uint64_t gen = 0;
+};
+
 struct StreamRes {
     DevBuf ws;      // workspace of the entry points when the caller passes none
     DevBuf plain;   // scratch of paco_svd_plain and of the wide jobs of a batched call
     DevBuf info;    // per-job info of a plain call or of a batched call with wide jobs
     BatchRes batch; // plan, scratch and forked stream of the batched path
+    LastMerges merges;
 };
 static std::unordered_map<hipStream_t, StreamRes *> g_streams; // guarded by g_mu
 
@@ -990,7 +1000,17 @@ struct GraphEntry {
     BatchPlan plan;
     hipGraphExec_t exec = nullptr;
     int *info = nullptr;
+    size_t o_states = 0;
 };
+
+static void note_merges(hipStream_t s, const void *dev, size_t o_states, const BatchPlan &bp)
+{
+    LastMerges &L = stream_res(s)->merges;
+    L.dev = reinterpret_cast<const MergeState *>(static_cast<const char *>(dev) + o_states);
+    L.ntask = bp.ntask;
+    L.off = bp.state_off;
+    L.gen = g_release_gen;
+}
 static std::unordered_map<uint64_t, GraphEntry *> g_graphs;
 
 static uint64_t batch_signature(const paco_svd_job *jobs, int count)
@@ -1395,6 +1415,7 @@ static int batched_narrow(const paco_svd_job *jobs, int count, hipStream_t s, in
     e = enqueue_batch(bp, res, pr, s, true);
     if (e != hipSuccess) return (int)e;
     if (!info_ext) set_last_info(pr.info, count);
+    note_merges(s, res.dev, pr.o_states, bp);
     return 0;
 }
 
@@ -1429,6 +1450,7 @@ static int batched_graph(const paco_svd_job *jobs, int count, hipStream_t s, boo
             return (e == hipErrorInvalidValue) ? PACO_EINVAL : (int)e;
         }
         g->info = pr.info;
+        g->o_states = pr.o_states;
         // record on a library stream (the caller's may be the legacy null stream, which
         // cannot be captured); the graph is launched on the caller's stream below
         static hipStream_t cap = nullptr;
@@ -1462,6 +1484,7 @@ static int batched_graph(const paco_svd_job *jobs, int count, hipStream_t s, boo
     hipError_t e = hipGraphLaunch(g->exec, s);
     if (e != hipSuccess) return (int)e;
     set_last_info(g->info, count);
+    note_merges(s, g->res.dev, g->o_states, g->plan);
     return 0;
 }
 
@@ -1603,6 +1626,43 @@ void paco_release(void)
     }
     g_streams.clear();
     ++g_release_gen;
+}
+
+int paco_merge_stats(double *out, int max_rounds, void *stream)
+{
+    if (!out || max_rounds < 1) return PACO_EINVAL;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    LastMerges L;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        L = stream_res(s)->merges;
+        if (!L.dev || L.gen != g_release_gen) return 0;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return PACO_EINVAL;
+    size_t total = 0;
+    for (size_t r = 0; r < L.ntask.size(); ++r) total = std::max(total, L.off[r] + (size_t)L.ntask[r]);
+    std::vector<MergeState> st(std::max<size_t>(total, 1));
+    if (total && hipMemcpy(st.data(), L.dev, sizeof(MergeState) * total, hipMemcpyDeviceToHost) != hipSuccess)
+        return PACO_EINVAL;
+    const int R = std::min((int)L.ntask.size(), max_rounds);
+    for (int r = 0; r < R; ++r) {
+        double *o = out + (size_t)PACO_MERGE_STATS * r;
+        for (int q = 0; q < PACO_MERGE_STATS; ++q) o[q] = 0.0;
+        for (int t = 0; t < L.ntask[r]; ++t) {
+            const MergeState &S = st[L.off[r] + t];
+            const double k = S.k;
+            o[0] += 1;
+            o[1] += S.n;
+            o[2] += S.k;
+            o[3] += S.kl;
+            o[4] += S.kr;
+            o[5] += S.km;
+            o[6] += 2.0 * k * k * (S.n + S.m);
+            o[7] += 2.0 * k * ((double)(S.kl + S.km) * S.n1 + (double)(S.kr + S.km) * S.n2 + 1.0) +
+                    2.0 * k * ((double)(S.kl + S.km + 1) * S.m1 + (double)(S.kr + S.km + 1) * S.m2);
+        }
+    }
+    return R;
 }
 
 int paco_set_leaf_size(int ls)

@@ -37,7 +37,8 @@ struct MergeState {
     int n, m;      // merged rows / columns
     int n1, m1, n2, m2;
     int sq;
-    int pad[3];
+    int kl, kr;    // non-deflated poles from the left / right child (the pole of z aside)
+    int km;        // deflation rotations that mixed a left and a right pole
 };
 
 static_assert(sizeof(MergeState) % 8 == 0, "MergeState alignment");

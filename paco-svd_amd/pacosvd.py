@@ -15,7 +15,7 @@ EXPORTS = [
     "paco_workspace_size", "paco_set_precision", "paco_svd_plain", "paco_svd_batched",
     "paco_svd_batched_graph", "paco_svd_values_batched", "paco_sgesdd", "paco_svd_plain_noalloc", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
     "paco_timing_enable", "paco_timing_read", "paco_info_offset", "paco_set_secular_maxiter",
-    "paco_set_leaf_size", "paco_leaf_size",
+    "paco_set_leaf_size", "paco_leaf_size", "paco_merge_stats",
     "paco_server_create", "paco_server_submit", "paco_server_poll", "paco_server_drain", "paco_server_stats",
     "paco_server_destroy",
 ]
@@ -60,6 +60,7 @@ def lib():
     L.paco_info_offset.restype = ctypes.c_size_t
     L.paco_set_secular_maxiter.argtypes = [ctypes.c_int]
     L.paco_set_leaf_size.argtypes = [ctypes.c_int]
+    L.paco_merge_stats.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int, _vp]
     L.paco_leaf_size.argtypes = []
     L.paco_gemm_f32.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_float, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, _vp, ctypes.c_int]
@@ -238,6 +239,19 @@ def gemm(A, B, C=None, transA=False, transB=False, alpha=1.0, beta=0.0):
 
 def set_precision(mode):
     return lib().paco_set_precision(mode)
+
+
+MERGE_STATS = ("merges", "sum_n", "sum_k", "sum_k_left", "sum_k_right", "mixed_rotations", "dense_flops",
+               "block_flops")
+
+
+def merge_stats(max_rounds=32):
+    """Per merge round of the last batched call on the current stream: dicts of
+    MERGE_STATS (realised k of deflation, flops of the merge products)."""
+    buf = (ctypes.c_double * (len(MERGE_STATS) * max_rounds))()
+    n = lib().paco_merge_stats(buf, max_rounds, _stream())
+    _check(min(n, 0), "paco_merge_stats")
+    return [dict(zip(MERGE_STATS, buf[len(MERGE_STATS) * r:len(MERGE_STATS) * (r + 1)])) for r in range(n)]
 
 
 def set_leaf_size(ls):
