@@ -648,6 +648,32 @@ __global__ __launch_bounds__(DT) void dc_deflate_kernel(ProbList pl, LevelArgs l
         T.S->kl = kl;
         T.S->kr = kr;
         T.S->km = km;
+        // support of each sorted pole's basis row: 1 left child, 2 right child, 3 both (the
+        // pole of z; a row that absorbed the other child's row by a rotation, in scan order)
+        int *ty = flag; // free after the compaction
+        for (int j = 0; j < n; ++j) {
+            const int s = Ar.src[j];
+            ty[j] = s == SRC_POLE0 ? 3 : s >= 0 ? 1 : 2;
+        }
+        for (int r = 0; r < s_nrot; ++r) {
+            const int pr = Ar.rpair[r];
+            ty[pr >> 16] |= ty[pr & 0xffff];
+        }
+        int gl = 0, gb = 0;
+        for (int t = 0; t < k; ++t) {
+            const int y = ty[Ar.order[t]];
+            gl += y == 1;
+            gb += y == 3;
+        }
+        int pl_ = 0, pb = gl, pr_ = gl + gb;
+        for (int t = 0; t < k; ++t) {
+            const int j = Ar.order[t], y = ty[j];
+            const int p = y == 1 ? pl_++ : y == 3 ? pb++ : pr_++;
+            Ar.pinv[t] = p;
+            Ar.bperm[p] = j;
+        }
+        T.S->gl = gl;
+        T.S->gb = gb;
     }
 }
 
@@ -1039,10 +1065,11 @@ __global__ __launch_bounds__(DT) void dc_vectors_kernel(ProbList pl, LevelArgs l
     float *um = P.a + (size_t)(c0 + i) * N + c0;
     float *vm = P.vmt + (size_t)(c0 + i) * N + c0;
     const bool left = !la.vonly;
+    // coefficient column of pole j: grouped by the support of its basis row (pinv)
     if (k == 1 && z[0] == 0.0) {
         if (lane == 0) {
-            if (left) um[0] = 1.f;
-            vm[0] = 1.f;
+            if (left) um[Ar.pinv[0]] = 1.f;
+            vm[Ar.pinv[0]] = 1.f;
         }
         return;
     }
@@ -1061,8 +1088,9 @@ __global__ __launch_bounds__(DT) void dc_vectors_kernel(ProbList pl, LevelArgs l
     for (int j = lane; j < k; j += G) {
         const double df = (d[j] - dK) * (d[j] + dK) - mu;
         const double v = z[j] / df, u = (j == 0) ? -1.0 : d[j] * z[j] / df;
-        if (left) um[j] = (float)(u * nu);
-        vm[j] = (float)(v * nv);
+        const int p = Ar.pinv[j];
+        if (left) um[p] = (float)(u * nu);
+        vm[p] = (float)(v * nv);
     }
 }
 
@@ -1105,7 +1133,8 @@ __global__ __launch_bounds__(DT) void dc_finalize_kernel(ProbList pl, LevelArgs 
 
 // ---------------------------------------------------------------------------
 // merge products (svd.cpp:1504-1516): rows pos[i] of the node block <-
-//   sum_j coef[i][j] * basis[order[j]], i, j < k, one 64x64 tile per workgroup.
+//   sum_p coef[i][p] * basis[bperm[p]], i, p < k, one 64x64 tile per workgroup (the
+//   coefficient columns grouped by the support of their basis rows, see the deflation).
 // side 0: coef = left secular vectors (in a), basis = gathered left bases, out qt;
 // side 1: coef = right secular vectors (vmt), basis = gathered right bases, out wt.
 // ---------------------------------------------------------------------------
@@ -1124,7 +1153,7 @@ __global__ __launch_bounds__(GT) void dc_merge_gemm_kernel(ProbList pl, LevelArg
     t.A = (side == 0 ? P.a : P.vmt) + base;
     t.B = (side == 0 ? P.qt : P.wt) + (size_t)N * N + base;
     t.C = (side == 0 ? P.qt : P.wt) + base;
-    t.brow = Ar.order;
+    t.brow = Ar.bperm;
     t.crow = Ar.pos;
     t.lda = t.ldb = t.ldc = N;
     t.m = k;
@@ -1136,11 +1165,23 @@ __global__ __launch_bounds__(GT) void dc_merge_gemm_kernel(ProbList pl, LevelArg
     t.m0 = (tile / tpt) * 64;
     t.n0 = (tile % tpt) * 64;
     if (t.m0 >= t.m || t.n0 >= t.n) return;
+    // a column tile inside one child's block needs only the coefficient columns whose basis
+    // rows have support there: left block [0, n1) of U / [0, m1) of W, right block
+    // [n1 + 1, n) / [m1, m) (U's column n1 belongs to the pole of z alone)
+    const int lend = side == 0 ? S.n1 : S.m1, rbeg = side == 0 ? S.n1 + 1 : S.m1;
+    const int n1t = min(t.n0 + 64, t.n);
+    if (n1t <= lend) {
+        t.K = S.gl + S.gb;
+    } else if (t.n0 >= rbeg) {
+        t.A += S.gl;
+        t.brow += S.gl;
+        t.K = k - S.gl;
+    }
     tile_gemm<MODE, TKV>(t);
 }
 
 // Right-vector product of a merge with singular values only: the first and last columns
-// of rows pos[i] of the node block <- sum_j VMt[i][j] * gathered[order[j]] (i, j < k), one
+// of rows pos[i] of the node block <- sum_p VMt[i][p] * gathered[bperm[p]] (i, p < k), one
 // thread per output entry, FP32 in the order j = 0, 1, ...
 __global__ __launch_bounds__(DT) void dc_merge_cols_kernel(ProbList pl, LevelArgs la)
 {
@@ -1158,7 +1199,7 @@ __global__ __launch_bounds__(DT) void dc_merge_cols_kernel(ProbList pl, LevelArg
     const float *vm = P.vmt + (size_t)(c0 + i) * N + c0;
     const float *gt = P.wt + (size_t)N * N + (size_t)c0 * N + c0 + x;
     float acc = 0.f;
-    for (int j = 0; j < k; ++j) acc += vm[j] * gt[(size_t)Ar.order[j] * N];
+    for (int p = 0; p < k; ++p) acc += vm[p] * gt[(size_t)Ar.bperm[p] * N];
     P.wt[(size_t)(c0 + Ar.pos[i]) * N + c0 + x] = acc;
 }
 

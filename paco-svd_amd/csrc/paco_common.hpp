@@ -39,6 +39,10 @@ struct MergeState {
     int sq;
     int kl, kr;    // non-deflated poles from the left / right child (the pole of z aside)
     int km;        // deflation rotations that mixed a left and a right pole
+    // coefficient columns of the merge products grouped by the basis rows' support:
+    // [0, gl) left child only, [gl, gl + gb) both (the pole of z, rows mixed by a
+    // rotation), [gl + gb, k) right child only
+    int gl, gb;
 };
 
 static_assert(sizeof(MergeState) % 8 == 0, "MergeState alignment");
@@ -76,7 +80,8 @@ struct Arena {
     int *rpair;   // [n] rotation t: (jp | j << 16)
     float *rc;    // [n] rotation cosine
     float *rs;    // [n] rotation sine
-    float *nu;    // [n] unused spare
+    int *pinv;    // [k] kept pole t -> its column in the secular-vector coefficients
+    int *bperm;   // [k] coefficient column p -> sorted pole index (basis row of the products)
 };
 
 __host__ __device__ inline Arena make_arena(uint32_t *idx, int c0, int n)
@@ -93,7 +98,8 @@ __host__ __device__ inline Arena make_arena(uint32_t *idx, int c0, int n)
     A.rpair = reinterpret_cast<int *>(base + 10 * (size_t)n);
     A.rc = reinterpret_cast<float *>(base + 11 * (size_t)n);
     A.rs = reinterpret_cast<float *>(base + 12 * (size_t)n);
-    A.nu = reinterpret_cast<float *>(base + 13 * (size_t)n);
+    A.pinv = reinterpret_cast<int *>(base + 13 * (size_t)n);
+    A.bperm = reinterpret_cast<int *>(base + 14 * (size_t)n);
     return A;
 }
 
