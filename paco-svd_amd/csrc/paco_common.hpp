@@ -144,9 +144,14 @@ __device__ inline float wave_sum(float v)
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
-// Sum over the 64 lanes with DPP moves inside each row of 16 lanes (quad swaps, then
-// the half-row and row mirrors) and four lane reads across the rows: no LDS round trip.
-// Every lane returns the same value; the order of additions is fixed.
+// Sum over the 64 lanes with DPP moves: inside each row of 16 lanes (quad swaps, then the
+// half-row and row mirrors: every lane of row q holds its row's sum R_q), then across the
+// rows with the row broadcasts (rows 1 and 3 add lane 15 of the row before, rows 2 and 3
+// add lane 31), which leaves (R3 + R2) + (R1 + R0) in lane 63: 7 vector instructions, no
+// LDS round trip.  Every lane returns the same value, the order of additions is fixed
+// (and gives the bits of (R0 + R1) + (R2 + R3)).  All 64 lanes must be active.  The
+// cross-row adds are written as DPP-modified adds, whose lanes outside the row mask keep
+// their value (two wait states before a DPP read of a just-written register).
 #define PACO_DPP(v, ctrl) __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, false))
 __device__ inline float wave_sum_dpp(float v)
 {
@@ -154,11 +159,10 @@ __device__ inline float wave_sum_dpp(float v)
     v += PACO_DPP(v, 0x4E);  // quad_perm [2,3,0,1]
     v += PACO_DPP(v, 0x141); // row_half_mirror
     v += PACO_DPP(v, 0x140); // row_mirror
-    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
-    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
-    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
-    return (r0 + r1) + (r2 + r3);
+    asm volatile("s_nop 1\n\tv_add_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+                 "s_nop 1\n\tv_add_f32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf\n\ts_nop 1"
+                 : "+v"(v));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 // FP64 form of wave_sum_dpp: the same lane moves applied to both 32-bit halves
 #define PACO_DPP_D(v, ctrl)                                                                                      \
