@@ -118,9 +118,8 @@ __device__ inline void pass_load(const T *base, int ld, int rbase, int cb, int r
 template <int C, int RQ, typename T>
 __device__ inline void pass_group(T *base, int ld, int rbase, int cb, int r0, int k, int M, int N, int lane, bool upd,
                                   float tau, float inv, float tau2, const float (&wr)[C], const float (&ur)[C],
-                                  float (&x)[RQ][C], lds_f32 *cbuf, lds_f32 *rowbuf, float (&pwp)[C], double &ss)
+                                  float (&x)[RQ][C], lds_f32 *cbuf, lds_f32 *rowbuf, float (&pwp)[C])
 {
-    const int i = k - 1;
     const bool tail = k + lane + 64 * (C - 1) < N;
     if (upd) {
         float acc[RQ];
@@ -137,7 +136,6 @@ __device__ inline void pass_group(T *base, int ld, int rbase, int cb, int r0, in
                 a += x[q][c] * ur[c];
             }
             acc[q] = a;
-            if (on && lane == 0) base[(size_t)(rr - rbase) * ld + (i - cb)] = (tau != 0.f) ? vr : cr;
         }
 #pragma unroll
         for (int q = 0; q < RQ; ++q) acc[q] = wave_sum_dpp(acc[q]) * tau2;
@@ -165,10 +163,7 @@ __device__ inline void pass_group(T *base, int ld, int rbase, int cb, int r0, in
             for (int c = 0; c < C - 1; ++c) rb[64 * c] = x[q][c];
             if (tail) rb[64 * (C - 1)] = x[q][C - 1];
         } else {
-            if (lane == 0) {
-                cbuf[rr] = cv;
-                ss += (double)cv * cv;
-            }
+            if (lane == 0) cbuf[rr] = cv;
 #pragma unroll
             for (int c = 0; c < C; ++c) pwp[c] += cv * x[q][c];
         }
@@ -211,7 +206,8 @@ __device__ inline bool arrive_last(unsigned *cnt, int lane)
 // taken in groups of RQ at multiples of RQ, group g by wavefront g % NW, so that every
 // wavefront visits its rows in increasing order whatever the LDS split (the reductions
 // do not depend on it) and no group straddles the split (a multiple of RQ below M).
-// Wave partials of pw go to pwr[wave]; the return value is lane 0's partial sum of squares.
+// Wave partials of pw go to pwr[wave]; the return value is the wavefront's sum of c_r^2 over
+// its rows (FP64, formed from cbuf after the pass: one lane per row, then a wave sum).
 template <int NT, int C>
 __device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0, int split, int M, int N, int i,
                                       float tau, float inv, float tau2, float inv2, lds_f32 *cbuf,
@@ -229,15 +225,15 @@ __device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G,
         ur[c] = !(upd && j < N) ? 0.f : j == k ? 1.f : (tau2 != 0.f) ? ybuf[j] * inv2 : ybuf[j];
         pwp[c] = 0.f;
     }
-    double ss = 0.0;
     const int g0 = k / RQ;
-    int g = g0 + ((wv - g0 % NW) % NW + NW) % NW; // first group of this wavefront
+    const int gf = g0 + ((wv - g0 % NW) % NW + NW) % NW; // first group of this wavefront
+    int g = gf;
     float xa[RQ][C], xb[RQ][C];
     // rows in global memory: [k, split)
     if (RQ * g < split) pass_load<C, RQ>(G, ldg, 0, 0, RQ * g, k, split, N, lane, xa);
     for (; RQ * g < split; g += NW) {
         if (RQ * (g + NW) < split) pass_load<C, RQ>(G, ldg, 0, 0, RQ * (g + NW), k, split, N, lane, xb);
-        pass_group<C, RQ>(G, ldg, 0, 0, RQ * g, k, M, N, lane, upd, tau, inv, tau2, wr, ur, xa, cbuf, rowbuf, pwp, ss);
+        pass_group<C, RQ>(G, ldg, 0, 0, RQ * g, k, M, N, lane, upd, tau, inv, tau2, wr, ur, xa, cbuf, rowbuf, pwp);
 #pragma unroll
         for (int q = 0; q < RQ; ++q)
 #pragma unroll
@@ -248,7 +244,7 @@ __device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G,
     for (; RQ * g < M; g += NW) {
         if (RQ * (g + NW) < M) pass_load<C, RQ>(L, ldl, split, c0, RQ * (g + NW), k, M, N, lane, xb);
         pass_group<C, RQ>(L, ldl, split, c0, RQ * g, k, M, N, lane, upd, tau, inv, tau2, wr, ur, xa, cbuf, rowbuf,
-                          pwp, ss);
+                          pwp);
 #pragma unroll
         for (int q = 0; q < RQ; ++q)
 #pragma unroll
@@ -259,7 +255,18 @@ __device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G,
         const int j = k + lane + 64 * c;
         if (c < C - 1 || j < N) pwr[(size_t)wv * N + j] = pwp[c];
     }
-    return ss;
+    // sum of c_r^2 over this wavefront's rows r > k (the e-th row of the wavefront is row q
+    // of its (e / RQ)-th group), lane e mod 64 taking row e: lane 0 wrote them to cbuf
+    double ss = 0.0;
+    for (int e = lane;; e += 64) {
+        const int r = RQ * (gf + NW * (e / RQ)) + e % RQ;
+        if (r >= M) break;
+        if (r > k) {
+            const double c = cbuf[r];
+            ss += c * c;
+        }
+    }
+    return wave_sum_dpp_d(ss);
 }
 
 template <int NT>
@@ -389,11 +396,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
             P.d[i] = beta;
             P.tauq[i] = tau;
         }
-        if (i == N - 1) {
-            for (int r = i + 1 + tid; r < M; r += NT)
-                store_elem(G, ldg, L, ldl, c0, split, r, i, (tau != 0.f) ? cbuf[r] * inv : cbuf[r]);
-            break;
-        }
+        // the column reflector's vector into column i below the diagonal (read by no pass)
+        for (int r = i + 1 + tid; r < M; r += NT)
+            store_elem(G, ldg, L, ldl, c0, split, r, i, (tau != 0.f) ? cbuf[r] * inv : cbuf[r]);
+        if (i == N - 1) break;
         // ---- w = tau (row i + (1/v1) pw) and y = row i - w on this thread's columns
         float yv[JPT];
         double part = 0.0;
