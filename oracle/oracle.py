@@ -41,6 +41,7 @@ def lib():
         L.orc_bidiag.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 7
         _fp = ctypes.POINTER(ctypes.c_float)
         L.orc_bidiag_f32.argtypes = [ctypes.c_int, ctypes.c_int] + [_fp] * 5
+        L.orc_bidiag_f32_order.argtypes = [ctypes.c_int, ctypes.c_int] + [_fp] * 5 + [ctypes.c_int]
         L.orc_leaf_partition.argtypes = [ctypes.c_int, _ip, _ip]
         L.orc_leaf_partition.restype = ctypes.c_int
         L.orc_leaf_svd.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
@@ -67,9 +68,10 @@ def house(x):
     return x, beta.value, tau.value
 
 
-def bidiag_f32(A):
+def bidiag_f32(A, order=0):
     """The bidiagonalization in FP32 (the yardstick of FP32 forward error): dict(a, d, e,
-    tauq, taup) in the reference layout."""
+    tauq, taup) in the reference layout.  order selects the evaluation order of the
+    reflector inner products (0 forward, 1 backward, 2 eight interleaved partial sums)."""
     a = np.ascontiguousarray(A, dtype=np.float32).copy()
     M, N = a.shape
     d = np.zeros(N, np.float32)
@@ -77,8 +79,8 @@ def bidiag_f32(A):
     tq = np.zeros(N, np.float32)
     tp = np.zeros(max(N - 1, 1), np.float32)
     fp = ctypes.POINTER(ctypes.c_float)
-    lib().orc_bidiag_f32(M, N, a.ctypes.data_as(fp), d.ctypes.data_as(fp), e.ctypes.data_as(fp), tq.ctypes.data_as(fp),
-                         tp.ctypes.data_as(fp))
+    lib().orc_bidiag_f32_order(M, N, a.ctypes.data_as(fp), d.ctypes.data_as(fp), e.ctypes.data_as(fp),
+                               tq.ctypes.data_as(fp), tp.ctypes.data_as(fp), int(order))
     return dict(a=a, d=d, e=e[: N - 1], tauq=tq, taup=tp[: N - 1])
 
 

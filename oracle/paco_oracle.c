@@ -15,8 +15,9 @@
  *   orc_bidiag           invariants vs FP64 LAPACK: U, Vt orthogonal; U B Vt = A;
  *                        sigma(B) = dgesdd(A); d_i, e_i >= 0 (uniqueness of Golub-Kahan
  *                        with this sign convention); brute-force 2x2 / 3x2 cases
- *   orc_bidiag_f32       the FP32 yardstick: within FP32 forward error of orc_bidiag, tau in
- *                        [0, 2], tau = 2 / |v|^2 for every stored vector
+ *   orc_bidiag_f32       the FP32 yardstick (three evaluation orders of its inner products):
+ *                        each within FP32 forward error of orc_bidiag, tau in [0, 2],
+ *                        tau = 2 / |v|^2 for every stored vector
  *   orc_leaf_svd         brute force vs numpy.linalg.svd on every 1x1/1x2/2x2/2x3 case,
  *                        including rank-0/1 and zero-pattern branches
  *   orc_secular_root     scipy.linalg.lapack.dlasd4 per root (LAPACK's own solver)
@@ -143,29 +144,50 @@ static void house_f32(int n, float *x, long inc, float *beta, float *tau)
     *beta = (float)mu;
 }
 
-static void apply_reflector_f32(int n, const float *hv, long hinc, float tau, float *y, long yinc)
+/* The inner product v^T y of a reflector application in one of three FP32 evaluation
+ * orders, all exact-arithmetic equal: 0 forward, 1 backward, 2 eight interleaved partial
+ * sums added pairwise at the end (the order of a vector unit).  Their spread samples how
+ * far FP32 rounding alone moves the forward-sensitive outputs. */
+static void apply_reflector_f32(int n, const float *hv, long hinc, float tau, float *y, long yinc, int order)
 {
-    float c = y[0];
-    for (int k = 1; k < n; k++) c += hv[k * hinc] * y[k * yinc];
+    float c;
+    if (order == 1) {
+        c = 0.f;
+        for (int k = n - 1; k >= 1; k--) c += hv[k * hinc] * y[k * yinc];
+        c += y[0];
+    } else if (order == 2) {
+        float p[8] = {y[0], 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int k = 1; k < n; k++) p[k & 7] += hv[k * hinc] * y[k * yinc];
+        c = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+    } else {
+        c = y[0];
+        for (int k = 1; k < n; k++) c += hv[k * hinc] * y[k * yinc];
+    }
     c *= tau;
     y[0] -= c;
     for (int k = 1; k < n; k++) y[k * yinc] -= c * hv[k * hinc];
 }
 
-ORC_EXPORT void orc_bidiag_f32(int M, int N, float *a, float *d, float *e, float *tauq, float *taup)
+ORC_EXPORT void orc_bidiag_f32_order(int M, int N, float *a, float *d, float *e, float *tauq, float *taup, int order)
 {
     for (int i = 0; i < N; i++) {
         house_f32(M - i, a + (long)i * N + i, N, d + i, tauq + i);
         if (tauq[i] != 0.f)
             for (int j = i + 1; j < N; j++)
-                apply_reflector_f32(M - i, a + (long)i * N + i, N, tauq[i], a + (long)i * N + j, N);
+                apply_reflector_f32(M - i, a + (long)i * N + i, N, tauq[i], a + (long)i * N + j, N, order);
         if (i < N - 1) {
             house_f32(N - i - 1, a + (long)i * N + i + 1, 1, e + i, taup + i);
             if (taup[i] != 0.f)
                 for (int r = i + 1; r < M; r++)
-                    apply_reflector_f32(N - i - 1, a + (long)i * N + i + 1, 1, taup[i], a + (long)r * N + i + 1, 1);
+                    apply_reflector_f32(N - i - 1, a + (long)i * N + i + 1, 1, taup[i], a + (long)r * N + i + 1, 1,
+                                        order);
         }
     }
+}
+
+ORC_EXPORT void orc_bidiag_f32(int M, int N, float *a, float *d, float *e, float *tauq, float *taup)
+{
+    orc_bidiag_f32_order(M, N, a, d, e, tauq, taup, 0);
 }
 
 /* ------------------------------------------------------------------------- */

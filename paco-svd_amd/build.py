@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libpacosvd.so")
 DRIVER = os.path.join(HERE, "svd_main")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["bidiag.hip", "bidiag_blk.hip", "bidiag_grid.hip", "qp_gemm.hip", "dc.hip", "gemm.hip", "engine.hip", "serve.hip", "tiling.cpp", "autotune.cpp"]
+SOURCES = ["bidiag.hip", "bidiag_blk.hip", "bidiag_grid.hip", "bidiag_res.hip", "qp_gemm.hip", "dc.hip", "gemm.hip", "engine.hip", "serve.hip", "tiling.cpp", "autotune.cpp"]
 FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
          # packed FP32 instructions stay off (only the compiler's scalar FP32 forms are used)
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -92,7 +92,8 @@ static size_t carve(int M, int N, char *base, GridWs *W)
     return o;
 }
 
-size_t bidiag_grid_ws_bytes(int M, int N) { return carve(M, N, nullptr, nullptr); }
+// the panel kernels' part, then the resident kernel's exchange buffers
+size_t bidiag_grid_ws_bytes(int M, int N) { return carve(M, N, nullptr, nullptr) + bidiag_res_ws_bytes(N); }
 
 struct PanelArgs {
     ProbDesc P;
@@ -900,8 +901,32 @@ static int grid_tail_start(int M, int N)
     return p;
 }
 
+// Workgroups asked of the resident kernel (bidiag_res.hip): grid^2 for a tile grid the
+// caller gives (a share of the chip), else PACO_RES_P, else 128 (the kernel takes more when
+// the matrix needs more registers).  Its time per column is bound by the latency of the
+// two exchanges, which more workgroups do not shorten: 128 and 256 measure within 5 % of
+// each other from 1100 x 1000 to 4096 x 1024, and 128 is 12 % faster at 2048 x 2048.
+static int res_request(int grid)
+{
+    if (grid > 0) return grid * grid;
+    const char *e = getenv("PACO_RES_P");
+    return e ? std::max(1, atoi(e)) : 128;
+}
+
+// First panel boundary from which the trailing matrix fits the registers of the chip
+// (the resident kernel finishes it); -1: it never does, or PACO_GRID_RES=0.
+static int res_start(int M, int N, int preq)
+{
+    const char *e = getenv("PACO_GRID_RES");
+    if (e && e[0] == '0') return -1;
+    for (int p = 0; N - p >= 2; p += NB)
+        if (bidiag_res_workgroups(M - p, N - p, preq) > 0) return p;
+    return -1;
+}
+
 bool bidiag_grid_supported(int M, int N)
 {
+    if (res_start(M, N, res_request(0)) >= 0) return true;
     const int p = grid_tail_start(M, N);
     return bidiag_wg_lds_bytes(M - p, N - p, 1024) <= bidiag_wg_lds_limit();
 }
@@ -910,9 +935,13 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int gr
 {
     const int M = P.M, N = P.N;
     GridWs W;
-    carve(M, N, static_cast<char *>(ws), &W);
-    const int p_tail = grid_tail_start(M, N);
-    if (bidiag_wg_lds_bytes(M - p_tail, N - p_tail, 1024) > bidiag_wg_lds_limit()) return hipErrorInvalidValue;
+    const size_t panel_ws = carve(M, N, static_cast<char *>(ws), &W);
+    // panels up to p_end, then the resident kernel (from p_res) or the one-workgroup kernel
+    const int preq = res_request(grid);
+    const int p_res = res_start(M, N, preq);
+    const int p_end = p_res >= 0 ? p_res : grid_tail_start(M, N);
+    if (p_res < 0 && bidiag_wg_lds_bytes(M - p_end, N - p_end, 1024) > bidiag_wg_lds_limit())
+        return hipErrorInvalidValue;
     // tile grid: the caller's, else PACO_GRID, else the autotuner's table, else 16 (4 .. GMAX);
     // PACO_GRID_PERSIST=0: two launches per column
     const char *genv = getenv("PACO_GRID");
@@ -937,9 +966,9 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int gr
     // at most GBT rows and TW_MAX columns per tile (one row / column per thread)
     const int GR = std::min(GMAX, std::max(G, (M + GBT - 1) / GBT));
     const int GC = std::min(GMAX, std::max(G, (N + TW_MAX - 1) / TW_MAX));
-    hipLaunchKernelGGL(gb_sync_reset_kernel, dim3(1), dim3(256), 0, s, W.sync, W.flags, 1);
+    if (p_end > 0) hipLaunchKernelGGL(gb_sync_reset_kernel, dim3(1), dim3(256), 0, s, W.sync, W.flags, 1);
     int p = 0;
-    while (p < p_tail) {
+    while (p < p_end) {
         PanelArgs a;
         a.P = P;
         a.W = W;
@@ -978,7 +1007,8 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int gr
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    // finish the trailing (M-p) x (N-p) matrix with one workgroup
+    // finish the trailing (M-p) x (N-p) matrix in the registers of the chip, or with one
+    // workgroup
     ProbList pl;
     pl.ptr = nullptr;
     pl.count = 1;
@@ -991,6 +1021,7 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int gr
     pl.single.e = P.e + p;
     pl.single.tauq = P.tauq + p;
     pl.single.taup = P.taup + p;
+    if (p_res >= 0) return launch_bidiag_res(pl.single, static_cast<char *>(ws) + panel_ws, s, preq);
     return launch_bidiag_wg(pl, nullptr, 1, bidiag_wg_lds_limit(), s, 1024);
 }
 

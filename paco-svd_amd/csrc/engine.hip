@@ -109,6 +109,8 @@ static bool large_matrix(int M, int N) { return N > BIDIAG_WG_MAXN || (size_t)M 
 // merges of the D&C tree: most with the smallest leaves (scratch is sized for those)
 static int total_merges(int N) { return leaf_count(N, LEAF_MIN) - 1; }
 
+static bool use_grid_bidiag(int M, int N);
+
 static WsLayout ws_layout(int M, int N)
 {
     WsLayout L;
@@ -123,8 +125,8 @@ static WsLayout ws_layout(int M, int N)
     o = align_up(o + 64, 256);
     L.states = o;
     o = align_up(o + sizeof(MergeState) * (size_t)std::max(total_merges(N), 1), 256);
-    L.grid = o;
-    o = align_up(o + bidiag_grid_ws_bytes(M, N), 256);
+    L.grid = o; // the chip-wide bidiagonalization's buffers, for the shapes that take it
+    if (use_grid_bidiag(M, N)) o = align_up(o + bidiag_grid_ws_bytes(M, N), 256);
     L.qp = o;
     if (large_matrix(M, N)) o = align_up(o + qp_gemm_ws_bytes(M, N), 256);
     L.total = o;

@@ -113,6 +113,12 @@ size_t bidiag_grid_ws_bytes(int M, int N);
 bool bidiag_grid_supported(int M, int N);
 // grid: tiles per side of the panel kernel's tile grid (0: the default, 16, or PACO_GRID)
 hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int grid = 0);
+// bidiag_res.hip: the whole matrix in the registers of P workgroups (one per CU)
+size_t bidiag_res_ws_bytes(int N);
+// workgroups the kernel takes for an M x N matrix when preq are asked for (0: it does not
+// fit the registers of the chip)
+int bidiag_res_workgroups(int M, int N, int preq);
+hipError_t launch_bidiag_res(const ProbDesc &P, void *ws, hipStream_t s, int preq);
 
 // dc.hip
 // leaves of ls columns (leaf_off: prefix of the leaf counts over the problems, or null)

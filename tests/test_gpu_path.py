@@ -40,6 +40,33 @@ def np_of(t):
     return t.detach().cpu().numpy()
 
 
+def assert_entry_a_parity(r, A):
+    """Every output of entry point (a) element-wise against the FP64 oracle: d, e, tauq,
+    taup and the whole of a (d / e on the band, the column reflectors below it, the row
+    reflectors right of it; upper_bidiagonalization.cpp:204,271-282).  Golub-Kahan
+    coefficients are forward-sensitive, so the bar is derived from the same algorithm run in
+    FP32 on the CPU: max|gpu - fp64| <= 4 yard + 16 eps32 scale, per output, where yard is
+    the largest max|fp32 - fp64| over three evaluation orders of the FP32 run's inner
+    products (scale: ||A||_2 for d, e; 1 for tau in [0, 2]; max|a| for a).  One order alone
+    is a thin sample: on a 1100 x 1000 N(0,1) matrix the FP32 error of d is 0.0098, 0.021
+    and 0.034 in the three orders, and 0.045 to 0.105 on row permutations of the matrix
+    (which leave d, e unchanged in exact arithmetic)."""
+    shape = A.shape
+    g = {k: np_of(v).astype(np.float64) for k, v in r.items() if k in ("a", "d", "e", "tauq", "taup")}
+    o = oracle.bidiag(A.astype(np.float64))
+    fs = [oracle.bidiag_f32(A, order) for order in (0, 1, 2)]
+    an = np.linalg.norm(A.astype(np.float64), 2)
+    eps = float(np.finfo(np.float32).eps)
+    for k, scale in (("d", an), ("e", an), ("tauq", 1.0), ("taup", 1.0), ("a", max(1.0, np.max(np.abs(o["a"]))))):
+        if g[k].size == 0 or (k in ("e", "taup") and shape[1] < 2):
+            continue
+        err = float(np.max(np.abs(g[k] - o[k])))
+        yard = max(float(np.max(np.abs(f[k].astype(np.float64) - o[k]))) for f in fs)
+        bar = 4.0 * yard + 16.0 * eps * scale
+        say(k, "gpu err %.3e fp32 err %.3e bar %.3e" % (err, yard, bar))
+        assert err <= bar, (k, err, yard)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("shape", [(64, 64, 64, False, False), (100, 37, 73, False, True), (130, 65, 129, True, False),
                                    (5, 200, 17, True, True), (257, 96, 300, False, False)])
@@ -123,26 +150,27 @@ def test_entry_points_shapes(torch_dev, shape):
     assert m["resid_ratio"] <= RESID_TOL
 
 
+@pytest.mark.parametrize("path", ["resident", "panel"])
 @pytest.mark.parametrize("shape", [(1100, 1000), (2048, 700), (1200, 1023)])
-def test_grid_bidiag_vs_oracle(torch_dev, shape):
-    """Large matrices take the chip-wide panel bidiagonalization (M*N >= 2^20)."""
+def test_grid_bidiag_vs_oracle(torch_dev, shape, path):
+    """Large matrices (M*N >= 2^20) take the chip-wide bidiagonalization: the matrix in the
+    registers of the chip (default) or the panel kernels (PACO_GRID_RES=0)."""
     torch = torch_dev
     import pacosvd
-    say("grid bidiag", shape)
+    say("grid bidiag", shape, path)
     rng = np.random.default_rng(shape[1])
     A = rng.standard_normal(shape).astype(np.float32)
     a = torch.from_numpy(A.copy()).cuda()
-    r = pacosvd.upper_bidiagonalization(a)
-    torch.cuda.synchronize()
+    if path == "panel":
+        os.environ["PACO_GRID_RES"] = "0"
+    try:
+        r = pacosvd.upper_bidiagonalization(a)
+        torch.cuda.synchronize()
+    finally:
+        os.environ.pop("PACO_GRID_RES", None)
+    assert pacosvd.entry_info(r["ws"], *shape) == 0
+    assert_entry_a_parity(r, A)
     d, e = np_of(r["d"]), np_of(r["e"])
-    orc = oracle.bidiag(A.astype(np.float64))
-    an = np.linalg.norm(A, 2)
-    say("d err", np.max(np.abs(d - orc["d"])) / an, "e err", np.max(np.abs(e - orc["e"])) / an)
-    assert np.max(np.abs(d - orc["d"])) <= BIDIAG_TOL * an
-    assert np.max(np.abs(e - orc["e"])) <= BIDIAG_TOL * an
-    # tau (in [0, 2]) carries the same FP32 forward error as d and e, growing along the
-    # factorization (~1e-8 for the first columns, ~1e-3 near column 1000)
-    assert np.max(np.abs(np_of(r["tauq"]) - orc["tauq"])) <= 2 * 2 * BIDIAG_TOL
     u, vt = np_of(r["u"]), np_of(r["vt"])
     M, N = shape
     assert np.max(np.abs(u.T @ u - np.eye(M))) <= ORTH_TOL
@@ -477,6 +505,14 @@ def test_grid_persistent_matches_launch_per_phase(torch_dev, shape):
     import pacosvd
     rng = np.random.default_rng(shape[0] + 7 * shape[1])
     A = rng.standard_normal(shape).astype(np.float32)
+    os.environ["PACO_GRID_RES"] = "0"  # the panel kernels throughout
+    try:
+        grid_panel_schedules(torch, pacosvd, shape, A)
+    finally:
+        del os.environ["PACO_GRID_RES"]
+
+
+def grid_panel_schedules(torch, pacosvd, shape, A):
     outs = {}
     for persist in ("1", "0"):
         say("grid", shape, "persistent", persist)
@@ -539,16 +575,60 @@ def test_entry_a_outputs_vs_oracle(torch_dev, shape, blocked):
         torch.cuda.synchronize()
     finally:
         os.environ.pop("PACO_BIDIAG_BLOCKED", None)
-    g = {k: np_of(v).astype(np.float64) for k, v in r.items() if k in ("a", "d", "e", "tauq", "taup")}
-    o = oracle.bidiag(A.astype(np.float64))
-    f = oracle.bidiag_f32(A)
-    an = np.linalg.norm(A.astype(np.float64), 2)
-    eps = float(np.finfo(np.float32).eps)
-    for k, scale in (("d", an), ("e", an), ("tauq", 1.0), ("taup", 1.0), ("a", max(1.0, np.max(np.abs(o["a"]))))):
-        if g[k].size == 0 or (k in ("e", "taup") and shape[1] < 2):
-            continue
-        err = float(np.max(np.abs(g[k] - o[k])))
-        yard = float(np.max(np.abs(f[k].astype(np.float64) - o[k])))
-        bar = 4.0 * yard + 16.0 * eps * scale
-        say(k, "gpu err %.3e fp32 err %.3e bar %.3e" % (err, yard, bar))
-        assert err <= bar, (k, err, yard)
+    assert_entry_a_parity(r, A)
+
+
+@pytest.mark.parametrize("shape,pmax", [((1100, 1000), 0), ((1100, 1000), 16), ((2048, 700), 0), ((1500, 1500), 0),
+                                        ((1500, 1500), 40), ((4100, 300), 0)])
+def test_resident_bidiag_vs_oracle(torch_dev, shape, pmax):
+    """The bidiagonalization held in the registers of P workgroups (bidiag_res.hip), every
+    output of entry point (a) element-wise against the FP64 oracle with the bar of
+    test_entry_a_outputs_vs_oracle.  pmax caps the workgroups (PACO_RES_PMAX): with 16
+    (1100 x 1000) and 40 (1500 x 1500) the matrix does not fit at first, so panels run until
+    the trailing matrix does.  Two runs give the same bits."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(shape[0] + shape[1] + pmax)
+    A = rng.standard_normal(shape).astype(np.float32)
+    say("resident bidiag", shape, "pmax", pmax)
+    if pmax:
+        os.environ["PACO_RES_PMAX"] = str(pmax)
+    try:
+        runs = []
+        for _ in range(2):
+            r = pacosvd.upper_bidiagonalization(torch.from_numpy(A.copy()).cuda())
+            torch.cuda.synchronize()
+            assert pacosvd.entry_info(r["ws"], *shape) == 0
+            runs.append(r)
+    finally:
+        os.environ.pop("PACO_RES_PMAX", None)
+    for k in ("a", "d", "e", "tauq", "taup"):
+        assert torch.equal(runs[0][k], runs[1][k]), k
+    assert_entry_a_parity(runs[0], A)
+
+
+def test_resident_bidiag_timeout_reports_info(torch_dev):
+    """A wait of the resident kernel that exceeds its bound stops the call with info -2 (the
+    bound forced to zero from the host: every wait that is not satisfied at once times out;
+    every workgroup still exits).  The next call on the same workspace, with the normal
+    bound, gives the same bits as a call on a fresh workspace."""
+    torch = torch_dev
+    import pacosvd
+    shape = (1100, 1000)
+    A = np.random.default_rng(5).standard_normal(shape).astype(np.float32)
+    say("resident timeout", shape)
+    ref = pacosvd.upper_bidiagonalization(torch.from_numpy(A.copy()).cuda())
+    torch.cuda.synchronize()
+    assert pacosvd.entry_info(ref["ws"], *shape) == 0
+    os.environ["PACO_RES_TIMEOUT_US"] = "0"
+    try:
+        r = pacosvd.upper_bidiagonalization(torch.from_numpy(A.copy()).cuda())
+        torch.cuda.synchronize()
+    finally:
+        del os.environ["PACO_RES_TIMEOUT_US"]
+    assert pacosvd.entry_info(r["ws"], *shape) == -2
+    r = pacosvd.upper_bidiagonalization(torch.from_numpy(A.copy()).cuda(), ws=r["ws"])
+    torch.cuda.synchronize()
+    assert pacosvd.entry_info(r["ws"], *shape) == 0
+    for k in ("a", "u", "vt", "d", "e", "tauq", "taup"):
+        assert torch.equal(r[k], ref[k]), k

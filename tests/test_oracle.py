@@ -252,15 +252,21 @@ def test_svd_DC_matches_bdc_backtransform():
     assert np.array_equal(U[:, 45:], r["u"][:, 45:])
 
 
+@pytest.mark.parametrize("order", [0, 1, 2])
 @pytest.mark.parametrize("shape", [(128, 96), (57, 13), (200, 200), (300, 7)])
-def test_bidiag_f32_yardstick(shape):
+def test_bidiag_f32_yardstick(shape, order):
     """The FP32 run of the bidiagonalization (the yardstick of FP32 forward error used by
-    the GPU parity test) stays within FP32 forward error of the FP64 oracle, and its own
-    reflectors are consistent (tau in [0, 2], tau (1 + |v_tail|^2) = 2)."""
+    the GPU parity test), in each evaluation order of its inner products, stays within FP32
+    forward error of the FP64 oracle, and its own reflectors are consistent (tau in [0, 2],
+    tau (1 + |v_tail|^2) = 2).  The orders differ only by rounding: never bit-identical
+    over a whole factorization, always within FP32 forward error of each other."""
     rng = np.random.default_rng(shape[0] + shape[1])
     A = rng.uniform(-1, 1, shape).astype(np.float32)
     M, N = shape
-    f = oracle.bidiag_f32(A)
+    f = oracle.bidiag_f32(A, order)
+    if order and N > 8:
+        f0 = oracle.bidiag_f32(A, 0)
+        assert not np.array_equal(f["a"], f0["a"])
     o = oracle.bidiag(A.astype(np.float64))
     an = np.linalg.norm(A.astype(np.float64), 2)
     assert np.max(np.abs(f["d"] - o["d"])) <= 1e-4 * an
