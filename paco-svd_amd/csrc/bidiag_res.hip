@@ -479,23 +479,30 @@ static bool res_pick(int M, int N, int preq, ResShape &sh)
     const int pmax = std::min(std::min(RES_PMAX, device_cus()), ecap ? std::max(1, atoi(ecap)) : RES_PMAX);
     const int P0 = std::max(1, std::min(preq, pmax));
     static const int vrs[3] = {8, 16, 32};
-    // the asked-for count when its registers hold the matrix, else the fewest that do
-    for (int pass = 0; pass < 2; ++pass) {
-        for (int vr : vrs) {
-            if (vr * vc > 128) break;
-            if (vr * vc == 64 && vc == 8) continue; // (M <= 8 P < N cannot occur: M >= N)
-            const int p = (M + vr - 1) / vr; // workgroups that vr rows each need
-            if (p > (pass == 0 ? P0 : pmax)) continue;
-            sh.VR = vr;
-            sh.VC = vc;
-            sh.P = pass == 0 ? std::min(P0, p) : p;
-            if ((N + sh.P - 1) / sh.P <= RT) return true; // a reduction slice fits one workgroup
-        }
+    // the most workgroups up to the asked-for count whose registers hold the matrix ...
+    for (int vr : vrs) {
+        if (vr * vc > 128) break;
+        if (vr * vc == 64 && vc == 8) continue; // (M <= 8 P < N cannot occur: M >= N)
+        const int p = (M + vr - 1) / vr;       // workgroups that vr rows each need
+        if (p > P0 || (N + p - 1) / p > RT) continue; // (a reduction slice fits one workgroup)
+        sh.VR = vr;
+        sh.VC = vc;
+        sh.P = std::min(P0, p);
+        return true;
+    }
+    // ... else the fewest that do (the largest register share per workgroup)
+    for (int i = 2; i >= 0; --i) {
+        const int vr = vrs[i];
+        if (vr * vc > 128 || (vr * vc == 64 && vc == 8)) continue;
+        const int p = (M + vr - 1) / vr;
+        if (p > pmax || (N + p - 1) / p > RT) continue;
+        sh.VR = vr;
+        sh.VC = vc;
+        sh.P = p;
+        return true;
     }
     return false;
 }
-
-bool bidiag_res_fits(int M, int N) { ResShape sh; return res_pick(M, N, RES_PMAX, sh); }
 
 // Workgroups of the resident launches in flight per device: a launch waits (on its stream)
 // for the oldest ones until its own fit the CU count, so every launch's workgroups can be
