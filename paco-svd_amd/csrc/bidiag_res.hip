@@ -17,7 +17,8 @@
 //     partial GEMV sums g[c] = sum_{r > j+1} c_r A[r, c] (the reflector's 1/v_1 scaling is
 //     deferred to the reduction)
 //   exchange 1: every workgroup publishes its partial sums, and the owner of row j+1 that
-//     row; workgroup q reduces the columns of its slice [q S, q S + S) over the P partials
+//     row; workgroup q reduces the columns of its slice (positions [q S, q S + S) of the
+//     exchange buffers) over the P partials
 //     in a fixed order, forms the left reflector's parameters (every workgroup forms them
 //     from the same sums in the same order, so they agree bit for bit) and the slice's
 //     entries of w = A^T v and of row j+1 after the left reflector
@@ -51,6 +52,12 @@ __host__ __device__ inline int res_vc(int N)
     const int need = (N + RT - 1) / RT;
     return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : 8;
 }
+
+// The exchange buffers index column c = t VC + k (thread t's k-th column) at position
+// k RT + t, so the VC stores or loads of a thread are each coalesced across the lanes of a
+// wavefront; a reduction slice is a range of positions.
+__host__ __device__ inline int res_pos(int c, int VC) { return (c % VC) * RT + c / VC; }
+__host__ __device__ inline int res_col(int pos, int VC) { return (pos % RT) * VC + pos / RT; }
 
 struct ResWs {
     uint64_t *part;  // [2][P][NP] partial GEMV sums (NP = RT * res_vc(N))
@@ -90,7 +97,7 @@ struct ResArgs {
     int *info;
     ResWs W;
     int M, N, ld, P;
-    int S;    // columns per reduction slice (ceil(N / P))
+    int S;    // positions per reduction slice (ceil(NP / P))
     int tpc;  // threads per slice column (a power of two, at most 64)
     int nq;   // partials per thread of a slice column (ceil(P / tpc))
     long long timeout; // bound of one wait, in wall-clock ticks
@@ -241,18 +248,18 @@ __global__ __launch_bounds__(RT) void bidiag_res_kernel(ResArgs a)
         float cr[VR];
 #pragma unroll
         for (int lr = 0; lr < VR; ++lr) cr[lr] = cb[buf][lr];
-        uint64_t *pp = a.W.part + ((size_t)buf * P + q) * NP + c0;
+        uint64_t *pp = a.W.part + ((size_t)buf * P + q) * NP + tid;
 #pragma unroll
         for (int k = 0; k < VC; ++k) {
             float g = 0.f;
 #pragma unroll
             for (int lr = 0; lr < VR; ++lr) g = fmaf(cr[lr], v[lr][k], g);
-            if (c0 + k > jj) rst(pp + k, tgf(g, tag)); // (only the trailing columns are read)
+            if (c0 + k > jj) rst(pp + k * RT, tgf(g, tag)); // (only the trailing columns are read)
         }
         if (q == qj) {
-            uint64_t *rp = a.W.row + (size_t)buf * NP + c0;
+            uint64_t *rp = a.W.row + (size_t)buf * NP + tid;
 #pragma unroll
-            for (int k = 0; k < VC; ++k) rst(rp + k, tgf(pick_row<0, VR>(v, k, lj), tag));
+            for (int k = 0; k < VC; ++k) rst(rp + k * RT, tgf(pick_row<0, VR>(v, k, lj), tag));
         }
     };
 
@@ -268,13 +275,13 @@ __global__ __launch_bounds__(RT) void bidiag_res_kernel(ResArgs a)
             const long long t0 = wall_clock64();
             bool ok = true;
             const int cc = tid / a.tpc, s = tid % a.tpc;
-            const int c = q * a.S + cc;
-            const bool live = cc < a.S && c > jj && c < N;
+            const int pos = q * a.S + cc, c = res_col(pos, VC);
+            const bool live = cc < a.S && pos < NP && c > jj && c < N;
             float g = 0.f;
             if (live) {
                 // partials of workgroups s + tpc m, m < nq, eight at a time, in that order
                 const size_t st = (size_t)a.tpc * NP;
-                const uint64_t *pc = a.W.part + ((size_t)buf * P + s) * NP + c;
+                const uint64_t *pc = a.W.part + ((size_t)buf * P + s) * NP + pos;
                 const int nsrc = (P - s + a.tpc - 1) / a.tpc; // sources of this thread
                 for (int m0 = 0; m0 < nsrc && ok; m0 += 8) {
                     uint64_t x[8];
@@ -294,12 +301,12 @@ __global__ __launch_bounds__(RT) void bidiag_res_kernel(ResArgs a)
             const uint64_t *rp = a.W.row + (size_t)buf * NP;
             if (live && s == 0) {
                 uint64_t x[1];
-                ok = ok && poll<1>(rp + c, 0, 0, 1, tag, x, a, t0);
+                ok = ok && poll<1>(rp + pos, 0, 0, 1, tag, x, a, t0);
                 rowc = tval(x[0]);
             }
             if (tid == RT - 1) {
                 uint64_t x[1];
-                ok = ok && poll<1>(rp + jj, 0, 0, 1, tag, x, a, t0);
+                ok = ok && poll<1>(rp + res_pos(jj, VC), 0, 0, 1, tag, x, a, t0);
                 bc[0] = tval(x[0]);
             }
             const double ws = wave_sum_dpp_d(sp);
@@ -311,8 +318,8 @@ __global__ __launch_bounds__(RT) void bidiag_res_kernel(ResArgs a)
             house_params_d((double)bc[0], sig, &beta, &tau, &inv);
             if (live && s == 0) {
                 const float wc = rowc + inv * g;
-                rst(a.W.gw + (size_t)buf * NP + c, tgf(wc, tag));
-                rst(a.W.ga + (size_t)buf * NP + c, tgf(rowc - tau * wc, tag));
+                rst(a.W.gw + (size_t)buf * NP + pos, tgf(wc, tag));
+                rst(a.W.ga + (size_t)buf * NP + pos, tgf(rowc - tau * wc, tag));
             }
         }
         // ---- exchange 2: w and row jj after the left reflector (zero outside the trailing
@@ -322,8 +329,8 @@ __global__ __launch_bounds__(RT) void bidiag_res_kernel(ResArgs a)
             const long long t0 = wall_clock64();
             const int klo = max(0, min(VC, jj + 1 - c0)), khi = max(klo, min(VC, N - c0));
             uint64_t xw_[VC], xa_[VC];
-            bool ok = poll<VC>(a.W.gw + (size_t)buf * NP + c0, 1, klo, khi, tag, xw_, a, t0);
-            ok = ok && poll<VC>(a.W.ga + (size_t)buf * NP + c0, 1, klo, khi, tag, xa_, a, t0);
+            bool ok = poll<VC>(a.W.gw + (size_t)buf * NP + tid, RT, klo, khi, tag, xw_, a, t0);
+            ok = ok && poll<VC>(a.W.ga + (size_t)buf * NP + tid, RT, klo, khi, tag, xa_, a, t0);
             float ap[VC];
             double sp = 0.0;
 #pragma unroll
@@ -484,7 +491,7 @@ static bool res_pick(int M, int N, int preq, ResShape &sh)
         if (vr * vc > 128) break;
         if (vr * vc == 64 && vc == 8) continue; // (M <= 8 P < N cannot occur: M >= N)
         const int p = (M + vr - 1) / vr;       // workgroups that vr rows each need
-        if (p > P0 || (N + p - 1) / p > RT) continue; // (a reduction slice fits one workgroup)
+        if (p > P0 || (RT * vc + p - 1) / p > RT) continue; // (a reduction slice fits one workgroup)
         sh.VR = vr;
         sh.VC = vc;
         sh.P = std::min(P0, p);
@@ -495,7 +502,7 @@ static bool res_pick(int M, int N, int preq, ResShape &sh)
         const int vr = vrs[i];
         if (vr * vc > 128 || (vr * vc == 64 && vc == 8)) continue;
         const int p = (M + vr - 1) / vr;
-        if (p > pmax || (N + p - 1) / p > RT) continue;
+        if (p > pmax || (RT * vc + p - 1) / p > RT) continue;
         sh.VR = vr;
         sh.VC = vc;
         sh.P = p;
@@ -608,7 +615,7 @@ hipError_t launch_bidiag_res(const ProbDesc &P, void *ws, hipStream_t s, int pre
     a.N = P.N;
     a.ld = P.ld ? P.ld : P.N;
     a.P = sh.P;
-    a.S = (P.N + sh.P - 1) / sh.P;
+    a.S = (RT * res_vc(P.N) + sh.P - 1) / sh.P;
     a.tpc = 64;
     while (a.tpc > 1 && a.tpc * a.S > RT) a.tpc >>= 1;
     a.nq = (sh.P + a.tpc - 1) / a.tpc;
