@@ -16,6 +16,13 @@ import os
 import sys
 import time
 
+# The serving run keeps two batch lanes and eight wide lanes (ten HIP streams) in flight;
+# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4, HIP's default), so streams
+# beyond that share queues and their work serialises.  The setting is read when HIP
+# initialises, before anything here touches the GPU: at least 16 (a larger value is kept).
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -142,7 +149,7 @@ def accuracy_sample(torch, host, offs, Ms, Ns, outs, n=40):
     return worst
 
 
-def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0, wide_lanes=4):
+def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0, wide_lanes=8):
     """The serving configuration: Poisson arrivals through the serving loop (continuous
     batching), then the same requests through the plain path (one matrix per launch
     sequence, one after the other) for the served / plain ratio.  Accuracy of every
@@ -195,7 +202,8 @@ def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0, wide_lanes=4):
     worst["matrices_checked"] = count
     flops = sum(svd_flops(r["M"], r["N"]) for r in reqs)
     served = st.get("svds_per_s", 0.0)
-    return dict(requests=count, offered_rate=rate, submit_span_s=round(t_sub, 3), total_s=round(t_all, 3),
+    return dict(requests=count, offered_rate=rate, wide_lanes=wide_lanes, batch_lanes=2,
+                hw_queues=int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), submit_span_s=round(t_sub, 3), total_s=round(t_all, 3),
                 achieved_svds_per_s=round(served, 1),
                 p50_ms=round(st.get("p50_ms", 0.0), 2), p99_ms=round(st.get("p99_ms", 0.0), 2),
                 mean_ms=round(st.get("mean_ms", 0.0), 2), batches=int(st.get("batches", 0)),
@@ -318,7 +326,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
-    ap.add_argument("--serve-wide-lanes", type=int, default=4, help="serving: lanes for wide requests")
+    ap.add_argument("--serve-wide-lanes", type=int, default=8, help="serving: lanes for wide requests")
     ap.add_argument("--no-plain-large", action="store_true", help="skip the 4096^2 plain-path configuration")
     ap.add_argument("--no-precision", action="store_true", help="skip the 8192x4096 precision configuration")
     ap.add_argument("--leaf-size", type=int, default=0, help="columns per D&C leaf (0: the library's default)")
