@@ -211,10 +211,20 @@ def svd_values(As, S=None, graph=False):
 def svd_batched(As, outs=None, graph=False):
     """Batched path (one launch per phase over all matrices)."""
     outs = outs if outs is not None else alloc_outputs(As)
-    jobs = _jobs(As, outs)
-    fn = lib().paco_svd_batched_graph if graph else lib().paco_svd_batched
-    _check(fn(jobs, len(As), _stream()), "paco_svd_batched")
+    svd_batched_jobs(_jobs(As, outs), len(As), graph)
     return outs
+
+
+def prepare_jobs(As, outs):
+    """The job descriptor array of matrices As and outputs outs (device pointers), built
+    once for repeated calls of svd_batched_jobs on the same buffers."""
+    return _jobs(As, outs)
+
+
+def svd_batched_jobs(jobs, count, graph=False):
+    """Batched path on a prepared job array (prepare_jobs)."""
+    fn = lib().paco_svd_batched_graph if graph else lib().paco_svd_batched
+    _check(fn(jobs, count, _stream()), "paco_svd_batched")
 
 
 def last_info(count):

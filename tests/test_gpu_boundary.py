@@ -73,6 +73,37 @@ def test_nonfinite_input_sets_negative_info(torch_dev, path):
         assert np.all(np_of(outs[i][0]) == 0.0)  # solved as the zero matrix
 
 
+@pytest.mark.skip(reason="the host process crashed (SIGSEGV) inside paco_svd_batched_graph on the first "
+                         "recording of this batch when recorded batches were split into parallel sub-batch "
+                         "branches; the split was taken out with no cause shown, and the case is not run again "
+                         "on the GPU until one is")
+def test_graph_sub_batches_same_bits_and_info_order(torch_dev):
+    """A recorded batch of 1,100 jobs: every result equals the eager call's bit for bit,
+    and the info words come back in call order (NaN inputs at known positions report -1),
+    on two replays."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(43)
+    shapes = [(int(m), int(rng.integers(4, m + 1))) for m in rng.integers(8, 64, size=1100)]
+    mats = [rng.standard_normal(sh).astype(np.float32) for sh in shapes]
+    bad = [3, 517, 1099]
+    for i in bad:
+        mats[i][0, 0] = np.nan
+    say("graph sub-batches", len(mats))
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    eager = pacosvd.svd_batched(As)
+    torch.cuda.synchronize()
+    want = [-1 if i in bad else 0 for i in range(len(mats))]
+    assert pacosvd.last_info(len(As)) == want
+    for rep in range(2):
+        outs = pacosvd.svd_batched(As, graph=True)
+        torch.cuda.synchronize()
+        assert pacosvd.last_info(len(As)) == want, rep
+        for i in range(len(mats)):
+            for x, y in zip(outs[i], eager[i]):
+                assert torch.equal(x, y), (rep, i, shapes[i])
+
+
 def test_entry_point_info_word(torch_dev):
     torch = torch_dev
     import pacosvd
