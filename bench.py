@@ -469,6 +469,9 @@ def main():
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel="bidiagonalization phase: bidiag_wg_kernel streamed (main stream) + LDS-resident (second stream)",
                 algorithmic_bytes_per_launch=bb, launch_ms=round(ph["bidiag"], 3),
+                # the measured HBM bytes over the same time: rows cached in LDS are not HBM
+                # traffic, so this, not frac, is the phase's share of the HBM bandwidth
+                traffic_frac=round(traffic / t_bid / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
                 traffic_source="profiles/traffic_bidiag_batched_c4.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                "calibrated with the copy kernel)" if traffic else None)
     flops = sum(svd_flops(int(m), int(n)) for m, n in zip(Ms, Ns))

@@ -269,6 +269,188 @@ __device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G,
     return wave_sum_dpp_d(ss);
 }
 
+// ---------------------------------------------------------------------------
+// Packed form of the pass for trailing widths <= 256: a row takes LW = 16 or 32 lanes
+// (lane = slot LW + sub), so one wave instruction covers R = 64 / LW rows and the
+// per-row work (reflector coefficient, row address, reduction, broadcast of c_r) is paid
+// once per R rows instead of once per row.  Lane (slot, sub) holds columns
+// k + sub + LW c, c < CP.  A group is RQP instructions of R rows: rows r0 + q R + slot.
+// Group sizes R RQP divide WG_SPLIT_ALIGN, so groups never straddle the LDS split and
+// every lane visits its rows in increasing order whatever the split.
+// ---------------------------------------------------------------------------
+// sum over the LW lanes of a row, the same bits in every lane of it
+template <int LW>
+__device__ inline float row_sum_dpp(float v)
+{
+#define PACO_ROW_DPP(v, ctrl) __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, false))
+    v += PACO_ROW_DPP(v, 0xB1);  // quad_perm [1,0,3,2]
+    v += PACO_ROW_DPP(v, 0x4E);  // quad_perm [2,3,0,1]
+    v += PACO_ROW_DPP(v, 0x141); // row_half_mirror
+    v += PACO_ROW_DPP(v, 0x140); // row_mirror: every lane holds its 16-lane sum
+#undef PACO_ROW_DPP
+    if (LW == 32) v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401f)); // lane ^ 16
+    return v;
+}
+
+// the value of sub-lane 0 of this lane's row
+template <int LW>
+__device__ inline float row_first(float v)
+{
+    // swizzle bitmask mode within 32 lanes: source lane = lane & and_mask
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), LW == 16 ? 0x10 : 0x00));
+}
+
+template <int LW, int CP, int RQP, typename T>
+__device__ inline void passp_load(const T *base, int ld, int rbase, int cb, int r0, int k, int lim, int N, int sub,
+                                  int slot, float (&x)[RQP][CP])
+{
+    constexpr int R = 64 / LW;
+    const bool tail = k + sub + LW * (CP - 1) < N;
+#pragma unroll
+    for (int q = 0; q < RQP; ++q) {
+        int rr = r0 + q * R + slot;
+        rr = rr < k ? k : rr;
+        rr = rr >= lim ? lim - 1 : rr;
+        const T *row = base + (size_t)(rr - rbase) * ld + (k - cb) + sub;
+#pragma unroll
+        for (int c = 0; c < CP - 1; ++c) x[q][c] = row[LW * c];
+        x[q][CP - 1] = tail ? row[LW * (CP - 1)] : 0.f;
+    }
+}
+
+template <int LW, int CP, int RQP, typename T>
+__device__ inline void passp_group(T *base, int ld, int rbase, int cb, int r0, int k, int M, int N, int sub, int slot,
+                                   bool upd, float tau, float inv, float tau2, const float (&wr)[CP],
+                                   const float (&ur)[CP], float (&x)[RQP][CP], lds_f32 *cbuf, lds_f32 *rowbuf,
+                                   float (&pwp)[CP])
+{
+    constexpr int R = 64 / LW;
+    const bool tail = k + sub + LW * (CP - 1) < N;
+    if (upd) {
+        float acc[RQP];
+#pragma unroll
+        for (int q = 0; q < RQP; ++q) {
+            const int rr = r0 + q * R + slot;
+            const bool on = rr >= k && rr < M;
+            const float cr = cbuf[rr < k ? k : (rr >= M ? M - 1 : rr)];
+            const float vr = (on && tau != 0.f) ? cr * inv : 0.f;
+            float a = 0.f;
+#pragma unroll
+            for (int c = 0; c < CP; ++c) {
+                x[q][c] -= vr * wr[c];
+                a += x[q][c] * ur[c];
+            }
+            acc[q] = a;
+        }
+#pragma unroll
+        for (int q = 0; q < RQP; ++q) acc[q] = row_sum_dpp<LW>(acc[q]) * tau2;
+#pragma unroll
+        for (int q = 0; q < RQP; ++q) {
+            const int rr = r0 + q * R + slot;
+#pragma unroll
+            for (int c = 0; c < CP; ++c) x[q][c] -= acc[q] * ur[c];
+            if (rr >= k && rr < M) {
+                T *row = base + (size_t)(rr - rbase) * ld + (k - cb) + sub;
+#pragma unroll
+                for (int c = 0; c < CP - 1; ++c) row[LW * c] = x[q][c];
+                if (tail) row[LW * (CP - 1)] = x[q][CP - 1];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RQP; ++q) {
+        const int rr = r0 + q * R + slot;
+        const float cv = row_first<LW>(x[q][0]);
+        if (rr == k) {
+            lds_f32 *rb = rowbuf + k + sub;
+#pragma unroll
+            for (int c = 0; c < CP - 1; ++c) rb[LW * c] = x[q][c];
+            if (tail) rb[LW * (CP - 1)] = x[q][CP - 1];
+        } else if (rr > k && rr < M) {
+            if (sub == 0) cbuf[rr] = cv;
+#pragma unroll
+            for (int c = 0; c < CP; ++c) pwp[c] += cv * x[q][c];
+        }
+    }
+}
+
+// Packed fused pass over rows r > i; the same contract as fused_pass_c.  Groups of
+// RG = R RQP rows at multiples of RG, group g by wavefront g % NW.  The R slots' partials
+// of pw are added (slot pairs, then pairs of pairs) before the wave's partial goes to pwr.
+template <int NT, int LW, int CP, int RQP>
+__device__ __attribute__((always_inline)) inline double fused_pass_p(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0,
+                                                                     int split, int M, int N, int i, float tau,
+                                                                     float inv, float tau2, float inv2, lds_f32 *cbuf,
+                                                                     const lds_f32 *wbuf, const lds_f32 *ybuf,
+                                                                     lds_f32 *rowbuf, lds_f32 *pwr)
+{
+    constexpr int NW = NT / 64, R = 64 / LW, RG = R * RQP;
+    static_assert(WG_SPLIT_ALIGN % RG == 0, "a group must not straddle the LDS split");
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, k = i + 1;
+    const int sub = lane & (LW - 1), slot = lane / LW;
+    const bool upd = i >= 0;
+    float wr[CP], ur[CP], pwp[CP];
+#pragma unroll
+    for (int c = 0; c < CP; ++c) {
+        const int j = k + sub + LW * c;
+        wr[c] = (upd && j < N) ? wbuf[j] : 0.f;
+        ur[c] = !(upd && j < N) ? 0.f : j == k ? 1.f : (tau2 != 0.f) ? ybuf[j] * inv2 : ybuf[j];
+        pwp[c] = 0.f;
+    }
+    const int g0 = k / RG;
+    const int gf = g0 + ((wv - g0 % NW) % NW + NW) % NW;
+    int g = gf;
+    float xa[RQP][CP], xb[RQP][CP];
+    if (RG * g < split) passp_load<LW, CP, RQP>(G, ldg, 0, 0, RG * g, k, split, N, sub, slot, xa);
+    for (; RG * g < split; g += NW) {
+        if (RG * (g + NW) < split) passp_load<LW, CP, RQP>(G, ldg, 0, 0, RG * (g + NW), k, split, N, sub, slot, xb);
+        passp_group<LW, CP, RQP>(G, ldg, 0, 0, RG * g, k, M, N, sub, slot, upd, tau, inv, tau2, wr, ur, xa, cbuf,
+                                 rowbuf, pwp);
+#pragma unroll
+        for (int q = 0; q < RQP; ++q)
+#pragma unroll
+            for (int c = 0; c < CP; ++c) xa[q][c] = xb[q][c];
+    }
+    if (RG * g < M) passp_load<LW, CP, RQP>(L, ldl, split, c0, RG * g, k, M, N, sub, slot, xa);
+    for (; RG * g < M; g += NW) {
+        if (RG * (g + NW) < M) passp_load<LW, CP, RQP>(L, ldl, split, c0, RG * (g + NW), k, M, N, sub, slot, xb);
+        passp_group<LW, CP, RQP>(L, ldl, split, c0, RG * g, k, M, N, sub, slot, upd, tau, inv, tau2, wr, ur, xa, cbuf,
+                                 rowbuf, pwp);
+#pragma unroll
+        for (int q = 0; q < RQP; ++q)
+#pragma unroll
+            for (int c = 0; c < CP; ++c) xa[q][c] = xb[q][c];
+    }
+#pragma unroll
+    for (int c = 0; c < CP; ++c) {
+        float v = pwp[c];
+        if (LW == 16) v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401f)); // slot ^ 1
+        v += __int_as_float(__builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, __float_as_int(v)));  // slot ^ (R / 2)
+        const int j = k + sub + LW * c;
+        if (slot == 0 && (c < CP - 1 || j < N)) pwr[(size_t)wv * N + j] = v;
+    }
+    // sum of c_r^2 over this wavefront's rows r > k (its e-th row is row e % RG of its
+    // (e / RG)-th group; lane 0 of each row's slot wrote c_r to cbuf)
+    double ss = 0.0;
+    for (int e = lane;; e += 64) {
+        const int r = RG * (gf + NW * (e / RG)) + e % RG;
+        if (r >= M) break;
+        if (r > k) {
+            const double c = cbuf[r];
+            ss += c * c;
+        }
+    }
+    return wave_sum_dpp_d(ss);
+}
+
+// rows per slot per group of the packed pass: with two, the kernels spill registers (128
+// VGPRs per lane at 4 wavefronts per SIMD)
+#ifndef RQP16_4
+#define RQP16_4 1 // 16 lanes per row, CP <= 4
+#define RQP16_8 1 // 16 lanes per row, CP 5..8
+#define RQP32_8 1 // 32 lanes per row, CP 5..8
+#endif
+
 template <int NT>
 __device__ __attribute__((always_inline)) inline double fused_pass(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0, int split, int M, int N, int i,
                                     float tau, float inv, float tau2, float inv2, lds_f32 *cbuf,
@@ -277,16 +459,37 @@ __device__ __attribute__((always_inline)) inline double fused_pass(glb_f32 *G, i
 #define PACO_PASS(CC)                                                                                              \
     return fused_pass_c<NT, CC>(G, ldg, L, ldl, c0, split, M, N, i, tau, inv, tau2, inv2, cbuf, wbuf, ybuf, rowbuf, \
                                 pwr)
-    switch ((N - i - 1 + 63) >> 6) {
-    case 1: PACO_PASS(1);
-    case 2: PACO_PASS(2);
-    case 3: PACO_PASS(3);
-    case 4: PACO_PASS(4);
+#define PACO_PASSP(LW, CP, RQP)                                                                                   \
+    return fused_pass_p<NT, LW, CP, RQP>(G, ldg, L, ldl, c0, split, M, N, i, tau, inv, tau2, inv2, cbuf, wbuf, ybuf, \
+                                         rowbuf, pwr)
+    const int w = N - i - 1; // trailing width, columns [i + 1, N)
+    if (w <= 128) {          // 16 lanes per row
+        switch ((w + 15) >> 4) {
+        case 1: PACO_PASSP(16, 1, RQP16_4);
+        case 2: PACO_PASSP(16, 2, RQP16_4);
+        case 3: PACO_PASSP(16, 3, RQP16_4);
+        case 4: PACO_PASSP(16, 4, RQP16_4);
+        case 5: PACO_PASSP(16, 5, RQP16_8);
+        case 6: PACO_PASSP(16, 6, RQP16_8);
+        case 7: PACO_PASSP(16, 7, RQP16_8);
+        default: PACO_PASSP(16, 8, RQP16_8);
+        }
+    }
+    if (w <= 256) { // 32 lanes per row
+        switch ((w + 31) >> 5) {
+        case 5: PACO_PASSP(32, 5, RQP32_8);
+        case 6: PACO_PASSP(32, 6, RQP32_8);
+        case 7: PACO_PASSP(32, 7, RQP32_8);
+        default: PACO_PASSP(32, 8, RQP32_8);
+        }
+    }
+    switch ((w + 63) >> 6) {
     case 5: PACO_PASS(5);
     case 6: PACO_PASS(6);
     case 7: PACO_PASS(7);
     default: PACO_PASS(8);
     }
+#undef PACO_PASSP
 #undef PACO_PASS
 }
 
