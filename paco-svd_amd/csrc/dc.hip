@@ -1138,12 +1138,19 @@ __global__ __launch_bounds__(DT) void dc_finalize_kernel(ProbList pl, LevelArgs 
 // side 0: coef = left secular vectors (in a), basis = gathered left bases, out qt;
 // side 1: coef = right secular vectors (vmt), basis = gathered right bases, out wt.
 // ---------------------------------------------------------------------------
+// Workgroup b runs on XCD b % 8 (workgroups are handed to the 8 XCDs in turn): it takes
+// merge (b / 8 / tiles) 8 + b % 8 and that merge's tile (b / 8) % tiles, so all tiles of a
+// merge share one XCD's L2 (its coefficient and basis blocks are read by every tile row
+// and column) while consecutive merges still alternate between XCDs.  The grid holds
+// ceil(ntask / 8) 8 merges; the ones past ntask exit.
 template <int MODE, int TKV>
 __global__ __launch_bounds__(GT) void dc_merge_gemm_kernel(ProbList pl, LevelArgs la, int side)
 {
     const int tpt = (la.stride + 63) / 64, tiles = tpt * tpt;
-    const int tile = blockIdx.x % tiles;
-    const MergeCtx T = merge_ctx(pl, la, blockIdx.x / tiles);
+    const int b = blockIdx.x, slot = b >> 3;
+    const int task = (slot / tiles) * 8 + (b & 7), tile = slot % tiles;
+    if (task >= la.ntask) return;
+    const MergeCtx T = merge_ctx(pl, la, task);
     const MergeState &S = *T.S;
     const ProbDesc P = pl.get(T.prob);
     const int N = P.N, c0 = T.c0, k = S.k;
@@ -1285,7 +1292,7 @@ template <int MODE>
 static void launch_merge_gemms(ProbList pl, LevelArgs la, hipStream_t s)
 {
     const int tpt = (la.stride + 63) / 64;
-    const int grid = la.ntask * tpt * tpt;
+    const int grid = (la.ntask + 7) / 8 * 8 * tpt * tpt; // whole groups of 8 merges (one per XCD)
     // K <= stride: steps of 16 for the short products of the first rounds
     if (la.stride <= 48) {
         hipLaunchKernelGGL((dc_merge_gemm_kernel<MODE, 16>), dim3(grid), dim3(GT), 0, s, pl, la, 0);

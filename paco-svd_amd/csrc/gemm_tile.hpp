@@ -54,7 +54,7 @@ __device__ inline void tile_fetch(const TileCtx &t, int k0, float (&ra)[TileEl<T
         }
         const int gr = t.m0 + r, gk = k0 + kk;
         float v = 0.f;
-        if (gr < t.m && gk < t.K) v = t.transA ? t.A[(size_t)gk * t.lda + gr] : t.A[(size_t)gr * t.lda + gk];
+        if (gr < t.m && gk < t.K) v = t.transA ? gmem(t.A)[(size_t)gk * t.lda + gr] : gmem(t.A)[(size_t)gr * t.lda + gk];
         ra[e] = v;
     }
 #pragma unroll
@@ -71,8 +71,8 @@ __device__ inline void tile_fetch(const TileCtx &t, int k0, float (&ra)[TileEl<T
         const int gc = t.n0 + c, gk = k0 + kk;
         float v = 0.f;
         if (gc < t.n && gk < t.K) {
-            const int krow = t.brow ? t.brow[gk] : gk;
-            v = t.transB ? t.B[(size_t)gc * t.ldb + krow] : t.B[(size_t)krow * t.ldb + gc];
+            const int krow = t.brow ? gmem(t.brow)[gk] : gk;
+            v = t.transB ? gmem(t.B)[(size_t)gc * t.ldb + krow] : gmem(t.B)[(size_t)krow * t.ldb + gc];
         }
         rb[e] = v;
     }
@@ -103,8 +103,8 @@ __device__ inline void tile_store(const TileCtx &t, int r, int c, float v)
 {
     const int gr = t.m0 + r, gc = t.n0 + c;
     if (gr < t.m && gc < t.n) {
-        const int cr = t.crow ? t.crow[gr] : gr;
-        float *p = t.C + (size_t)cr * t.ldc + gc;
+        const int cr = t.crow ? gmem(t.crow)[gr] : gr;
+        __attribute__((address_space(1))) float *p = gmem(t.C) + (size_t)cr * t.ldc + gc;
         float o = t.alpha * v;
         if (t.beta != 0.f) o += t.beta * *p;
         *p = o;

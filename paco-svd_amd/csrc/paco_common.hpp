@@ -7,6 +7,17 @@
 
 namespace paco {
 
+// Global-memory view of a pointer.  Accesses through a generic pointer compile to FLAT
+// instructions, which count in both the vector-memory and the LDS counters, so a wait for
+// an LDS access also waits for every FLAT load in flight: a load issued ahead of an LDS
+// stage to hide its latency is then waited for at once.  Every buffer of the problem
+// descriptors is device memory; through this view its accesses compile to global_* ones.
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T *gmem(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // One SVD problem: the reference's global-memory buffers (main.cpp:33-48) plus
 // the engine's scratch.  All row-major FP32, M >= N.
 struct ProbDesc {

@@ -307,7 +307,7 @@ __device__ inline void qpb_load_v(float (&v)[8], const float *A, int N, int side
         const int r = base + rr, g = i0 + i, unit = g + side;
         float x = 0.f;
         if (i < nb && r < L && r >= unit)
-            x = r == unit ? 1.f : (side == 0 ? A[(size_t)r * N + g] : A[(size_t)g * N + r]);
+            x = r == unit ? 1.f : (side == 0 ? gmem(A)[(size_t)r * N + g] : gmem(A)[(size_t)g * N + r]);
         v[e] = x;
     }
 }
@@ -328,7 +328,7 @@ __device__ inline void qpb_load_x(float (&x)[16], const float *X, int L, int cx,
 #pragma unroll
     for (int kk = 0; kk < 16; ++kk) {
         const int r = base + r0 + 2 * kk;
-        x[kk] = (cok && r < L) ? X[(size_t)r * L + cx] : 0.f;
+        x[kk] = (cok && r < L) ? gmem(X)[(size_t)r * L + cx] : 0.f;
     }
 }
 
@@ -353,7 +353,7 @@ __device__ inline void qpb_strip_block(const ProbDesc &P, int side, int b, int c
     float vn[8], xn[16];
     qpb_load_v(vn, A, N, side, i0, nb, R0, L);
     qpb_load_x(xn, X, L, cx, cok, R0, 32 * kh + lh);
-    for (int q = tid; q < 1024; q += 256) Ts[q >> 5][q & 31] = T[q];
+    for (int q = tid; q < 1024; q += 256) Ts[q >> 5][q & 31] = gmem(T)[q];
     floatx16 acc;
 #pragma unroll
     for (int j = 0; j < 16; ++j) acc[j] = 0.f;
@@ -403,7 +403,7 @@ __device__ inline void qpb_strip_block(const ProbDesc &P, int side, int b, int c
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const int r = base + 32 * kh + (j >> 2) * 8 + lh * 4 + (j & 3);
-            xo[j] = (cok && r < L) ? X[(size_t)r * L + cx] : 0.f;
+            xo[j] = (cok && r < L) ? gmem(X)[(size_t)r * L + cx] : 0.f;
         }
         __syncthreads();
         floatx16 a3;
@@ -416,7 +416,7 @@ __device__ inline void qpb_strip_block(const ProbDesc &P, int side, int b, int c
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const int r = base + 32 * kh + (j >> 2) * 8 + lh * 4 + (j & 3);
-            if (cok && r < L) X[(size_t)r * L + cx] = xo[j] - a3[j];
+            if (cok && r < L) gmem(X)[(size_t)r * L + cx] = xo[j] - a3[j];
         }
     }
     __syncthreads();
