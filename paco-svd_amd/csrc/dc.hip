@@ -1062,7 +1062,7 @@ __global__ __launch_bounds__(DT) void dc_vectors_kernel(ProbList pl, LevelArgs l
     const int N = P.N, c0 = T.c0;
     const Arena Ar = make_arena(P.idx, c0, S.n);
     const double *d = Ar.dk, *z = Ar.zk;
-    float *um = P.a + (size_t)(c0 + i) * N + c0;
+    float *um = P.lsv + (size_t)(c0 + i) * N + c0;
     float *vm = P.vmt + (size_t)(c0 + i) * N + c0;
     const bool left = !la.vonly;
     // coefficient column of pole j: grouped by the support of its basis row (pinv)
@@ -1135,7 +1135,7 @@ __global__ __launch_bounds__(DT) void dc_finalize_kernel(ProbList pl, LevelArgs 
 // merge products (svd.cpp:1504-1516): rows pos[i] of the node block <-
 //   sum_p coef[i][p] * basis[bperm[p]], i, p < k, one 64x64 tile per workgroup (the
 //   coefficient columns grouped by the support of their basis rows, see the deflation).
-// side 0: coef = left secular vectors (in a), basis = gathered left bases, out qt;
+// side 0: coef = left secular vectors (in lsv), basis = gathered left bases, out qt;
 // side 1: coef = right secular vectors (vmt), basis = gathered right bases, out wt.
 // ---------------------------------------------------------------------------
 // Workgroup b runs on XCD b % 8 (workgroups are handed to the 8 XCDs in turn): it takes
@@ -1157,7 +1157,7 @@ __global__ __launch_bounds__(GT) void dc_merge_gemm_kernel(ProbList pl, LevelArg
     const Arena Ar = make_arena(P.idx, c0, S.n);
     TileCtx t;
     const size_t base = (size_t)c0 * N + c0;
-    t.A = (side == 0 ? P.a : P.vmt) + base;
+    t.A = (side == 0 ? P.lsv : P.vmt) + base;
     t.B = (side == 0 ? P.qt : P.wt) + (size_t)N * N + base;
     t.C = (side == 0 ? P.qt : P.wt) + base;
     t.brow = Ar.bperm;

@@ -198,6 +198,7 @@ static ProbDesc bidiag_desc(int M, int N, float *a, float *u, float *vt, float *
     P.M = M;
     P.N = N;
     P.a = a;
+    P.lsv = a;
     P.u = u;
     P.vt = vt;
     P.d = d;
@@ -217,9 +218,21 @@ static bool g_timing = false;
 static hipEvent_t g_tev[kPhases + 1] = {};
 static bool g_tev_recorded = false;
 
+// start and end of the D&C when it runs on its own stream beside Q and P^T (batched path)
+static hipEvent_t g_tev_dc[2] = {};
+static bool g_tev_dc_on = false;
+
 static void tmark(hipStream_t s, int i, bool on)
 {
+    if (on && i == 0) g_tev_dc_on = false;
     if (on && g_tev[i]) (void)hipEventRecord(g_tev[i], s);
+}
+static void tmark_dc(hipStream_t s, int i, bool on)
+{
+    if (on && g_tev_dc[i]) {
+        (void)hipEventRecord(g_tev_dc[i], s);
+        g_tev_dc_on = true;
+    }
 }
 
 // Launch shape of the one-workgroup bidiagonalization for an M x N matrix: matrices
@@ -444,7 +457,7 @@ static int dc_single(int M, int N, float *a, float *u, float *vt, float *d, floa
 // batched path
 // ---------------------------------------------------------------------------
 struct BatchScratch {
-    size_t a, e, tauq, taup, qt, wt, idx, vmt, tq, tp, states, qp, total;
+    size_t a, e, tauq, taup, qt, wt, idx, vmt, lsv, tq, tp, states, qp, total;
 };
 
 static BatchScratch scratch_layout(int M, int N)
@@ -464,6 +477,7 @@ static BatchScratch scratch_layout(int M, int N)
     B.wt = take(sizeof(float) * 2 * (size_t)N * N);
     B.idx = take(sizeof(uint32_t) * 16 * (size_t)N);
     B.vmt = take(sizeof(float) * (size_t)N * N);
+    B.lsv = take(sizeof(float) * (size_t)N * N);
     B.tq = take(sizeof(float) * 1024 * (size_t)cdiv(N, 32));
     B.tp = take(sizeof(float) * 1024 * (size_t)cdiv(std::max(N - 1, 1), 32));
     B.states = take(sizeof(MergeState) * (size_t)std::max(total_merges(N), 1));
@@ -747,6 +761,8 @@ struct BatchRes {
     hipEvent_t ev = nullptr;
     hipStream_t side = nullptr;      // forked stream for the concurrent bidiagonalization
     hipEvent_t evf = nullptr, evj = nullptr;
+    hipStream_t dcs = nullptr;       // forked stream of the D&C (concurrent with Q and P^T)
+    hipEvent_t evdf = nullptr, evdj = nullptr;
 };
 
 // Frees everything a BatchRes holds (device and pinned memory, the forked stream, events).
@@ -758,6 +774,9 @@ static void batch_res_destroy(BatchRes &r)
     if (r.evf) (void)hipEventDestroy(r.evf);
     if (r.evj) (void)hipEventDestroy(r.evj);
     if (r.side) (void)hipStreamDestroy(r.side);
+    if (r.evdf) (void)hipEventDestroy(r.evdf);
+    if (r.evdj) (void)hipEventDestroy(r.evdj);
+    if (r.dcs) (void)hipStreamDestroy(r.dcs);
     r = BatchRes();
 }
 
@@ -784,6 +803,9 @@ static hipError_t ensure(BatchRes &r, size_t dev_bytes, size_t pin_bytes)
     if (!r.side && (e = hipStreamCreateWithFlags(&r.side, hipStreamNonBlocking)) != hipSuccess) return e;
     if (!r.evf && (e = hipEventCreateWithFlags(&r.evf, hipEventDisableTiming)) != hipSuccess) return e;
     if (!r.evj && (e = hipEventCreateWithFlags(&r.evj, hipEventDisableTiming)) != hipSuccess) return e;
+    if (!r.dcs && (e = hipStreamCreateWithFlags(&r.dcs, hipStreamNonBlocking)) != hipSuccess) return e;
+    if (!r.evdf && (e = hipEventCreateWithFlags(&r.evdf, hipEventDisableTiming)) != hipSuccess) return e;
+    if (!r.evdj && (e = hipEventCreateWithFlags(&r.evdj, hipEventDisableTiming)) != hipSuccess) return e;
     return hipSuccess;
 }
 
@@ -834,6 +856,7 @@ static hipError_t prepare_batch(const paco_svd_job *jobs, int count, BatchPlan &
         P.wt = reinterpret_cast<float *>(base + L.wt);
         P.idx = reinterpret_cast<uint32_t *>(base + L.idx);
         P.vmt = reinterpret_cast<float *>(base + L.vmt);
+        P.lsv = reinterpret_cast<float *>(base + L.lsv);
         P.tq = reinterpret_cast<float *>(base + L.tq);
         P.tp = reinterpret_cast<float *>(base + L.tp);
         P.info = info_ext ? info_ext + info_idx[i] : pr.info + i;
@@ -895,35 +918,6 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
             return e;
     if (n_side > 0 && !serial && (e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
     tmark(s, 2, tm);
-    const char *qenv = getenv("PACO_QP_BATCH");
-    if (bp.vonly) {
-        // singular values only: Q and P^T are not formed
-    } else if (qenv && qenv[0] == '0') { // strips of 64 columns, FMA
-        if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
-                           I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)
-            return e;
-    } else { // rounds of grouped GEMMs on the matrix cores
-        if ((e = launch_identity(pl, I(bp.o_idq), bp.nidq, 0, s)) != hipSuccess) return e;
-        if ((e = launch_identity(pl, I(bp.o_idp), bp.nidp, 1, s)) != hipSuccess) return e;
-        if ((e = launch_tfactor(pl, I(bp.o_bq), bp.nbq, 0, s)) != hipSuccess) return e;
-        if ((e = launch_tfactor(pl, I(bp.o_bp), bp.nbp, 1, s)) != hipSuccess) return e;
-        float *const *qp = reinterpret_cast<float *const *>(dev + bp.o_qp);
-        // P's rounds on the forked stream, concurrently with Q's
-        if ((e = hipEventRecord(res.evf, s)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(res.side, res.evf, 0)) != hipSuccess) return e;
-        for (int side = 1; side >= 0; --side) {
-            hipStream_t st = side == 1 ? res.side : s;
-            for (size_t r = 0; r < bp.o_qext[side].size(); ++r)
-                if ((e = launch_qpb_round(pl, qp, side, (int)r, I(bp.o_qext[side][r]), bp.n_qext[side][r],
-                                          I(bp.o_qg1[side][r]), bp.n_qg1[side][r], I(bp.o_qg3[side][r]),
-                                          bp.n_qg3[side][r], st)) != hipSuccess)
-                    return e;
-        }
-        if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
-        if ((e = launch_transpose_vt(pl, I(bp.o_tp), bp.ntp, s)) != hipSuccess) return e;
-    }
-    tmark(s, 3, tm);
     DcOffsets o;
     o.vonly = bp.vonly;
     o.ls = bp.ls;
@@ -945,8 +939,60 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     o.nru = bp.nru;
     o.bt_rv = I(bp.o_rv);
     o.nrv = bp.nrv;
-    hipLaunchKernelGGL(dc_guard_kernel, dim3(count), dim3(256), 0, s, pl, nullptr, bp.ls);
-    if ((e = run_dc_levels(pl, o, s)) != hipSuccess) return e;
+    // The D&C reads d and e and writes only its own buffers (lsv, vmt, qt, wt, idx, the
+    // merge states, d, info); the formation of Q and P^T reads a, tauq and taup and writes u,
+    // vt and the T factors.  With full vectors the D&C runs on a forked stream beside it.
+    // PACO_DC_SERIAL=1: the D&C after Q and P^T on the caller's stream (schedule comparisons).
+    const char *dser = getenv("PACO_DC_SERIAL");
+    const bool dc_fork = !bp.vonly && !(dser && dser[0] == '1');
+    auto run_dc = [&](hipStream_t sd) -> hipError_t {
+        hipLaunchKernelGGL(dc_guard_kernel, dim3(count), dim3(256), 0, sd, pl, nullptr, bp.ls);
+        return run_dc_levels(pl, o, sd);
+    };
+    if (dc_fork) {
+        if ((e = hipEventRecord(res.evdf, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(res.dcs, res.evdf, 0)) != hipSuccess) return e;
+        tmark_dc(res.dcs, 0, tm);
+        if ((e = run_dc(res.dcs)) != hipSuccess) return e;
+        tmark_dc(res.dcs, 1, tm);
+        if ((e = hipEventRecord(res.evdj, res.dcs)) != hipSuccess) return e;
+    }
+    const char *qenv = getenv("PACO_QP_BATCH");
+    if (bp.vonly) {
+        // singular values only: Q and P^T are not formed
+    } else if (qenv && qenv[0] == '0') { // strips of 64 columns, FMA
+        if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
+                           I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)
+            return e;
+    } else { // rounds of grouped GEMMs on the matrix cores
+        if ((e = launch_identity(pl, I(bp.o_idq), bp.nidq, 0, s)) != hipSuccess) return e;
+        if ((e = launch_identity(pl, I(bp.o_idp), bp.nidp, 1, s)) != hipSuccess) return e;
+        if ((e = launch_tfactor(pl, I(bp.o_bq), bp.nbq, 0, s)) != hipSuccess) return e;
+        if ((e = launch_tfactor(pl, I(bp.o_bp), bp.nbp, 1, s)) != hipSuccess) return e;
+        float *const *qp = reinterpret_cast<float *const *>(dev + bp.o_qp);
+        // P's rounds on the forked stream, concurrently with Q's (PACO_QP_FORK=0: after them)
+        const char *qf = getenv("PACO_QP_FORK");
+        const bool qp_fork = !(qf && qf[0] == '0');
+        if ((e = hipEventRecord(res.evf, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(res.side, res.evf, 0)) != hipSuccess) return e;
+        for (int side = 1; side >= 0; --side) {
+            hipStream_t st = side == 1 && qp_fork ? res.side : s;
+            for (size_t r = 0; r < bp.o_qext[side].size(); ++r)
+                if ((e = launch_qpb_round(pl, qp, side, (int)r, I(bp.o_qext[side][r]), bp.n_qext[side][r],
+                                          I(bp.o_qg1[side][r]), bp.n_qg1[side][r], I(bp.o_qg3[side][r]),
+                                          bp.n_qg3[side][r], st)) != hipSuccess)
+                    return e;
+        }
+        if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
+        if ((e = launch_transpose_vt(pl, I(bp.o_tp), bp.ntp, s)) != hipSuccess) return e;
+    }
+    tmark(s, 3, tm);
+    if (dc_fork) {
+        if ((e = hipStreamWaitEvent(s, res.evdj, 0)) != hipSuccess) return e;
+    } else if ((e = run_dc(s)) != hipSuccess) {
+        return e;
+    }
     tmark(s, 4, tm);
     if (!bp.vonly) e = run_dc_backtransform(pl, o, s);
     tmark(s, 5, tm);
@@ -1595,6 +1641,8 @@ int paco_timing_enable(int on)
     if (on) {
         for (int i = 0; i <= kPhases; ++i)
             if (!g_tev[i] && hipEventCreate(&g_tev[i]) != hipSuccess) return PACO_ENOMEM;
+        for (int i = 0; i < 2; ++i)
+            if (!g_tev_dc[i] && hipEventCreate(&g_tev_dc[i]) != hipSuccess) return PACO_ENOMEM;
     }
     g_timing = on != 0;
     return 0;
@@ -1607,7 +1655,13 @@ int paco_timing_read(float *ms, int max)
     int n = 0;
     for (int i = 0; i < kPhases && i < max; ++i) {
         float t = 0.f;
-        if (hipEventElapsedTime(&t, g_tev[i], g_tev[i + 1]) != hipSuccess) t = -1.f;
+        // the D&C beside Q and P^T: its own duration (phases 2 and 3 then overlap)
+        hipEvent_t a = g_tev[i], b = g_tev[i + 1];
+        if (i == 3 && g_tev_dc_on) {
+            a = g_tev_dc[0];
+            b = g_tev_dc[1];
+        }
+        if (hipEventElapsedTime(&t, a, b) != hipSuccess) t = -1.f;
         ms[n++] = t;
     }
     return n;

@@ -22,7 +22,7 @@ __device__ inline __attribute__((address_space(1))) T *gmem(T *p)
 // the engine's scratch.  All row-major FP32, M >= N.
 struct ProbDesc {
     int M, N;
-    float *a;      // M*N   in: A; out of bidiag: reflectors; D&C scratch
+    float *a;      // M*N   in: A; out of bidiag: reflectors; back-transform scratch
     float *u;      // M*M   Q, then left singular vectors
     float *vt;     // N*N   P^T, then right singular vectors (rows)
     float *d;      // N     diagonal, then sigma (descending)
@@ -33,6 +33,10 @@ struct ProbDesc {
     float *wt;     // 2*N*N right vectors of D&C nodes | gathered bases
     uint32_t *idx; // 16*N  per-column arena of the merges
     float *vmt;    // N*N   right secular vectors of the merges
+    float *lsv;    // N*N   left secular vectors of the merges (leading dimension N): the top
+                   //       N x N block of a on the single-matrix paths; a block of their own
+                   //       in the batched scratch, so that the D&C never writes a, which
+                   //       the formation of Q and P^T reads, and the two run concurrently
     float *tq;     // ceil(N/32)*1024 block reflector factors T (Q side)
     float *tp;     // ceil(N/32)*1024 block reflector factors T (P side)
     int *info;     // 1     LAPACK-style status: 0 ok, <0 bad input, >0 no convergence
@@ -142,6 +146,7 @@ __device__ inline ProbDesc uniform_desc(const ProbDesc &P)
     U.wt = uni(P.wt);
     U.idx = uni(P.idx);
     U.vmt = uni(P.vmt);
+    U.lsv = uni(P.lsv);
     U.tq = uni(P.tq);
     U.tp = uni(P.tp);
     U.info = uni(P.info);

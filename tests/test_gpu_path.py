@@ -267,6 +267,34 @@ def test_batched_matches_plain_and_alone(torch_dev):
             assert torch.equal(x, y), i
 
 
+def test_batched_dc_beside_qp_same_bits(torch_dev):
+    """The batched D&C runs on a forked stream beside the formation of Q and P^T (it keeps
+    its left secular vectors in a buffer of its own, so it never writes a, which that
+    formation reads).  The same kernels run in the same order either way: every output
+    must equal the serial schedule's (PACO_DC_SERIAL=1) bit for bit, with accurate results."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(41)
+    mats = mixed_batch(rng, 32, mmax=512)
+    say("dc beside qp", [m.shape for m in mats])
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    forked = pacosvd.svd_batched(As)
+    torch.cuda.synchronize()
+    assert pacosvd.last_info(len(As)) == [0] * len(As)
+    os.environ["PACO_DC_SERIAL"] = "1"
+    try:
+        serial = pacosvd.svd_batched(As)
+        torch.cuda.synchronize()
+    finally:
+        del os.environ["PACO_DC_SERIAL"]
+    for i, A in enumerate(mats):
+        for x, y in zip(forked[i], serial[i]):
+            assert torch.equal(x, y), i
+        m = measures(A, *(np_of(x) for x in forked[i]))
+        assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL, (i, m)
+        assert m["resid_ratio"] <= RESID_TOL
+
+
 @pytest.mark.parametrize("variant", ["strip", "rounds", "bf16x3", "fma_strips"])
 def test_batched_qp_formation_paths(torch_dev, variant):
     """Q and P^T of a batch by every formation path (one strip launch per round on the
