@@ -1138,17 +1138,29 @@ __global__ __launch_bounds__(DT) void dc_finalize_kernel(ProbList pl, LevelArgs 
 // side 0: coef = left secular vectors (in lsv), basis = gathered left bases, out qt;
 // side 1: coef = right secular vectors (vmt), basis = gathered right bases, out wt.
 // ---------------------------------------------------------------------------
-// Workgroup b runs on XCD b % 8 (workgroups are handed to the 8 XCDs in turn): it takes
-// merge (b / 8 / tiles) 8 + b % 8 and that merge's tile (b / 8) % tiles, so all tiles of a
-// merge share one XCD's L2 (its coefficient and basis blocks are read by every tile row
-// and column) while consecutive merges still alternate between XCDs.  The grid holds
-// ceil(ntask / 8) 8 merges; the ones past ntask exit.
+// Rounds of at least MERGE_XCD_MIN merges: workgroup b runs on XCD b % 8 (workgroups are
+// handed to the 8 XCDs in turn) and takes merge (b / 8 / tiles) 8 + b % 8 and that merge's
+// tile (b / 8) % tiles, so all tiles of a merge share one XCD's L2 (its coefficient and
+// basis blocks are read by every tile row and column) while consecutive merges still
+// alternate between XCDs; the grid holds ceil(ntask / 8) 8 merges and the ones past ntask
+// exit.  Rounds of fewer merges (one large matrix: down to a single merge of thousands of
+// tiles) take tile b % tiles of merge b / tiles, spread over every XCD.
+constexpr int MERGE_XCD_MIN = 64;
+__host__ __device__ inline bool merge_xcd_grouped(const LevelArgs &la) { return la.ntask >= MERGE_XCD_MIN; }
+
 template <int MODE, int TKV>
 __global__ __launch_bounds__(GT) void dc_merge_gemm_kernel(ProbList pl, LevelArgs la, int side)
 {
     const int tpt = (la.stride + 63) / 64, tiles = tpt * tpt;
-    const int b = blockIdx.x, slot = b >> 3;
-    const int task = (slot / tiles) * 8 + (b & 7), tile = slot % tiles;
+    const int b = blockIdx.x;
+    int task, tile;
+    if (merge_xcd_grouped(la)) {
+        task = ((b >> 3) / tiles) * 8 + (b & 7);
+        tile = (b >> 3) % tiles;
+    } else {
+        task = b / tiles;
+        tile = b % tiles;
+    }
     if (task >= la.ntask) return;
     const MergeCtx T = merge_ctx(pl, la, task);
     const MergeState &S = *T.S;
@@ -1248,7 +1260,7 @@ __global__ __launch_bounds__(GT) void dc_bt_gemm_kernel(ProbList pl, const int *
     t.m0 = (tile / tn) * 64;
     t.n0 = (tile % tn) * 64;
     if (t.m0 >= t.m) return;
-    tile_gemm<MODE>(t);
+    tile_gemm<MODE, 16>(t);
 }
 
 // copy of the back-transform results into u[:, :N] (side 0) and vt (side 1)
@@ -1292,9 +1304,13 @@ template <int MODE>
 static void launch_merge_gemms(ProbList pl, LevelArgs la, hipStream_t s)
 {
     const int tpt = (la.stride + 63) / 64;
-    const int grid = (la.ntask + 7) / 8 * 8 * tpt * tpt; // whole groups of 8 merges (one per XCD)
-    // K <= stride: steps of 16 for the short products of the first rounds
-    if (la.stride <= 48) {
+    // grouped by XCD: whole groups of 8 merges (one per XCD)
+    const int grid = (merge_xcd_grouped(la) ? (la.ntask + 7) / 8 * 8 : la.ntask) * tpt * tpt;
+    // Merges of up to 512 columns: K steps of 16, 76 VGPRs against 128 at steps of 32, so
+    // six workgroups share a CU instead of four and hide more of the operand loads' latency
+    // (D&C of the 10,000-matrix batch 31.8 -> 29.6 ms).  Larger merges (one large matrix:
+    // few tiles, K up to N) keep steps of 32 (at 16 the 4096^2 D&C took 27.3 ms, not 17.3).
+    if (la.stride <= 512) {
         hipLaunchKernelGGL((dc_merge_gemm_kernel<MODE, 16>), dim3(grid), dim3(GT), 0, s, pl, la, 0);
         hipLaunchKernelGGL((dc_merge_gemm_kernel<MODE, 16>), dim3(grid), dim3(GT), 0, s, pl, la, 1);
     } else {
