@@ -472,6 +472,10 @@ def main():
                 # the measured HBM bytes over the same time: rows cached in LDS are not HBM
                 # traffic, so this, not frac, is the phase's share of the HBM bandwidth
                 traffic_frac=round(traffic / t_bid / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                note="achieved counts the unblocked algorithm's GEMV bytes (two reads of the trailing "
+                     "matrix per column); rows cached in LDS are not re-read from HBM, so achieved can "
+                     "pass the HBM peak; traffic_frac is the measured share of HBM bandwidth, and the "
+                     "kernel is bound by instruction issue and its per-column barriers",
                 traffic_source="profiles/traffic_bidiag_batched_c4.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                "calibrated with the copy kernel)" if traffic else None)
     flops = sum(svd_flops(int(m), int(n)) for m, n in zip(Ms, Ns))
