@@ -270,8 +270,10 @@ def test_batched_matches_plain_and_alone(torch_dev):
 def test_batched_dc_beside_qp_same_bits(torch_dev):
     """The batched D&C runs on a forked stream beside the formation of Q and P^T (it keeps
     its left secular vectors in a buffer of its own, so it never writes a, which that
-    formation reads).  The same kernels run in the same order either way: every output
-    must equal the serial schedule's (PACO_DC_SERIAL=1) bit for bit, with accurate results."""
+    formation reads), and V^T's back-transform follows P's rounds on theirs.  The same
+    kernels run in the same order on each matrix whatever the schedule: every output must
+    equal the serial schedule's (PACO_DC_SERIAL=1) and PACO_QP_FORK=0's bit for bit, with
+    accurate results."""
     torch = torch_dev
     import pacosvd
     rng = np.random.default_rng(41)
@@ -281,15 +283,20 @@ def test_batched_dc_beside_qp_same_bits(torch_dev):
     forked = pacosvd.svd_batched(As)
     torch.cuda.synchronize()
     assert pacosvd.last_info(len(As)) == [0] * len(As)
-    os.environ["PACO_DC_SERIAL"] = "1"
-    try:
-        serial = pacosvd.svd_batched(As)
-        torch.cuda.synchronize()
-    finally:
-        del os.environ["PACO_DC_SERIAL"]
+    others = []
+    # the serial schedule; and the D&C forked with P's rounds after Q's on the caller's
+    # stream (then both back-transforms run there, after the join)
+    for env in ("PACO_DC_SERIAL", "PACO_QP_FORK"):
+        os.environ[env] = "1" if env == "PACO_DC_SERIAL" else "0"
+        try:
+            others.append(pacosvd.svd_batched(As))
+            torch.cuda.synchronize()
+        finally:
+            del os.environ[env]
     for i, A in enumerate(mats):
-        for x, y in zip(forked[i], serial[i]):
-            assert torch.equal(x, y), i
+        for other in others:
+            for x, y in zip(forked[i], other[i]):
+                assert torch.equal(x, y), i
         m = measures(A, *(np_of(x) for x in forked[i]))
         assert m["sigma_err"] <= SIG_TOL and m["orth_u"] <= ORTH_TOL and m["orth_v"] <= ORTH_TOL, (i, m)
         assert m["resid_ratio"] <= RESID_TOL
