@@ -202,14 +202,27 @@ __device__ inline bool arrive_last(unsigned *cnt, int lane)
     return true;
 }
 
+// Sum of squares (FP64) of the LDS floats p[lo, hi) by one wavefront: lane l takes entries
+// lo + l, lo + l + 64, ..., then a wave sum (all 64 lanes active); the order depends on
+// lo and hi alone.
+__device__ inline double wave_sumsq_lds(const lds_f32 *p, int lo, int hi, int lane)
+{
+    double s = 0.0;
+    for (int r = lo + lane; r < hi; r += 64) {
+        const double c = p[r];
+        s += c * c;
+    }
+    return wave_sum_dpp_d(s);
+}
+
 // Fused pass over rows r > i with C = ceil((N - i - 1) / 64) chunks per row.  Rows are
 // taken in groups of RQ at multiples of RQ, group g by wavefront g % NW, so that every
 // wavefront visits its rows in increasing order whatever the LDS split (the reductions
 // do not depend on it) and no group straddles the split (a multiple of RQ below M).
-// Wave partials of pw go to pwr[wave]; the return value is the wavefront's sum of c_r^2 over
-// its rows (FP64, formed from cbuf after the pass: one lane per row, then a wave sum).
+// Wave partials of pw go to pwr[wave]; column k's entries c_r go to cbuf (their sum of
+// squares is formed from there by the wavefront that arrives last).
 template <int NT, int C>
-__device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0, int split, int M, int N, int i,
+__device__ __attribute__((always_inline)) inline void fused_pass_c(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0, int split, int M, int N, int i,
                                       float tau, float inv, float tau2, float inv2, lds_f32 *cbuf,
                                       const lds_f32 *wbuf, const lds_f32 *ybuf, lds_f32 *rowbuf, lds_f32 *pwr)
 {
@@ -255,18 +268,6 @@ __device__ __attribute__((always_inline)) inline double fused_pass_c(glb_f32 *G,
         const int j = k + lane + 64 * c;
         if (c < C - 1 || j < N) pwr[(size_t)wv * N + j] = pwp[c];
     }
-    // sum of c_r^2 over this wavefront's rows r > k (the e-th row of the wavefront is row q
-    // of its (e / RQ)-th group), lane e mod 64 taking row e: lane 0 wrote them to cbuf
-    double ss = 0.0;
-    for (int e = lane;; e += 64) {
-        const int r = RQ * (gf + NW * (e / RQ)) + e % RQ;
-        if (r >= M) break;
-        if (r > k) {
-            const double c = cbuf[r];
-            ss += c * c;
-        }
-    }
-    return wave_sum_dpp_d(ss);
 }
 
 // ---------------------------------------------------------------------------
@@ -378,7 +379,7 @@ __device__ inline void passp_group(T *base, int ld, int rbase, int cb, int r0, i
 // RG = R RQP rows at multiples of RG, group g by wavefront g % NW.  The R slots' partials
 // of pw are added (slot pairs, then pairs of pairs) before the wave's partial goes to pwr.
 template <int NT, int LW, int CP, int RQP>
-__device__ __attribute__((always_inline)) inline double fused_pass_p(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0,
+__device__ __attribute__((always_inline)) inline void fused_pass_p(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0,
                                                                      int split, int M, int N, int i, float tau,
                                                                      float inv, float tau2, float inv2, lds_f32 *cbuf,
                                                                      const lds_f32 *wbuf, const lds_f32 *ybuf,
@@ -429,18 +430,6 @@ __device__ __attribute__((always_inline)) inline double fused_pass_p(glb_f32 *G,
         const int j = k + sub + LW * c;
         if (slot == 0 && (c < CP - 1 || j < N)) pwr[(size_t)wv * N + j] = v;
     }
-    // sum of c_r^2 over this wavefront's rows r > k (its e-th row is row e % RG of its
-    // (e / RG)-th group; lane 0 of each row's slot wrote c_r to cbuf)
-    double ss = 0.0;
-    for (int e = lane;; e += 64) {
-        const int r = RG * (gf + NW * (e / RG)) + e % RG;
-        if (r >= M) break;
-        if (r > k) {
-            const double c = cbuf[r];
-            ss += c * c;
-        }
-    }
-    return wave_sum_dpp_d(ss);
 }
 
 // rows per slot per group of the packed pass: with two, the kernels spill registers (128
@@ -452,7 +441,7 @@ __device__ __attribute__((always_inline)) inline double fused_pass_p(glb_f32 *G,
 #endif
 
 template <int NT>
-__device__ __attribute__((always_inline)) inline double fused_pass(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0, int split, int M, int N, int i,
+__device__ __attribute__((always_inline)) inline void fused_pass(glb_f32 *G, int ldg, lds_f32 *L, int ldl, int c0, int split, int M, int N, int i,
                                     float tau, float inv, float tau2, float inv2, lds_f32 *cbuf,
                                     const lds_f32 *wbuf, const lds_f32 *ybuf, lds_f32 *rowbuf, lds_f32 *pwr)
 {
@@ -574,14 +563,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
                 __syncthreads();
             }
         }
-        const double ss =
-            fused_pass<NT>(G, ldg, L, ldl, c0, split, M, N, i, tau, inv, tau2, inv2, cbuf, wbuf, ubuf, rowbuf, pwr);
+        fused_pass<NT>(G, ldg, L, ldl, c0, split, M, N, i, tau, inv, tau2, inv2, cbuf, wbuf, ubuf, rowbuf, pwr);
         ++i;
-        // the previous column's readers of dred finished before the barrier ahead of the pass
-        if (lane == 0) dred[wv] = ss;
-        // ---- column reflector of column i
+        // ---- column reflector of column i: the last-arriving wavefront sums c_r^2 over the
+        // rows r > i from cbuf (every wavefront wrote its rows' entries before arriving)
         if (arrive_last<NW>(wcnt, lane)) {
-            const double sg = tree_sum<NW, double>(dred, 1);
+            const double sg = wave_sumsq_lds(cbuf, i + 1, M, lane);
             float b, t, v;
             house_params_d(rowbuf[i], sg, &b, &t, &v);
             if (lane == 0) {
@@ -605,7 +592,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
         if (i == N - 1) break;
         // ---- w = tau (row i + (1/v1) pw) and y = row i - w on this thread's columns
         float yv[JPT];
-        double part = 0.0;
 #pragma unroll
         for (int t = 0; t < JPT; ++t) {
             const int j = i + 1 + tid + t * NT;
@@ -616,17 +602,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void bi
                 wbuf[j] = wj;
                 y = rowbuf[j] - wj;
                 ubuf[j] = y;
-                if (j > i + 1) part += (double)y * y;
                 if (j == i + 1) dred[2 * NW] = (double)y;
             }
             yv[t] = y;
         }
-        // a wavefront without columns (the first is column i + 1 + 64 wv) has part = 0
-        if (i + 1 + 64 * wv < N) part = wave_sum_dpp_d(part);
-        if (lane == 0) dred[NW + wv] = part;
-        // ---- row reflector of row i
+        // ---- row reflector of row i: the last-arriving wavefront sums y_j^2 over j > i + 1
+        // from ubuf
         if (arrive_last<NW>(wcnt, lane)) {
-            const double sigma2 = tree_sum<NW, double>(dred + NW, 1);
+            const double sigma2 = wave_sumsq_lds(ubuf, i + 2, N, lane);
             float b, t, v;
             house_params_d(dred[2 * NW], sigma2, &b, &t, &v);
             if (lane == 0) {
