@@ -75,7 +75,7 @@ __global__ __launch_bounds__(GT) void qp_gemm_kernel(GemmDesc G, int kchunk, siz
     t.m0 = (blockIdx.x / G.tiles_n) * 64;
     t.n0 = (blockIdx.x % G.tiles_n) * 64;
     if (t.m0 >= t.m || t.n0 >= t.n || t.K <= 0) return;
-    tile_gemm<MODE>(t);
+    tile_gemm<MODE, 16>(t);
 }
 
 // W[q] = sum_s Wp[s][q], q < count
