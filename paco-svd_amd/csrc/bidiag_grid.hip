@@ -847,7 +847,7 @@ __global__ __launch_bounds__(GT) void gb_trail_kernel(TileCtx base, int tiles_n)
     TileCtx t = base;
     t.m0 = (blockIdx.x / tiles_n) * 64;
     t.n0 = (blockIdx.x % tiles_n) * 64;
-    tile_gemm<MODE>(t);
+    tile_gemm<MODE, 16>(t);
 }
 
 template <int MODE>
