@@ -12,8 +12,8 @@ PACO_* environment variables of each parameter still take precedence):
                           (PACO_RES_THREADS); bidiagonalization phase of the resident
                           matrices of the 10,000-matrix batch
   "leaf_size": ls         columns per D&C leaf (paco_set_leaf_size); D&C phase of the
-                          10,000-matrix batch (the 4096 x 4096 matrix's D&C phase is reported
-                          beside it)
+                          10,000-matrix batch run after Q and P^T (PACO_DC_SERIAL=1; the
+                          4096 x 4096 matrix's D&C phase is reported beside it)
   "grid_tiles:MxN": g     tiles per side of the chip-wide bidiagonalization (PACO_GRID) and
   "grid_switch:MxN": s    the columns it leaves to the one-workgroup kernel
                           (PACO_GRID_SWITCH); bidiagonalization phase of one matrix
@@ -156,6 +156,9 @@ def tune_leaf(torch, pacosvd, table, log, As):
 
     times, times2 = {}, {}
     old = pacosvd.set_leaf_size(0)
+    # the batched D&C alone: beside the formation of Q and P^T (the default schedule) its
+    # duration also depends on how much of the chip it takes from that formation
+    os.environ["PACO_DC_SERIAL"] = "1"
     try:
         for ls in LEAF_SIZES:
             pacosvd.set_leaf_size(ls)
@@ -166,6 +169,7 @@ def tune_leaf(torch, pacosvd, table, log, As):
             say(log[-1])
     finally:
         pacosvd.set_leaf_size(old)
+        del os.environ["PACO_DC_SERIAL"]
     choice = pick(times, 6)
     table["leaf_size"] = choice
     log.append("leaf_size -> %d" % choice)
