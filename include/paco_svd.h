@@ -204,9 +204,8 @@ int paco_gemm_f32(void *stream, int transA, int transB, int m, int n, int k, flo
 /* Phase timing of the batched path with hipEvents recorded on the caller's stream
  * (phases: 0 copy-in, 1 bidiagonalization, 2 Q/P^T formation, 3 D&C leaves + merges,
  * 4 back-transforms).  With full vectors the batched D&C runs on a forked stream beside
- * the Q/P^T formation: phase 3 is then its own duration, overlapping phase 2 (which then
- * ends with Q's rounds; P^T and V^T's back-transform run beside them), and phase 4 is U's
- * back-transform after both.  paco_timing_enable(1) turns recording on for later
+ * the Q/P^T formation: phase 3 is then its own duration, overlapping phase 2, and phase 4
+ * starts when both are done.  paco_timing_enable(1) turns recording on for later
  * paco_svd_batched calls; paco_timing_read synchronises on the last call's events and
  * writes up to max durations in milliseconds; returns the number written.
  * No counterpart in the reference (it has no timers, main.cpp:1). */

@@ -957,8 +957,6 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
         tmark_dc(res.dcs, 1, tm);
         if ((e = hipEventRecord(res.evdj, res.dcs)) != hipSuccess) return e;
     }
-    bool bt_v_done = false; // V^T's back-transform already enqueued (on the forked stream)
-    bool q_marked = false;  // end of Q's rounds marked before the join with P's stream
     const char *qenv = getenv("PACO_QP_BATCH");
     if (bp.vonly) {
         // singular values only: Q and P^T are not formed
@@ -985,40 +983,18 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
                                           bp.n_qg3[side][r], st)) != hipSuccess)
                     return e;
         }
-        if (qp_fork) {
-            // P^T's transpose follows P's rounds on their stream and, with the D&C forked,
-            // so does the back-transform of V^T (it reads vt and wt and writes the second
-            // half of qt and vt; U's reads u and the first half of qt and writes a), beside
-            // Q's last rounds
-            if ((e = launch_transpose_vt(pl, I(bp.o_tp), bp.ntp, res.side)) != hipSuccess) return e;
-            if (dc_fork) {
-                if ((e = hipStreamWaitEvent(res.side, res.evdj, 0)) != hipSuccess) return e;
-                if ((e = launch_dc_backtransform(pl, nullptr, 0, o.bt_tv, o.ntv, nullptr, 0, o.bt_rv, o.nrv,
-                                                 res.side)) != hipSuccess)
-                    return e;
-                bt_v_done = true;
-            }
-            tmark(s, 3, tm); // Q's rounds
-            q_marked = true;
-            if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
-            if ((e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
-        } else {
-            if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
-            if ((e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
-            if ((e = launch_transpose_vt(pl, I(bp.o_tp), bp.ntp, s)) != hipSuccess) return e;
-        }
+        if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
+        if ((e = launch_transpose_vt(pl, I(bp.o_tp), bp.ntp, s)) != hipSuccess) return e;
     }
-    if (!q_marked) tmark(s, 3, tm);
+    tmark(s, 3, tm);
     if (dc_fork) {
         if ((e = hipStreamWaitEvent(s, res.evdj, 0)) != hipSuccess) return e;
     } else if ((e = run_dc(s)) != hipSuccess) {
         return e;
     }
     tmark(s, 4, tm);
-    if (bt_v_done)
-        e = launch_dc_backtransform(pl, o.bt_tu, o.ntu, nullptr, 0, o.bt_ru, o.nru, nullptr, 0, s);
-    else if (!bp.vonly)
-        e = run_dc_backtransform(pl, o, s);
+    if (!bp.vonly) e = run_dc_backtransform(pl, o, s);
     tmark(s, 5, tm);
     return e;
 }

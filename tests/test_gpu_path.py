@@ -270,10 +270,9 @@ def test_batched_matches_plain_and_alone(torch_dev):
 def test_batched_dc_beside_qp_same_bits(torch_dev):
     """The batched D&C runs on a forked stream beside the formation of Q and P^T (it keeps
     its left secular vectors in a buffer of its own, so it never writes a, which that
-    formation reads), and V^T's back-transform follows P's rounds on theirs.  The same
-    kernels run in the same order on each matrix whatever the schedule: every output must
-    equal the serial schedule's (PACO_DC_SERIAL=1) and PACO_QP_FORK=0's bit for bit, with
-    accurate results."""
+    formation reads).  The same kernels run in the same order on each matrix whatever the
+    schedule: every output must equal the serial schedule's (PACO_DC_SERIAL=1) and
+    PACO_QP_FORK=0's (P's rounds after Q's) bit for bit, with accurate results."""
     torch = torch_dev
     import pacosvd
     rng = np.random.default_rng(41)
@@ -285,7 +284,7 @@ def test_batched_dc_beside_qp_same_bits(torch_dev):
     assert pacosvd.last_info(len(As)) == [0] * len(As)
     others = []
     # the serial schedule; and the D&C forked with P's rounds after Q's on the caller's
-    # stream (then both back-transforms run there, after the join)
+    # stream
     for env in ("PACO_DC_SERIAL", "PACO_QP_FORK"):
         os.environ[env] = "1" if env == "PACO_DC_SERIAL" else "0"
         try:
