@@ -228,11 +228,18 @@ int paco_server_submit(void *server, const paco_svd_job *job, uint64_t tag);
 /* Copy up to max finished requests (tag, latency in ms from submit to observed
  * completion; -1 for a request whose enqueue failed) and forget them; returns the count. */
 int paco_server_poll(void *server, uint64_t *tags, double *latency_ms, int max);
+/* paco_server_poll plus each request's info word (LAPACK-style, as paco_last_info: 0 ok,
+ * -1 a NaN / Inf in the bidiagonal, -2 a wait of the chip-wide bidiagonalization timed
+ * out, > 0 secular roots at the iteration limit; 0 for a failed enqueue).  A request
+ * with a nonzero word finished, but its results are not to be trusted as they stand.
+ * No counterpart in the reference (it has no server). */
+int paco_server_poll_info(void *server, uint64_t *tags, double *latency_ms, int *info, int max);
 /* Block until every queued request has finished (or failed); returns the first error or 0. */
 int paco_server_drain(void *server);
 /* Statistics since creation: [completed, p50 ms, p99 ms, mean ms, SVDs/s over first
  * submit .. last completion, batches, mean batch size, most lanes busy at once,
- * wide requests, failed requests]; returns the count written (at most 10). */
+ * wide requests, failed requests, requests with a nonzero info word]; returns the count
+ * written (at most 11). */
 int paco_server_stats(void *server, double *out, int max);
 /* Drain and release the server. */
 void paco_server_destroy(void *server);

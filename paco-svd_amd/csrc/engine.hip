@@ -1095,6 +1095,8 @@ struct Lane {
     BatchRes res;
     size_t capacity = 0;
     int max_jobs = 0;
+    int *info = nullptr; // per-job info words of the last enqueued batch (device)
+    int count = 0;
 };
 
 size_t job_footprint(int M, int N)
@@ -1135,12 +1137,29 @@ int lane_enqueue(Lane *l, const paco_svd_job *jobs, int count, hipStream_t s)
     hipError_t e = prepare_batch(jobs, count, bp, l->res, pr);
     if (e != hipSuccess) return (e == hipErrorInvalidValue) ? PACO_EINVAL : (int)e;
     e = enqueue_batch(bp, l->res, pr, s, true);
+    l->info = pr.info;
+    l->count = count;
     return (int)e;
+}
+
+int lane_info(Lane *l, int *out, int count)
+{
+    if (!l->info || count > l->count) return PACO_EINVAL;
+    return hipMemcpy(out, l->info, sizeof(int) * (size_t)count, hipMemcpyDeviceToHost) == hipSuccess ? 0
+                                                                                                         : PACO_EINVAL;
 }
 
 size_t wide_scratch_bytes(int M, int N) { return scratch_layout(M, N).total + ws_layout(M, N).total; }
 
 bool job_supported(int M, int N) { return M >= N && N >= 1 && N <= PACO_MAX_N && bidiag_fits(M, N); }
+
+int wide_info(const paco_svd_job &job, const void *scratch, int *out)
+{
+    const char *ws = static_cast<const char *>(scratch) + scratch_layout(job.M, job.N).total;
+    return hipMemcpy(out, ws + ws_layout(job.M, job.N).info, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess
+               ? 0
+               : PACO_EINVAL;
+}
 
 int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s, int grid)
 {

@@ -21,6 +21,9 @@ void lane_destroy(Lane *l);
 // Enqueue the batched pipeline of the jobs on stream s (stream-ordered, no host sync
 // beyond the lane's previous plan upload).  Returns 0 or an error code.
 int lane_enqueue(Lane *l, const paco_svd_job *jobs, int count, hipStream_t s);
+// Info words (LAPACK-style, one per job) of the lane's last enqueued batch, copied to the
+// host; call after the lane's stream has passed the batch.  0 or an error code.
+int lane_info(Lane *l, int *out, int count);
 // Whether the engine takes an M x N job (sizes within the kernels' limits).
 bool job_supported(int M, int N);
 // Scratch for one wide job through the two entry points.
@@ -28,5 +31,7 @@ size_t wide_scratch_bytes(int M, int N);
 // grid: tile grid of the chip-wide bidiagonalization (0: default); several wide lanes
 // running at once each take a smaller grid
 int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s, int grid);
+// Info word of the wide job last run in scratch (after its stream has passed it).
+int wide_info(const paco_svd_job &job, const void *scratch, int *out);
 
 } // namespace paco

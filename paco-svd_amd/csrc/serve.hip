@@ -44,6 +44,15 @@ struct Req {
     int err; // nonzero: its enqueue failed; reported with latency -1
 };
 
+// A finished request: tag, latency (-1: its enqueue failed) and its info word (0, or the
+// LAPACK-style status of its SVD: < 0 bad input or a wait that timed out, > 0 roots at
+// the iteration limit; 0 for a failed enqueue, which the latency reports)
+struct Done {
+    uint64_t tag;
+    double ms;
+    int info;
+};
+
 struct ServeLane {
     hipStream_t s = nullptr;
     hipEvent_t done = nullptr;
@@ -63,7 +72,7 @@ struct Server {
     int wide_grid = 0; // tile grid of the wide lanes' chip-wide bidiagonalization
     std::vector<ServeLane> lanes;
     std::deque<Req> pending;
-    std::vector<std::pair<uint64_t, double>> finished; // not yet polled
+    std::vector<Done> finished; // not yet polled
     std::vector<double> lat_ms;
     std::mutex mu;
     std::condition_variable cv;
@@ -71,7 +80,7 @@ struct Server {
     bool stop = false;
     int error = 0;
     double t_first = 0.0, t_last = 0.0;
-    long batches = 0, batched_jobs = 0, wide_jobs = 0, failed = 0;
+    long batches = 0, batched_jobs = 0, wide_jobs = 0, failed = 0, nonzero_info = 0;
     int max_inflight = 0;
     // PACO_SERVE_FAIL_EVERY=n: every n-th admission fails before anything is enqueued
     // (tests of the error path)
@@ -136,7 +145,7 @@ bool Server::admit_locked()
             (void)hipStreamSynchronize(L.s);
             const double t = now_s();
             for (auto &r : take) {
-                finished.emplace_back(r.tag, -1.0);
+                finished.push_back({r.tag, -1.0, 0});
                 ++failed;
             }
             t_last = t;
@@ -159,24 +168,40 @@ void Server::loop()
 {
     for (;;) {
         bool progress = false;
-        // retire finished lanes (outside the lock: hipEventQuery only)
+        // retire finished lanes (outside the lock: hipEventQuery, then the info words of
+        // their requests; a busy lane's requests are touched by nothing else)
         std::vector<int> done_idx;
-        for (size_t i = 0; i < lanes.size(); ++i)
-            if (lanes[i].busy && hipEventQuery(lanes[i].done) == hipSuccess) done_idx.push_back((int)i);
+        std::vector<std::vector<int>> infos;
+        for (size_t i = 0; i < lanes.size(); ++i) {
+            ServeLane &L = lanes[i];
+            if (!L.busy || hipEventQuery(L.done) != hipSuccess) continue;
+            done_idx.push_back((int)i);
+            std::vector<int> inf(L.reqs.size(), 0);
+            bool enq_ok = true;
+            for (auto &r : L.reqs) enq_ok &= r.err == 0;
+            if (enq_ok) {
+                const int rc = L.wide ? wide_info(L.reqs[0].job, L.scratch, inf.data())
+                                      : lane_info(L.lane, inf.data(), (int)inf.size());
+                if (rc != 0) std::fill(inf.begin(), inf.end(), PACO_EINVAL);
+            }
+            infos.push_back(std::move(inf));
+        }
         std::unique_lock<std::mutex> lk(mu);
         if (!done_idx.empty()) {
             const double t = now_s();
-            for (int i : done_idx) {
-                ServeLane &L = lanes[i];
-                for (auto &r : L.reqs) {
+            for (size_t d = 0; d < done_idx.size(); ++d) {
+                ServeLane &L = lanes[done_idx[d]];
+                for (size_t q = 0; q < L.reqs.size(); ++q) {
+                    const Req &r = L.reqs[q];
                     if (r.err) {
-                        finished.emplace_back(r.tag, -1.0);
+                        finished.push_back({r.tag, -1.0, 0});
                         ++failed;
                         continue;
                     }
                     const double ms = 1000.0 * (t - r.t_submit);
                     lat_ms.push_back(ms);
-                    finished.emplace_back(r.tag, ms);
+                    finished.push_back({r.tag, ms, infos[d][q]});
+                    nonzero_info += infos[d][q] != 0;
                 }
                 used -= L.fp;
                 L.reqs.clear();
@@ -268,18 +293,24 @@ int paco_server_submit(void *srv, const paco_svd_job *job, uint64_t tag)
     return 0;
 }
 
-int paco_server_poll(void *srv, uint64_t *tags, double *latency_ms, int max)
+int paco_server_poll_info(void *srv, uint64_t *tags, double *latency_ms, int *info, int max)
 {
     Server *S = static_cast<Server *>(srv);
     if (!S) return PACO_EINVAL;
     std::lock_guard<std::mutex> lk(S->mu);
     const int n = std::min<int>(max, (int)S->finished.size());
     for (int i = 0; i < n; ++i) {
-        if (tags) tags[i] = S->finished[i].first;
-        if (latency_ms) latency_ms[i] = S->finished[i].second;
+        if (tags) tags[i] = S->finished[i].tag;
+        if (latency_ms) latency_ms[i] = S->finished[i].ms;
+        if (info) info[i] = S->finished[i].info;
     }
     S->finished.erase(S->finished.begin(), S->finished.begin() + n);
     return n;
+}
+
+int paco_server_poll(void *srv, uint64_t *tags, double *latency_ms, int max)
+{
+    return paco_server_poll_info(srv, tags, latency_ms, nullptr, max);
 }
 
 int paco_server_drain(void *srv)
@@ -311,7 +342,7 @@ int paco_server_stats(void *srv, double *out, int max)
     for (double x : v) mean += x;
     if (!v.empty()) mean /= (double)v.size();
     const double span = S->t_last > S->t_first ? S->t_last - S->t_first : 0.0;
-    const double vals[10] = {(double)v.size(),
+    const double vals[11] = {(double)v.size(),
                             pct(0.50),
                             pct(0.99),
                             mean,
@@ -320,8 +351,9 @@ int paco_server_stats(void *srv, double *out, int max)
                             S->batches ? (double)S->batched_jobs / (double)S->batches : 0.0,
                             (double)S->max_inflight,
                             (double)S->wide_jobs,
-                            (double)S->failed};
-    const int n = std::min(max, 10);
+                            (double)S->failed,
+                            (double)S->nonzero_info};
+    const int n = std::min(max, 11);
     for (int i = 0; i < n; ++i) out[i] = vals[i];
     return n;
 }

@@ -16,8 +16,8 @@ EXPORTS = [
     "paco_svd_batched_graph", "paco_svd_values_batched", "paco_sgesdd", "paco_svd_plain_noalloc", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
     "paco_timing_enable", "paco_timing_read", "paco_info_offset", "paco_set_secular_maxiter",
     "paco_set_leaf_size", "paco_leaf_size", "paco_merge_stats",
-    "paco_server_create", "paco_server_submit", "paco_server_poll", "paco_server_drain", "paco_server_stats",
-    "paco_server_destroy",
+    "paco_server_create", "paco_server_submit", "paco_server_poll", "paco_server_poll_info", "paco_server_drain",
+    "paco_server_stats", "paco_server_destroy",
 ]
 PHASES = ["copy_in", "bidiag", "form_qp", "dc", "backtransform"]
 
@@ -71,6 +71,8 @@ def lib():
     L.paco_server_create.restype = _vp
     L.paco_server_submit.argtypes = [_vp, ctypes.POINTER(SvdJob), ctypes.c_uint64]
     L.paco_server_poll.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    L.paco_server_poll_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     L.paco_server_drain.argtypes = [_vp]
     L.paco_server_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     L.paco_server_destroy.argtypes = [_vp]
@@ -294,7 +296,7 @@ def timing_read():
 
 
 SERVER_STATS = ["completed", "p50_ms", "p99_ms", "mean_ms", "svds_per_s", "batches", "mean_batch", "max_lanes_busy",
-                "wide_requests", "failed"]
+                "wide_requests", "failed", "nonzero_info"]
 
 
 class Server:
@@ -315,6 +317,14 @@ class Server:
         lat = (ctypes.c_double * max_n)()
         n = lib().paco_server_poll(self.h, tags, lat, max_n)
         return [(int(tags[i]), float(lat[i])) for i in range(n)]
+
+    def poll_info(self, max_n=4096):
+        """Finished requests as (tag, latency ms, info word)."""
+        tags = (ctypes.c_uint64 * max_n)()
+        lat = (ctypes.c_double * max_n)()
+        info = (ctypes.c_int * max_n)()
+        n = lib().paco_server_poll_info(self.h, tags, lat, info, max_n)
+        return [(int(tags[i]), float(lat[i]), int(info[i])) for i in range(n)]
 
     def drain(self):
         _check(lib().paco_server_drain(self.h), "paco_server_drain")

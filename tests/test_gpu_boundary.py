@@ -215,6 +215,45 @@ def test_over_limit_shape_refused_before_launch(torch_dev):
         pacosvd.svd_batched([a], outs=[(z, u, u)])
 
 
+def test_server_reports_info_words(torch_dev):
+    """A served request's info word reaches the caller (paco_server_poll_info): a wide
+    request whose chip-wide bidiagonalization is forced to time out (the host sets the
+    bound of its waits to 0) reports -2, the requests around it 0, and the server goes on."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(37)
+    shapes = [(1100, 1000), (200, 150), (1100, 1000), (64, 40)]
+    mats = [rng.standard_normal(sh).astype(np.float32) for sh in shapes]
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    outs = pacosvd.alloc_outputs(As)
+    say("server info words", shapes)
+    srv = pacosvd.Server(2 << 30, max_batch=4, batch_lanes=1, wide_lanes=1, max_M=1100, max_N=1100)
+    try:
+        assert srv.submit(As[0], *outs[0], 0) == 0
+        srv.drain()
+        os.environ["PACO_RES_TIMEOUT_US"] = "0"
+        try:
+            assert srv.submit(As[2], *outs[2], 2) == 0
+            srv.drain()
+        finally:
+            del os.environ["PACO_RES_TIMEOUT_US"]
+        for i in (1, 3):
+            assert srv.submit(As[i], *outs[i], i) == 0
+        srv.drain()
+        fin = srv.poll_info()
+        st = srv.stats()
+    finally:
+        srv.close()
+    say("info words", fin, st)
+    info = {t: inf for t, _, inf in fin}
+    assert sorted(info) == [0, 1, 2, 3] and all(ms >= 0 for _, ms, _ in fin)
+    assert info[2] == -2 and info[0] == 0 and info[1] == 0 and info[3] == 0, info
+    assert st["nonzero_info"] == 1
+    for t in (0, 1, 3):
+        ok, m = good(mats[t], outs[t])
+        assert ok, (t, m)
+
+
 def test_server_reports_failed_enqueue(torch_dev):
     torch = torch_dev
     import pacosvd

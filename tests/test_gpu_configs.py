@@ -164,12 +164,15 @@ def test_c5_serving_stream(torch_dev):
                     pass
                 assert srv.submit(As[i], *outs[i], i) == 0
             srv.drain()
-            done = dict(srv.poll(4096))
+            fin = srv.poll_info(4096)
             st = srv.stats()
         finally:
             srv.close()
         say("C5 served", st)
+        done = {t: ms for t, ms, _ in fin}
         assert sorted(done) == list(range(count)) and all(ms >= 0 for ms in done.values())
+        # every request's info word: no wait timed out, no NaN, every secular root converged
+        assert all(info == 0 for _, _, info in fin) and st["nonzero_info"] == 0, [f for f in fin if f[2]]
         worst = {}
         for i in range(count):
             m = measures_gpu(torch, As[i], *outs[i], refs[i].result())
