@@ -238,8 +238,11 @@ int paco_server_poll_info(void *server, uint64_t *tags, double *latency_ms, int 
 int paco_server_drain(void *server);
 /* Statistics since creation: [completed, p50 ms, p99 ms, mean ms, SVDs/s over first
  * submit .. last completion, batches, mean batch size, most lanes busy at once,
- * wide requests, failed requests, requests with a nonzero info word]; returns the count
- * written (at most 11). */
+ * wide requests, failed requests, requests with a nonzero info word, wide requests run
+ * again]; returns the count written (at most 12).  A wide request whose chip-wide
+ * bidiagonalization timed out (-2: its workgroups could not all be resident beside the
+ * other lanes' work) runs once more, without kernels whose workgroups wait on each other,
+ * before it is reported. */
 int paco_server_stats(void *server, double *out, int max);
 /* Drain and release the server. */
 void paco_server_destroy(void *server);

@@ -931,14 +931,16 @@ bool bidiag_grid_supported(int M, int N)
     return bidiag_wg_lds_bytes(M - p, N - p, 1024) <= bidiag_wg_lds_limit();
 }
 
-hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int grid)
+hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int grid, bool no_wait)
 {
     const int M = P.M, N = P.N;
     GridWs W;
     const size_t panel_ws = carve(M, N, static_cast<char *>(ws), &W);
-    // panels up to p_end, then the resident kernel (from p_res) or the one-workgroup kernel
+    // panels up to p_end, then the resident kernel (from p_res) or the one-workgroup kernel;
+    // no_wait: no kernel whose workgroups wait on each other (panels two launches per
+    // column, then the one-workgroup kernel), for a solve that timed out beside other work
     const int preq = res_request(grid);
-    const int p_res = res_start(M, N, preq);
+    const int p_res = no_wait ? -1 : res_start(M, N, preq);
     const int p_end = p_res >= 0 ? p_res : grid_tail_start(M, N);
     if (p_res < 0 && bidiag_wg_lds_bytes(M - p_end, N - p_end, 1024) > bidiag_wg_lds_limit())
         return hipErrorInvalidValue;
@@ -948,7 +950,7 @@ hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int gr
     int G = grid > 0 ? grid : genv ? atoi(genv) : tuned_grid_tiles(M, N, 16);
     G = G < 4 ? 4 : G > GMAX ? GMAX : G;
     const char *penv = getenv("PACO_GRID_PERSIST");
-    const bool persist = !(penv && penv[0] == '0');
+    const bool persist = !no_wait && !(penv && penv[0] == '0');
     static int cus = -1;
     static long long timeout = 0;
     static size_t maxdyn = 0;

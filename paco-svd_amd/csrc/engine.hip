@@ -303,8 +303,9 @@ static hipError_t form_q_pt(ProbList pl, const int *idq, int nidq, const int *id
 
 // vonly: singular values only (Q and P^T are not formed; u, vt are not touched)
 // grid: tile grid of the chip-wide kernel (0: default)
+// no_wait: the chip-wide path without kernels whose workgroups wait on each other
 static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, float *e, float *tauq, float *taup,
-                         void *workspace, hipStream_t s, bool vonly = false, int grid = 0)
+                         void *workspace, hipStream_t s, bool vonly = false, int grid = 0, bool no_wait = false)
 {
     const WsLayout L = ws_layout(M, N);
     char *ws = static_cast<char *>(workspace ? workspace : internal_ws(s, L.total));
@@ -323,7 +324,7 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
         tmark(s, 1, tm);
     }
     if (use_grid_bidiag(M, N)) {
-        err = launch_bidiag_grid(pl.single, ws + L.grid, s, grid);
+        err = launch_bidiag_grid(pl.single, ws + L.grid, s, grid, no_wait);
     } else {
         const BidiagChoice c = bidiag_choice(M, N);
         if (!c.resident && blocked_bidiag(M, N, c.nt))
@@ -1161,7 +1162,7 @@ int wide_info(const paco_svd_job &job, const void *scratch, int *out)
                : PACO_EINVAL;
 }
 
-int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s, int grid)
+int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s, int grid, bool no_wait)
 {
     const int M = job.M, N = job.N;
     const BatchScratch L = scratch_layout(M, N);
@@ -1172,7 +1173,7 @@ int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s, int grid
     if (e != hipSuccess) return (int)e;
     int rc = bidiag_single(M, N, a, job.U, job.Vt, job.S, reinterpret_cast<float *>(base + L.e),
                            reinterpret_cast<float *>(base + L.tauq), reinterpret_cast<float *>(base + L.taup), ws, s,
-                           false, grid);
+                           false, grid, no_wait);
     if (rc) return rc;
     return dc_single(M, N, a, job.U, job.Vt, job.S, reinterpret_cast<float *>(base + L.e),
                      reinterpret_cast<float *>(base + L.qt), reinterpret_cast<float *>(base + L.wt),

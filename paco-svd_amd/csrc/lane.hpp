@@ -29,8 +29,9 @@ bool job_supported(int M, int N);
 // Scratch for one wide job through the two entry points.
 size_t wide_scratch_bytes(int M, int N);
 // grid: tile grid of the chip-wide bidiagonalization (0: default); several wide lanes
-// running at once each take a smaller grid
-int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s, int grid);
+// running at once each take a smaller grid; no_wait: without kernels whose workgroups wait
+// on each other (the re-run of a solve whose waits timed out beside other work)
+int wide_enqueue(const paco_svd_job &job, void *scratch, hipStream_t s, int grid, bool no_wait = false);
 // Info word of the wide job last run in scratch (after its stream has passed it).
 int wide_info(const paco_svd_job &job, const void *scratch, int *out);
 

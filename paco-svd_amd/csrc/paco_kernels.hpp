@@ -111,8 +111,10 @@ hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hip
 size_t bidiag_grid_ws_bytes(int M, int N);
 // whether the LDS of the chip-wide path's kernels holds an M x N problem
 bool bidiag_grid_supported(int M, int N);
-// grid: tiles per side of the panel kernel's tile grid (0: the default, 16, or PACO_GRID)
-hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int grid = 0);
+// grid: tiles per side of the panel kernel's tile grid (0: the default, 16, or PACO_GRID);
+// no_wait: only kernels whose workgroups never wait on each other (panels launched phase
+// by phase, then the one-workgroup kernel)
+hipError_t launch_bidiag_grid(const ProbDesc &P, void *ws, hipStream_t s, int grid = 0, bool no_wait = false);
 // bidiag_res.hip: the whole matrix in the registers of P workgroups (one per CU)
 size_t bidiag_res_ws_bytes(int N);
 // workgroups the kernel takes for an M x N matrix when preq are asked for (0: it does not

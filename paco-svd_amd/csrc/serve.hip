@@ -42,6 +42,9 @@ struct Req {
     size_t fp;
     bool wide;
     int err; // nonzero: its enqueue failed; reported with latency -1
+    // a wide request whose chip-wide bidiagonalization timed out (info -2) beside the other
+    // lanes' work runs once more, without kernels whose workgroups wait on each other
+    bool retry = false;
 };
 
 // A finished request: tag, latency (-1: its enqueue failed) and its info word (0, or the
@@ -80,7 +83,7 @@ struct Server {
     bool stop = false;
     int error = 0;
     double t_first = 0.0, t_last = 0.0;
-    long batches = 0, batched_jobs = 0, wide_jobs = 0, failed = 0, nonzero_info = 0;
+    long batches = 0, batched_jobs = 0, wide_jobs = 0, failed = 0, nonzero_info = 0, retried = 0;
     int max_inflight = 0;
     // PACO_SERVE_FAIL_EVERY=n: every n-th admission fails before anything is enqueued
     // (tests of the error path)
@@ -124,7 +127,7 @@ bool Server::admit_locked()
         if (fail_every > 0 && admissions % fail_every == 0) {
             rc = PACO_EINJECTED;
         } else if (L.wide) {
-            rc = wide_enqueue(take[0].job, L.scratch, L.s, wide_grid);
+            rc = wide_enqueue(take[0].job, L.scratch, L.s, wide_grid, take[0].retry);
             ++wide_jobs;
         } else {
             std::vector<paco_svd_job> jobs;
@@ -192,10 +195,16 @@ void Server::loop()
             for (size_t d = 0; d < done_idx.size(); ++d) {
                 ServeLane &L = lanes[done_idx[d]];
                 for (size_t q = 0; q < L.reqs.size(); ++q) {
-                    const Req &r = L.reqs[q];
+                    Req &r = L.reqs[q];
                     if (r.err) {
                         finished.push_back({r.tag, -1.0, 0});
                         ++failed;
+                        continue;
+                    }
+                    if (L.wide && infos[d][q] == -2 && !r.retry) {
+                        r.retry = true;
+                        pending.push_front(r);
+                        ++retried;
                         continue;
                     }
                     const double ms = 1000.0 * (t - r.t_submit);
@@ -342,7 +351,7 @@ int paco_server_stats(void *srv, double *out, int max)
     for (double x : v) mean += x;
     if (!v.empty()) mean /= (double)v.size();
     const double span = S->t_last > S->t_first ? S->t_last - S->t_first : 0.0;
-    const double vals[11] = {(double)v.size(),
+    const double vals[12] = {(double)v.size(),
                             pct(0.50),
                             pct(0.99),
                             mean,
@@ -352,8 +361,9 @@ int paco_server_stats(void *srv, double *out, int max)
                             (double)S->max_inflight,
                             (double)S->wide_jobs,
                             (double)S->failed,
-                            (double)S->nonzero_info};
-    const int n = std::min(max, 11);
+                            (double)S->nonzero_info,
+                            (double)S->retried};
+    const int n = std::min(max, 12);
     for (int i = 0; i < n; ++i) out[i] = vals[i];
     return n;
 }

@@ -296,7 +296,7 @@ def timing_read():
 
 
 SERVER_STATS = ["completed", "p50_ms", "p99_ms", "mean_ms", "svds_per_s", "batches", "mean_batch", "max_lanes_busy",
-                "wide_requests", "failed", "nonzero_info"]
+                "wide_requests", "failed", "nonzero_info", "retried"]
 
 
 class Server:
