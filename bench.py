@@ -209,7 +209,7 @@ def serving_bench(torch, pacosvd, count, rate, budget_gb=64.0, wide_lanes=8):
                 mean_ms=round(st.get("mean_ms", 0.0), 2), batches=int(st.get("batches", 0)),
                 mean_batch=round(st.get("mean_batch", 0.0), 1), max_lanes_busy=int(st.get("max_lanes_busy", 0)),
                 wide_requests=int(st.get("wide_requests", 0)), clustered=sum(r["clustered"] for r in reqs),
-                nonzero_info=int(st.get("nonzero_info", 0)),
+                nonzero_info=int(st.get("nonzero_info", 0)), retried=int(st.get("retried", 0)),
                 offered_tflops=round(flops * rate / count / 1e12, 2),
                 achieved_tflops=round(flops / max(t_all, 1e-9) / 1e12, 3),
                 plain_total_s=round(t_plain, 3), plain_svds_per_s=round(count / t_plain, 1),
