@@ -968,16 +968,18 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     } else { // rounds of grouped GEMMs on the matrix cores
         if ((e = launch_identity(pl, I(bp.o_idq), bp.nidq, 0, s)) != hipSuccess) return e;
         if ((e = launch_identity(pl, I(bp.o_idp), bp.nidp, 1, s)) != hipSuccess) return e;
-        if ((e = launch_tfactor(pl, I(bp.o_bq), bp.nbq, 0, s)) != hipSuccess) return e;
-        if ((e = launch_tfactor(pl, I(bp.o_bp), bp.nbp, 1, s)) != hipSuccess) return e;
         float *const *qp = reinterpret_cast<float *const *>(dev + bp.o_qp);
-        // P's rounds on the forked stream, concurrently with Q's (PACO_QP_FORK=0: after them)
+        // P's T factors and rounds on the forked stream, concurrently with Q's
+        // (PACO_QP_FORK=0: after them)
         const char *qf = getenv("PACO_QP_FORK");
         const bool qp_fork = !(qf && qf[0] == '0');
         if ((e = hipEventRecord(res.evf, s)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(res.side, res.evf, 0)) != hipSuccess) return e;
         for (int side = 1; side >= 0; --side) {
             hipStream_t st = side == 1 && qp_fork ? res.side : s;
+            if ((e = launch_tfactor(pl, I(side == 0 ? bp.o_bq : bp.o_bp), side == 0 ? bp.nbq : bp.nbp, side, st)) !=
+                hipSuccess)
+                return e;
             for (size_t r = 0; r < bp.o_qext[side].size(); ++r)
                 if ((e = launch_qpb_round(pl, qp, side, (int)r, I(bp.o_qext[side][r]), bp.n_qext[side][r],
                                           I(bp.o_qg1[side][r]), bp.n_qg1[side][r], I(bp.o_qg3[side][r]),
