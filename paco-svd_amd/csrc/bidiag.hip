@@ -895,19 +895,25 @@ __global__ __launch_bounds__(BT) void tfactor_kernel(ProbList pl, const int *blk
             const int r = (j >> 2) * 8 + lh * 4 + (j & 3);
             G[r][l32] = ((acc[j] + Gp[0][r][l32]) + Gp[1][r][l32]) + Gp[2][r][l32];
         }
-        for (int q = lane; q < 32 * 32; q += 64) T[q / 32][q % 32] = 0.f;
         __builtin_amdgcn_wave_barrier();
-        // T[0:t, t] = -tau_t * T[0:t, 0:t] * G[0:t, t]: one wavefront, lane i < t forms
-        // row i; LDS accesses of one wavefront complete in order
-        for (int t = 0; t < nb; ++t) {
-            const float tt = refl_tau(P, side, i0 + t);
+        // T[0:t, t] = -tau_t * T[0:t, 0:t] * G[0:t, t]: row i of the product needs only row i
+        // of T, so lane i < 32 keeps its row in registers and forms it alone (G read as
+        // broadcasts; LDS accesses of one wavefront complete in order).  Entries left of the
+        // diagonal are zero, so the sums over q >= i add the same terms in the same order
+        // as sums started at q = i.
+        const float taul = lane < nb ? refl_tau(P, side, i0 + lane) : 0.f;
+        float tr[32];
+#pragma unroll
+        for (int t = 0; t < 32; ++t) {
+            const float tt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(taul), t));
             float s = 0.f;
-            if (lane < t)
-                for (int q = lane; q < t; ++q) s += T[lane][q] * G[q][t];
-            __builtin_amdgcn_wave_barrier();
-            if (lane < t) T[lane][t] = -tt * s;
-            if (lane == t) T[t][t] = tt;
-            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int q = 0; q < t; ++q) s += tr[q] * G[q][t];
+            tr[t] = t >= nb ? 0.f : (lane < t ? -tt * s : (lane == t ? tt : 0.f));
+        }
+        if (lane < 32) {
+#pragma unroll
+            for (int t = 0; t < 32; ++t) T[lane][t] = tr[t];
         }
     }
     __syncthreads();
@@ -1001,19 +1007,26 @@ __global__ __launch_bounds__(BT) void wy_apply_kernel(ProbList pl, const int *st
 }
 
 // X = identity (L x L), grid over (problem, group of ROW_GROUP rows) through offsets.
-__global__ __launch_bounds__(BT) void identity_kernel(ProbList pl, const int *row_off, int side)
+// Rows [r0, r0 + rows) of X = I (L x L, row-major): one contiguous range of floats, written
+// in order by the whole workgroup; entry q is 1 where q is a multiple of L + 1.
+__global__ __launch_bounds__(BT) void identity_kernel(ProbList pl, const int *grp_off, int side, int rows)
 {
-    const int pid = find_task(row_off, pl.count, blockIdx.x);
-    const int g = blockIdx.x - off_at(row_off, pid);
+    const int pid = find_task(grp_off, pl.count, blockIdx.x);
+    const int g = blockIdx.x - off_at(grp_off, pid);
     const ProbDesc P = pl.get(pid);
     const int L = side == 0 ? P.M : P.N;
     float *X = side == 0 ? P.u : P.vt;
-    const int r0 = g * ROW_GROUP, nr = min(ROW_GROUP, L - r0);
-    if (nr <= 0) return;
-    // 32 threads per row (ROW_GROUP * 32 == BT)
-    const int r = r0 + (int)(threadIdx.x >> 5);
-    if (r >= r0 + nr) return;
-    for (int c = threadIdx.x & 31; c < L; c += 32) X[(size_t)r * L + c] = (c == r) ? 1.f : 0.f;
+    const int r0 = g * rows;
+    if (r0 >= L) return;
+    const size_t hi = (size_t)min(r0 + rows, L) * L;
+    const unsigned L1 = (unsigned)L + 1, step = BT % L1;
+    size_t q = (size_t)r0 * L + threadIdx.x;
+    unsigned m = (unsigned)(q % L1);
+    for (; q < hi; q += BT) {
+        gmem(X)[q] = m == 0 ? 1.f : 0.f;
+        m += step;
+        m = m >= L1 ? m - L1 : m;
+    }
 }
 
 // In-place transpose of vt (N x N) by 32x32 tile pairs; grid over (problem, tile pair).
@@ -1134,10 +1147,10 @@ hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int s
     return hipGetLastError();
 }
 
-hipError_t launch_identity(ProbList pl, const int *row_off, int ngroups, int side, hipStream_t s)
+hipError_t launch_identity(ProbList pl, const int *grp_off, int ngroups, int side, hipStream_t s, int rows)
 {
     if (ngroups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(identity_kernel, dim3(ngroups), dim3(BT), 0, s, pl, row_off, side);
+    hipLaunchKernelGGL(identity_kernel, dim3(ngroups), dim3(BT), 0, s, pl, grp_off, side, rows);
     return hipGetLastError();
 }
 

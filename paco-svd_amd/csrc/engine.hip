@@ -292,8 +292,8 @@ static hipError_t form_q_pt(ProbList pl, const int *idq, int nidq, const int *id
                             int ntp, hipStream_t s, int sw = 64)
 {
     hipError_t e;
-    if ((e = launch_identity(pl, idq, nidq, 0, s)) != hipSuccess) return e;
-    if ((e = launch_identity(pl, idp, nidp, 1, s)) != hipSuccess) return e;
+    if ((e = launch_identity(pl, idq, nidq, 0, s, IDENT_ROWS)) != hipSuccess) return e;
+    if ((e = launch_identity(pl, idp, nidp, 1, s, IDENT_ROWS)) != hipSuccess) return e;
     if ((e = launch_tfactor(pl, bq, nbq, 0, s)) != hipSuccess) return e;
     if ((e = launch_tfactor(pl, bp, nbp, 1, s)) != hipSuccess) return e;
     if ((e = launch_wy_apply(pl, sq, nsq, 0, s, sw)) != hipSuccess) return e;
@@ -341,8 +341,8 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     const int nt = cdiv(N, 32);
     if (large_matrix(M, N)) {
         // blocks of 32 reflectors applied to the whole trailing part of Q, P as GEMMs
-        if ((err = launch_identity(pl, nullptr, cdiv(M, ROW_GROUP), 0, s)) != hipSuccess) return (int)err;
-        if ((err = launch_identity(pl, nullptr, cdiv(N, ROW_GROUP), 1, s)) != hipSuccess) return (int)err;
+        if ((err = launch_identity(pl, nullptr, cdiv(M, ROW_GROUP), 0, s, ROW_GROUP)) != hipSuccess) return (int)err;
+        if ((err = launch_identity(pl, nullptr, cdiv(N, ROW_GROUP), 1, s, ROW_GROUP)) != hipSuccess) return (int)err;
         if ((err = launch_tfactor(pl, nullptr, cdiv(N, 32), 0, s)) != hipSuccess) return (int)err;
         if ((err = launch_tfactor(pl, nullptr, cdiv(std::max(N - 1, 0), 32), 1, s)) != hipSuccess) return (int)err;
         if ((err = launch_qp_gemm(pl.single, 0, ws + L.qp, s)) != hipSuccess) return (int)err;
@@ -679,8 +679,8 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         bp.o_order_wave[g] = blob_put(b, ow[g]);
     }
     bp.o_rowin = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, ROW_GROUP); }, bp.nrowin));
-    bp.o_idq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, ROW_GROUP); }, bp.nidq));
-    bp.o_idp = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, ROW_GROUP); }, bp.nidp));
+    bp.o_idq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, IDENT_ROWS); }, bp.nidq));
+    bp.o_idp = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, IDENT_ROWS); }, bp.nidp));
     {
         std::vector<float *> qpp(count, nullptr); // device addresses patched at upload
         bp.o_qp = blob_put(b, qpp);
@@ -966,10 +966,8 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
                            I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)
             return e;
     } else { // rounds of grouped GEMMs on the matrix cores
-        if ((e = launch_identity(pl, I(bp.o_idq), bp.nidq, 0, s)) != hipSuccess) return e;
-        if ((e = launch_identity(pl, I(bp.o_idp), bp.nidp, 1, s)) != hipSuccess) return e;
         float *const *qp = reinterpret_cast<float *const *>(dev + bp.o_qp);
-        // P's T factors and rounds on the forked stream, concurrently with Q's
+        // P's identity, T factors and rounds on the forked stream, concurrently with Q's
         // (PACO_QP_FORK=0: after them)
         const char *qf = getenv("PACO_QP_FORK");
         const bool qp_fork = !(qf && qf[0] == '0');
@@ -977,6 +975,9 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
         if ((e = hipStreamWaitEvent(res.side, res.evf, 0)) != hipSuccess) return e;
         for (int side = 1; side >= 0; --side) {
             hipStream_t st = side == 1 && qp_fork ? res.side : s;
+            if ((e = launch_identity(pl, I(side == 0 ? bp.o_idq : bp.o_idp), side == 0 ? bp.nidq : bp.nidp, side, st,
+                                     IDENT_ROWS)) != hipSuccess)
+                return e;
             if ((e = launch_tfactor(pl, I(side == 0 ? bp.o_bq : bp.o_bp), side == 0 ? bp.nbq : bp.nbp, side, st)) !=
                 hipSuccess)
                 return e;

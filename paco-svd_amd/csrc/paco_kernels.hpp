@@ -103,8 +103,10 @@ __device__ inline void copy_block(float *dst, const float *src, int n)
         for (int q = threadIdx.x; q < n; q += 256) dst[q] = src[q];
     }
 }
-// X = I; row_off counts groups of ROW_GROUP rows per problem
-hipError_t launch_identity(ProbList pl, const int *row_off, int ngroups, int side, hipStream_t s);
+// X = I; grp_off counts groups of `rows` rows per problem (IDENT_ROWS for batches: fewer,
+// larger workgroups; ROW_GROUP for one large matrix, so that every CU has a group)
+constexpr int IDENT_ROWS = 32;
+hipError_t launch_identity(ProbList pl, const int *grp_off, int ngroups, int side, hipStream_t s, int rows);
 hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hipStream_t s);
 
 // bidiag_grid.hip (large single matrices)
