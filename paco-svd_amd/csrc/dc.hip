@@ -1264,20 +1264,31 @@ __global__ __launch_bounds__(GT) void dc_bt_gemm_kernel(ProbList pl, const int *
 }
 
 // copy of the back-transform results into u[:, :N] (side 0) and vt (side 1)
-// rows [ROW_GROUP g, ROW_GROUP (g + 1)) of the back-transformed vectors into U (its first
-// N columns) or Vt
-__global__ __launch_bounds__(DT) void dc_bt_copy_kernel(ProbList pl, const int *row_off, int side)
+// rows [rows g, rows (g + 1)) of the back-transformed vectors into U (its first N columns)
+// or Vt
+__global__ __launch_bounds__(DT) void dc_bt_copy_kernel(ProbList pl, const int *row_off, int side, int rows)
 {
     const int pid = find_task(row_off, pl.count, blockIdx.x);
     const int g = blockIdx.x - off_at(row_off, pid);
     const ProbDesc P = pl.get(pid);
-    const int M = P.M, N = P.N, r0 = g * ROW_GROUP;
-    const int nr = min(ROW_GROUP, (side == 0 ? M : N) - r0);
+    const int M = P.M, N = P.N, r0 = g * rows;
+    const int nr = min(rows, (side == 0 ? M : N) - r0);
     if (nr <= 0) return;
-    if (side == 0) { // 32 threads per row (ROW_GROUP * 32 == DT)
-        const int r = r0 + (int)(threadIdx.x >> 5);
-        if (r >= r0 + nr) return;
-        for (int c = threadIdx.x & 31; c < N; c += 32) P.u[(size_t)r * M + c] = P.a[(size_t)r * N + c];
+    if (side == 0) { // the group's rows are contiguous in a (leading dimension N): element q of
+                     // the group is row q / N, column q % N, tracked incrementally
+        const float *src = P.a + (size_t)r0 * N;
+        float *dst = P.u + (size_t)r0 * M;
+        const int n = nr * N, dr = DT / N, dc = DT % N;
+        int r = (int)threadIdx.x / N, c = (int)threadIdx.x % N;
+        for (int q = threadIdx.x; q < n; q += DT) {
+            gmem(dst)[(size_t)r * M + c] = gmem(src)[q];
+            r += dr;
+            c += dc;
+            if (c >= N) {
+                c -= N;
+                ++r;
+            }
+        }
     } else {
         copy_block(P.vt + (size_t)r0 * N, P.qt + (size_t)N * N + (size_t)r0 * N, nr * N);
     }
@@ -1372,7 +1383,7 @@ static void launch_bt(ProbList pl, const int *tou, int ntu, const int *tov, int 
 
 hipError_t launch_dc_backtransform(ProbList pl, const int *tile_off_u, int ntiles_u, const int *tile_off_v,
                                    int ntiles_v, const int *row_off_u, int nrows_u, const int *row_off_v,
-                                   int nrows_v, hipStream_t s)
+                                   int nrows_v, int rows, hipStream_t s)
 {
     const int mode = gemm_get_mode();
     if (mode == 1)
@@ -1381,8 +1392,8 @@ hipError_t launch_dc_backtransform(ProbList pl, const int *tile_off_u, int ntile
         launch_bt<2>(pl, tile_off_u, ntiles_u, tile_off_v, ntiles_v, s);
     else
         launch_bt<0>(pl, tile_off_u, ntiles_u, tile_off_v, ntiles_v, s);
-    if (nrows_u > 0) hipLaunchKernelGGL(dc_bt_copy_kernel, dim3(nrows_u), dim3(DT), 0, s, pl, row_off_u, 0);
-    if (nrows_v > 0) hipLaunchKernelGGL(dc_bt_copy_kernel, dim3(nrows_v), dim3(DT), 0, s, pl, row_off_v, 1);
+    if (nrows_u > 0) hipLaunchKernelGGL(dc_bt_copy_kernel, dim3(nrows_u), dim3(DT), 0, s, pl, row_off_u, 0, rows);
+    if (nrows_v > 0) hipLaunchKernelGGL(dc_bt_copy_kernel, dim3(nrows_v), dim3(DT), 0, s, pl, row_off_v, 1, rows);
     return hipGetLastError();
 }
 

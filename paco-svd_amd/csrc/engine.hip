@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void copy_in_kernel(ProbList pl, const float *
     const int pid = find_task(row_off, pl.count, blockIdx.x);
     const int g = blockIdx.x - off_at(row_off, pid);
     const ProbDesc P = pl.get(pid);
-    const int r0 = g * ROW_GROUP, nr = min(ROW_GROUP, P.M - r0);
+    const int r0 = g * BATCH_ROW_GROUP, nr = min(BATCH_ROW_GROUP, P.M - r0);
     if (nr <= 0) return;
     // the rows of a group are contiguous in both (leading dimension N)
     copy_block(P.a + (size_t)r0 * P.N, src[pid] + (size_t)r0 * P.N, nr * P.N);
@@ -286,14 +286,15 @@ static bool bidiag_fits(int M, int N)
     return bidiag_choice(M, N).lds <= bidiag_wg_lds_limit();
 }
 
-// Q and P^T from the reflectors (GetUVt, upper_bidiagonalization.cpp:774-893), blocked WY.
+// Q and P^T from the reflectors (GetUVt, upper_bidiagonalization.cpp:774-893), blocked WY;
+// idq / idp count groups of id_rows rows.
 static hipError_t form_q_pt(ProbList pl, const int *idq, int nidq, const int *idp, int nidp, const int *bq, int nbq,
                             const int *bp, int nbp, const int *sq, int nsq, const int *sp, int nsp, const int *tpair,
-                            int ntp, hipStream_t s, int sw = 64)
+                            int ntp, hipStream_t s, int sw, int id_rows)
 {
     hipError_t e;
-    if ((e = launch_identity(pl, idq, nidq, 0, s, IDENT_ROWS)) != hipSuccess) return e;
-    if ((e = launch_identity(pl, idp, nidp, 1, s, IDENT_ROWS)) != hipSuccess) return e;
+    if ((e = launch_identity(pl, idq, nidq, 0, s, id_rows)) != hipSuccess) return e;
+    if ((e = launch_identity(pl, idp, nidp, 1, s, id_rows)) != hipSuccess) return e;
     if ((e = launch_tfactor(pl, bq, nbq, 0, s)) != hipSuccess) return e;
     if ((e = launch_tfactor(pl, bp, nbp, 1, s)) != hipSuccess) return e;
     if ((e = launch_wy_apply(pl, sq, nsq, 0, s, sw)) != hipSuccess) return e;
@@ -354,7 +355,7 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
     // a single matrix: 16-column strips unless 64-column ones already give every CU a strip
     const int sw = cdiv(M, 64) >= 512 ? 64 : 16;
     err = form_q_pt(pl, nullptr, cdiv(M, ROW_GROUP), nullptr, cdiv(N, ROW_GROUP), nullptr, cdiv(N, 32), nullptr, cdiv(std::max(N - 1, 0), 32), nullptr,
-                    cdiv(M, sw), nullptr, cdiv(N, sw), nullptr, nt * (nt + 1) / 2, s, sw);
+                    cdiv(M, sw), nullptr, cdiv(N, sw), nullptr, nt * (nt + 1) / 2, s, sw, ROW_GROUP);
     tmark(s, 3, tm);
     return (int)err;
 }
@@ -371,6 +372,7 @@ struct DcOffsets {
     const int *bt_tu = nullptr, *bt_tv = nullptr, *bt_ru = nullptr, *bt_rv = nullptr;
     bool vonly = false; // singular values only (no left vectors, two right-vector columns per node)
     int ntu = 0, ntv = 0, nru = 0, nrv = 0;
+    int bt_rows = ROW_GROUP; // rows per group of the row offsets
 };
 
 static hipError_t run_dc_levels(ProbList pl, const DcOffsets &o, hipStream_t s)
@@ -398,7 +400,7 @@ static hipError_t run_dc_levels(ProbList pl, const DcOffsets &o, hipStream_t s)
 
 static hipError_t run_dc_backtransform(ProbList pl, const DcOffsets &o, hipStream_t s)
 {
-    return launch_dc_backtransform(pl, o.bt_tu, o.ntu, o.bt_tv, o.ntv, o.bt_ru, o.nru, o.bt_rv, o.nrv, s);
+    return launch_dc_backtransform(pl, o.bt_tu, o.ntu, o.bt_tv, o.ntv, o.bt_ru, o.nru, o.bt_rv, o.nrv, o.bt_rows, s);
 }
 
 static hipError_t run_dc(ProbList pl, const DcOffsets &o, hipStream_t s)
@@ -678,9 +680,9 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
         bp.n_wave[g] = (int)ow[g].size();
         bp.o_order_wave[g] = blob_put(b, ow[g]);
     }
-    bp.o_rowin = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, ROW_GROUP); }, bp.nrowin));
-    bp.o_idq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, IDENT_ROWS); }, bp.nidq));
-    bp.o_idp = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, IDENT_ROWS); }, bp.nidp));
+    bp.o_rowin = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, BATCH_ROW_GROUP); }, bp.nrowin));
+    bp.o_idq = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, BATCH_ROW_GROUP); }, bp.nidq));
+    bp.o_idp = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, BATCH_ROW_GROUP); }, bp.nidp));
     {
         std::vector<float *> qpp(count, nullptr); // device addresses patched at upload
         bp.o_qp = blob_put(b, qpp);
@@ -745,8 +747,8 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     }
     bp.o_tu = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, 64) * cdiv(jobs[i].N, 64); }, bp.ntu));
     bp.o_tv = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, 64) * cdiv(jobs[i].N, 64); }, bp.ntv));
-    bp.o_ru = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, ROW_GROUP); }, bp.nru));
-    bp.o_rv = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, ROW_GROUP); }, bp.nrv));
+    bp.o_ru = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].M, BATCH_ROW_GROUP); }, bp.nru));
+    bp.o_rv = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, BATCH_ROW_GROUP); }, bp.nrv));
     std::vector<MergeState> states(std::max<size_t>(sacc, 1));
     bp.o_info = blob_put(b, std::vector<int>(count, 0));
     // merge states live after the blob in device memory (not uploaded)
@@ -937,6 +939,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
     o.bt_tv = I(bp.o_tv);
     o.ntv = bp.ntv;
     o.bt_ru = I(bp.o_ru);
+    o.bt_rows = BATCH_ROW_GROUP;
     o.nru = bp.nru;
     o.bt_rv = I(bp.o_rv);
     o.nrv = bp.nrv;
@@ -963,7 +966,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
         // singular values only: Q and P^T are not formed
     } else if (qenv && qenv[0] == '0') { // strips of 64 columns, FMA
         if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
-                           I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s)) != hipSuccess)
+                           I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s, 64, BATCH_ROW_GROUP)) != hipSuccess)
             return e;
     } else { // rounds of grouped GEMMs on the matrix cores
         float *const *qp = reinterpret_cast<float *const *>(dev + bp.o_qp);
@@ -976,7 +979,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
         for (int side = 1; side >= 0; --side) {
             hipStream_t st = side == 1 && qp_fork ? res.side : s;
             if ((e = launch_identity(pl, I(side == 0 ? bp.o_idq : bp.o_idp), side == 0 ? bp.nidq : bp.nidp, side, st,
-                                     IDENT_ROWS)) != hipSuccess)
+                                     BATCH_ROW_GROUP)) != hipSuccess)
                 return e;
             if ((e = launch_tfactor(pl, I(side == 0 ? bp.o_bq : bp.o_bp), side == 0 ? bp.nbq : bp.nbp, side, st)) !=
                 hipSuccess)

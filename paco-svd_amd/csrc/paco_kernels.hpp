@@ -88,6 +88,9 @@ hipError_t launch_wy_apply(ProbList pl, const int *strip_off, int nstrips, int s
 // Row-wise kernels (identity, copies) take ROW_GROUP rows per workgroup: one workgroup
 // per row made the launch bound by workgroup dispatch.  Their offsets count groups.
 constexpr int ROW_GROUP = 8;
+// batches: groups of 32 rows (a matrix of the batch is small, so 8-row groups made
+// hundreds of thousands of workgroups of a few KiB each)
+constexpr int BATCH_ROW_GROUP = 32;
 static_assert(ROW_GROUP * 32 == 256, "row-wise kernels: 32 threads per row of a 256-thread workgroup");
 // n contiguous floats by one workgroup of 256 threads: 16-byte vectors when both ends are
 // aligned, else single floats
@@ -103,9 +106,8 @@ __device__ inline void copy_block(float *dst, const float *src, int n)
         for (int q = threadIdx.x; q < n; q += 256) dst[q] = src[q];
     }
 }
-// X = I; grp_off counts groups of `rows` rows per problem (IDENT_ROWS for batches: fewer,
-// larger workgroups; ROW_GROUP for one large matrix, so that every CU has a group)
-constexpr int IDENT_ROWS = 32;
+// X = I; grp_off counts groups of `rows` rows per problem (BATCH_ROW_GROUP for batches;
+// ROW_GROUP for one large matrix)
 hipError_t launch_identity(ProbList pl, const int *grp_off, int ngroups, int side, hipStream_t s, int rows);
 hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hipStream_t s);
 
@@ -128,10 +130,11 @@ hipError_t launch_bidiag_res(const ProbDesc &P, void *ws, hipStream_t s, int pre
 // leaves of ls columns (leaf_off: prefix of the leaf counts over the problems, or null)
 hipError_t launch_dc_leaves(ProbList pl, const int *leaf_off, int nleaf, int ls, hipStream_t s);
 hipError_t launch_dc_level(ProbList pl, LevelArgs la, hipStream_t s);
-// back-transform: U[:, :N] <- U[:, :N] qt^T and Vt <- wt Vt for every problem
+// back-transform: U[:, :N] <- U[:, :N] qt^T and Vt <- wt Vt for every problem; the row
+// offsets count groups of `rows` rows
 hipError_t launch_dc_backtransform(ProbList pl, const int *tile_off_u, int ntiles_u, const int *tile_off_v,
                                    int ntiles_v, const int *row_off_u, int nrows_u, const int *row_off_v,
-                                   int nrows_v, hipStream_t s);
+                                   int nrows_v, int rows, hipStream_t s);
 
 // qp_gemm.hip: Q, P of one large matrix by blocked GEMMs (X already the identity, T factors
 // formed)
