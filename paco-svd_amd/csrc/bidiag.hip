@@ -1030,35 +1030,39 @@ __global__ __launch_bounds__(BT) void identity_kernel(ProbList pl, const int *gr
 }
 
 // In-place transpose of vt (N x N) by 32x32 tile pairs; grid over (problem, tile pair).
-__global__ __launch_bounds__(BT) void transpose_vt_kernel(ProbList pl, const int *pair_off)
+// A workgroup swaps `per` consecutive tile pairs (pair_off counts groups of `per` pairs).
+__global__ __launch_bounds__(BT) void transpose_vt_kernel(ProbList pl, const int *pair_off, int per)
 {
     const int pid = find_task(pair_off, pl.count, blockIdx.x);
-    int q = blockIdx.x - off_at(pair_off, pid);
+    const int g = blockIdx.x - off_at(pair_off, pid);
     const ProbDesc P = pl.get(pid);
-    const int N = P.N, nt = (N + 31) / 32;
-    int bi = 0;
-    while (q >= nt - bi) {
-        q -= nt - bi;
-        ++bi;
-    }
-    const int bj = bi + q;
+    const int N = P.N, nt = (N + 31) / 32, qend = min(g * per + per, nt * (nt + 1) / 2);
     __shared__ float t1[32][33], t2[32][33];
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 8 rows per pass
     float *X = P.vt;
-    for (int y = ty; y < 32; y += 8) {
-        int r = bi * 32 + y, c = bj * 32 + tx;
-        t1[y][tx] = (r < N && c < N) ? X[(size_t)r * N + c] : 0.f;
-        r = bj * 32 + y;
-        c = bi * 32 + tx;
-        t2[y][tx] = (r < N && c < N) ? X[(size_t)r * N + c] : 0.f;
-    }
-    __syncthreads();
-    for (int y = ty; y < 32; y += 8) {
-        int r = bi * 32 + y, c = bj * 32 + tx;
-        if (r < N && c < N) X[(size_t)r * N + c] = t2[tx][y];
-        r = bj * 32 + y;
-        c = bi * 32 + tx;
-        if (r < N && c < N) X[(size_t)r * N + c] = t1[tx][y];
+    for (int q0 = g * per; q0 < qend; ++q0) {
+        int q = q0, bi = 0;
+        while (q >= nt - bi) {
+            q -= nt - bi;
+            ++bi;
+        }
+        const int bj = bi + q;
+        __syncthreads(); // the previous pair's reads of the tiles are done
+        for (int y = ty; y < 32; y += 8) {
+            int r = bi * 32 + y, c = bj * 32 + tx;
+            t1[y][tx] = (r < N && c < N) ? X[(size_t)r * N + c] : 0.f;
+            r = bj * 32 + y;
+            c = bi * 32 + tx;
+            t2[y][tx] = (r < N && c < N) ? X[(size_t)r * N + c] : 0.f;
+        }
+        __syncthreads();
+        for (int y = ty; y < 32; y += 8) {
+            int r = bi * 32 + y, c = bj * 32 + tx;
+            if (r < N && c < N) X[(size_t)r * N + c] = t2[tx][y];
+            r = bj * 32 + y;
+            c = bi * 32 + tx;
+            if (r < N && c < N) X[(size_t)r * N + c] = t1[tx][y];
+        }
     }
 }
 
@@ -1154,10 +1158,10 @@ hipError_t launch_identity(ProbList pl, const int *grp_off, int ngroups, int sid
     return hipGetLastError();
 }
 
-hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hipStream_t s)
+hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int ngroups, hipStream_t s, int per)
 {
-    if (npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(transpose_vt_kernel, dim3(npairs), dim3(BT), 0, s, pl, pair_off);
+    if (ngroups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(transpose_vt_kernel, dim3(ngroups), dim3(BT), 0, s, pl, pair_off, per);
     return hipGetLastError();
 }
 

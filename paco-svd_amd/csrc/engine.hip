@@ -287,10 +287,10 @@ static bool bidiag_fits(int M, int N)
 }
 
 // Q and P^T from the reflectors (GetUVt, upper_bidiagonalization.cpp:774-893), blocked WY;
-// idq / idp count groups of id_rows rows.
+// idq / idp count groups of id_rows rows, tpair groups of tp_per tile pairs.
 static hipError_t form_q_pt(ProbList pl, const int *idq, int nidq, const int *idp, int nidp, const int *bq, int nbq,
                             const int *bp, int nbp, const int *sq, int nsq, const int *sp, int nsp, const int *tpair,
-                            int ntp, hipStream_t s, int sw, int id_rows)
+                            int ntp, hipStream_t s, int sw, int id_rows, int tp_per)
 {
     hipError_t e;
     if ((e = launch_identity(pl, idq, nidq, 0, s, id_rows)) != hipSuccess) return e;
@@ -299,7 +299,7 @@ static hipError_t form_q_pt(ProbList pl, const int *idq, int nidq, const int *id
     if ((e = launch_tfactor(pl, bp, nbp, 1, s)) != hipSuccess) return e;
     if ((e = launch_wy_apply(pl, sq, nsq, 0, s, sw)) != hipSuccess) return e;
     if ((e = launch_wy_apply(pl, sp, nsp, 1, s, sw)) != hipSuccess) return e;
-    return launch_transpose_vt(pl, tpair, ntp, s);
+    return launch_transpose_vt(pl, tpair, ntp, s, tp_per);
 }
 
 // vonly: singular values only (Q and P^T are not formed; u, vt are not touched)
@@ -348,14 +348,14 @@ static int bidiag_single(int M, int N, float *a, float *u, float *vt, float *d, 
         if ((err = launch_tfactor(pl, nullptr, cdiv(std::max(N - 1, 0), 32), 1, s)) != hipSuccess) return (int)err;
         if ((err = launch_qp_gemm(pl.single, 0, ws + L.qp, s)) != hipSuccess) return (int)err;
         if ((err = launch_qp_gemm(pl.single, 1, ws + L.qp, s)) != hipSuccess) return (int)err;
-        err = launch_transpose_vt(pl, nullptr, nt * (nt + 1) / 2, s);
+        err = launch_transpose_vt(pl, nullptr, nt * (nt + 1) / 2, s, 1);
         tmark(s, 3, tm);
         return (int)err;
     }
     // a single matrix: 16-column strips unless 64-column ones already give every CU a strip
     const int sw = cdiv(M, 64) >= 512 ? 64 : 16;
     err = form_q_pt(pl, nullptr, cdiv(M, ROW_GROUP), nullptr, cdiv(N, ROW_GROUP), nullptr, cdiv(N, 32), nullptr, cdiv(std::max(N - 1, 0), 32), nullptr,
-                    cdiv(M, sw), nullptr, cdiv(N, sw), nullptr, nt * (nt + 1) / 2, s, sw, ROW_GROUP);
+                    cdiv(M, sw), nullptr, cdiv(N, sw), nullptr, nt * (nt + 1) / 2, s, sw, ROW_GROUP, 1);
     tmark(s, 3, tm);
     return (int)err;
 }
@@ -717,7 +717,7 @@ static void build_plan(const paco_svd_job *jobs, int count, BatchPlan &bp, std::
     bp.o_sp = blob_put(b, prefix(count, [&](int i) { return cdiv(jobs[i].N, 64); }, bp.nsp));
     bp.o_tp = blob_put(b, prefix(count, [&](int i) {
         const int nt = cdiv(jobs[i].N, 32);
-        return nt * (nt + 1) / 2;
+        return cdiv(nt * (nt + 1) / 2, TP_GROUP);
     }, bp.ntp));
     const int ls = bp.ls;
     bp.o_leaf = blob_put(b, prefix(count, [&](int i) { return leaf_count(jobs[i].N, ls); }, bp.nleaf));
@@ -966,7 +966,8 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
         // singular values only: Q and P^T are not formed
     } else if (qenv && qenv[0] == '0') { // strips of 64 columns, FMA
         if ((e = form_q_pt(pl, I(bp.o_idq), bp.nidq, I(bp.o_idp), bp.nidp, I(bp.o_bq), bp.nbq, I(bp.o_bp), bp.nbp,
-                           I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s, 64, BATCH_ROW_GROUP)) != hipSuccess)
+                           I(bp.o_sq), bp.nsq, I(bp.o_sp), bp.nsp, I(bp.o_tp), bp.ntp, s, 64, BATCH_ROW_GROUP,
+                           TP_GROUP)) != hipSuccess)
             return e;
     } else { // rounds of grouped GEMMs on the matrix cores
         float *const *qp = reinterpret_cast<float *const *>(dev + bp.o_qp);
@@ -992,7 +993,7 @@ static hipError_t enqueue_batch(const BatchPlan &bp, BatchRes &res, const Prepar
         }
         if ((e = hipEventRecord(res.evj, res.side)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(s, res.evj, 0)) != hipSuccess) return e;
-        if ((e = launch_transpose_vt(pl, I(bp.o_tp), bp.ntp, s)) != hipSuccess) return e;
+        if ((e = launch_transpose_vt(pl, I(bp.o_tp), bp.ntp, s, TP_GROUP)) != hipSuccess) return e;
     }
     tmark(s, 3, tm);
     if (dc_fork) {

@@ -109,7 +109,10 @@ __device__ inline void copy_block(float *dst, const float *src, int n)
 // X = I; grp_off counts groups of `rows` rows per problem (BATCH_ROW_GROUP for batches;
 // ROW_GROUP for one large matrix)
 hipError_t launch_identity(ProbList pl, const int *grp_off, int ngroups, int side, hipStream_t s, int rows);
-hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int npairs, hipStream_t s);
+// in-place transpose of vt by 32 x 32 tile pairs; pair_off counts groups of `per` pairs
+// (TP_GROUP for batches, 1 for one large matrix)
+constexpr int TP_GROUP = 4;
+hipError_t launch_transpose_vt(ProbList pl, const int *pair_off, int ngroups, hipStream_t s, int per);
 
 // bidiag_grid.hip (large single matrices)
 size_t bidiag_grid_ws_bytes(int M, int N);
