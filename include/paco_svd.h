@@ -234,15 +234,22 @@ int paco_server_poll(void *server, uint64_t *tags, double *latency_ms, int max);
  * with a nonzero word finished, but its results are not to be trusted as they stand.
  * No counterpart in the reference (it has no server). */
 int paco_server_poll_info(void *server, uint64_t *tags, double *latency_ms, int *info, int max);
+/* Withdraw a queued request that no lane has taken yet: it is never enqueued, its output
+ * buffers are left untouched and its tag never comes back from poll.  Returns 0 when
+ * the request was withdrawn, 1 when no queued request has that tag (already admitted,
+ * finished, being run again after a timed-out wait, or never submitted: it then
+ * completes as usual), PACO_EINVAL for a NULL server.  With several queued requests of
+ * one tag, the oldest is withdrawn.  No counterpart in the reference (it has no server). */
+int paco_server_cancel(void *server, uint64_t tag);
 /* Block until every queued request has finished (or failed); returns the first error or 0. */
 int paco_server_drain(void *server);
 /* Statistics since creation: [completed, p50 ms, p99 ms, mean ms, SVDs/s over first
  * submit .. last completion, batches, mean batch size, most lanes busy at once,
  * wide requests, failed requests, requests with a nonzero info word, wide requests run
- * again]; returns the count written (at most 12).  A wide request whose chip-wide
- * bidiagonalization timed out (-2: its workgroups could not all be resident beside the
- * other lanes' work) runs once more, without kernels whose workgroups wait on each other,
- * before it is reported. */
+ * again, requests withdrawn by paco_server_cancel]; returns the count written (at most
+ * 13).  A wide request whose chip-wide bidiagonalization timed out (-2: its workgroups
+ * could not all be resident beside the other lanes' work) runs once more, without
+ * kernels whose workgroups wait on each other, before it is reported. */
 int paco_server_stats(void *server, double *out, int max);
 /* Drain and release the server. */
 void paco_server_destroy(void *server);

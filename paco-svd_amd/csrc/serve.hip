@@ -83,7 +83,7 @@ struct Server {
     bool stop = false;
     int error = 0;
     double t_first = 0.0, t_last = 0.0;
-    long batches = 0, batched_jobs = 0, wide_jobs = 0, failed = 0, nonzero_info = 0, retried = 0;
+    long batches = 0, batched_jobs = 0, wide_jobs = 0, failed = 0, nonzero_info = 0, retried = 0, cancelled = 0;
     int max_inflight = 0;
     // PACO_SERVE_FAIL_EVERY=n: every n-th admission fails before anything is enqueued
     // (tests of the error path)
@@ -302,6 +302,22 @@ int paco_server_submit(void *srv, const paco_svd_job *job, uint64_t tag)
     return 0;
 }
 
+int paco_server_cancel(void *srv, uint64_t tag)
+{
+    Server *S = static_cast<Server *>(srv);
+    if (!S) return PACO_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->pending.begin();
+        while (it != S->pending.end() && (it->tag != tag || it->retry)) ++it;
+        if (it == S->pending.end()) return 1; // admitted, finished or unknown
+        S->pending.erase(it);
+        ++S->cancelled;
+    }
+    S->cv.notify_all(); // drain may be waiting for the queue to empty
+    return 0;
+}
+
 int paco_server_poll_info(void *srv, uint64_t *tags, double *latency_ms, int *info, int max)
 {
     Server *S = static_cast<Server *>(srv);
@@ -351,7 +367,7 @@ int paco_server_stats(void *srv, double *out, int max)
     for (double x : v) mean += x;
     if (!v.empty()) mean /= (double)v.size();
     const double span = S->t_last > S->t_first ? S->t_last - S->t_first : 0.0;
-    const double vals[12] = {(double)v.size(),
+    const double vals[13] = {(double)v.size(),
                             pct(0.50),
                             pct(0.99),
                             mean,
@@ -362,8 +378,9 @@ int paco_server_stats(void *srv, double *out, int max)
                             (double)S->wide_jobs,
                             (double)S->failed,
                             (double)S->nonzero_info,
-                            (double)S->retried};
-    const int n = std::min(max, 12);
+                            (double)S->retried,
+                            (double)S->cancelled};
+    const int n = std::min(max, 13);
     for (int i = 0; i < n; ++i) out[i] = vals[i];
     return n;
 }

@@ -16,7 +16,7 @@ EXPORTS = [
     "paco_svd_batched_graph", "paco_svd_values_batched", "paco_sgesdd", "paco_svd_plain_noalloc", "paco_batch_workspace_size", "paco_last_info", "paco_release", "paco_gemm_f32",
     "paco_timing_enable", "paco_timing_read", "paco_info_offset", "paco_set_secular_maxiter",
     "paco_set_leaf_size", "paco_leaf_size", "paco_merge_stats",
-    "paco_server_create", "paco_server_submit", "paco_server_poll", "paco_server_poll_info", "paco_server_drain",
+    "paco_server_create", "paco_server_submit", "paco_server_poll", "paco_server_poll_info", "paco_server_cancel", "paco_server_drain",
     "paco_server_stats", "paco_server_destroy",
 ]
 PHASES = ["copy_in", "bidiag", "form_qp", "dc", "backtransform"]
@@ -73,6 +73,7 @@ def lib():
     L.paco_server_poll.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     L.paco_server_poll_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    L.paco_server_cancel.argtypes = [_vp, ctypes.c_uint64]
     L.paco_server_drain.argtypes = [_vp]
     L.paco_server_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     L.paco_server_destroy.argtypes = [_vp]
@@ -296,7 +297,7 @@ def timing_read():
 
 
 SERVER_STATS = ["completed", "p50_ms", "p99_ms", "mean_ms", "svds_per_s", "batches", "mean_batch", "max_lanes_busy",
-                "wide_requests", "failed", "nonzero_info", "retried"]
+                "wide_requests", "failed", "nonzero_info", "retried", "cancelled"]
 
 
 class Server:
@@ -325,6 +326,14 @@ class Server:
         info = (ctypes.c_int * max_n)()
         n = lib().paco_server_poll_info(self.h, tags, lat, info, max_n)
         return [(int(tags[i]), float(lat[i]), int(info[i])) for i in range(n)]
+
+    def cancel(self, tag):
+        """Withdraw a queued request (paco_server_cancel): True when it was withdrawn,
+        False when no queued request has that tag (it then completes as usual)."""
+        rc = lib().paco_server_cancel(self.h, tag)
+        if rc not in (0, 1):
+            _check(rc, "paco_server_cancel")
+        return rc == 0
 
     def drain(self):
         _check(lib().paco_server_drain(self.h), "paco_server_drain")

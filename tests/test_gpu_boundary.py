@@ -288,3 +288,45 @@ def test_server_reports_failed_enqueue(torch_dev):
         if ms >= 0:
             ok, m = good(mats[t], outs[t])
             assert ok, (t, m)
+
+
+
+def test_server_cancel_withdraws_queued_requests(torch_dev):
+    """paco_server_cancel: a withdrawn request is never run (its outputs keep their
+    sentinel), never comes back from poll and is counted in the statistics; every other
+    request completes accurately; unknown and finished tags are not withdrawn."""
+    torch = torch_dev
+    import pacosvd
+    rng = np.random.default_rng(37)
+    shapes = [(int(m), int(n)) for m, n in zip(rng.integers(64, 257, 24), rng.integers(16, 65, 24))]
+    mats = [rng.standard_normal(sh).astype(np.float32) for sh in shapes]
+    As = [torch.from_numpy(m).cuda() for m in mats]
+    outs = pacosvd.alloc_outputs(As)
+    for S, _, _ in outs:
+        S.fill_(float("nan"))
+    torch.cuda.synchronize()
+    say("server cancel", len(shapes))
+    # one batch lane taking one request at a time: the last requests wait in the queue
+    srv = pacosvd.Server(2 << 30, max_batch=1, batch_lanes=1, wide_lanes=1, max_M=256, max_N=64)
+    try:
+        for i, (A, (S, U, Vt)) in enumerate(zip(As, outs)):
+            assert srv.submit(A, S, U, Vt, i) == 0
+        withdrawn = [t for t in range(len(As) - 1, len(As) - 9, -1) if srv.cancel(t)]
+        assert not srv.cancel(10_000)  # never submitted
+        srv.drain()
+        done = {t: (ms, info) for t, ms, info in srv.poll_info()}
+        st = srv.stats()
+        assert not srv.cancel(0)  # finished
+    finally:
+        srv.close()
+    say("withdrawn", withdrawn, st)
+    assert len(withdrawn) >= 1 and st["cancelled"] == len(withdrawn)
+    assert sorted(list(done) + withdrawn) == list(range(len(As)))
+    assert st["completed"] == len(done)
+    torch.cuda.synchronize()
+    for t in withdrawn:
+        assert torch.isnan(outs[t][0]).all(), t
+    for t, (ms, info) in done.items():
+        assert ms >= 0 and info == 0, (t, ms, info)
+        ok, m = good(mats[t], outs[t])
+        assert ok, (t, m)
