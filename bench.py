@@ -34,6 +34,8 @@ import workloads  # noqa: E402  (tests/workloads.py: the configurations' input g
 
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFS = 157.3
+BF16_MFMA_PEAK_TFS = 2500.0  # dense
+BF16X3_PRODUCTS = 6  # BF16 products issued per FP32 product by the three-way split
 
 
 def make_workload(torch, count=10000, seed=3, host_data=None):
@@ -248,18 +250,25 @@ def single_svd(torch, pacosvd, A):
     return d, u, vt, ph
 
 
-def large_phase_figures(M, N, ph):
-    """Roofline figures of a single large SVD's phases."""
+def large_phase_figures(M, N, ph, bf16x3=False):
+    """Roofline figures of a single large SVD's phases; with bf16x3, also the BF16 flops the
+    split issues on the matrix cores (six products per FP32 product) against the BF16 peak."""
     bb = bidiag_bytes(M, N)
     qp_flops = 4.0 * M * M * N - 4.0 * M * N * N + 4.0 * N ** 3 / 3 + 4.0 * N ** 3 / 3
     bt_flops = 2.0 * M * N * N + 2.0 * N ** 3
     t = {k: max(v, 1e-6) / 1000 for k, v in ph.items()}
-    return dict(bidiag_gbs=round(bb / t["bidiag"] / 1e9, 1),
-                bidiag_hbm_frac=round(bb / t["bidiag"] / 1e9 / HBM_PEAK_GBS, 4),
-                form_qp_tflops=round(qp_flops / t["form_qp"] / 1e12, 2),
-                form_qp_mfma_frac=round(qp_flops / t["form_qp"] / 1e12 / FP32_MFMA_PEAK_TFS, 4),
-                backtransform_tflops=round(bt_flops / t["backtransform"] / 1e12, 2),
-                backtransform_mfma_frac=round(bt_flops / t["backtransform"] / 1e12 / FP32_MFMA_PEAK_TFS, 4))
+    out = dict(bidiag_gbs=round(bb / t["bidiag"] / 1e9, 1),
+               bidiag_hbm_frac=round(bb / t["bidiag"] / 1e9 / HBM_PEAK_GBS, 4),
+               form_qp_tflops=round(qp_flops / t["form_qp"] / 1e12, 2),
+               form_qp_mfma_frac=round(qp_flops / t["form_qp"] / 1e12 / FP32_MFMA_PEAK_TFS, 4),
+               backtransform_tflops=round(bt_flops / t["backtransform"] / 1e12, 2),
+               backtransform_mfma_frac=round(bt_flops / t["backtransform"] / 1e12 / FP32_MFMA_PEAK_TFS, 4))
+    if bf16x3:
+        for key, fl in (("form_qp", qp_flops), ("backtransform", bt_flops)):
+            issued = BF16X3_PRODUCTS * fl / t[key] / 1e12
+            out[key + "_bf16_issued_tflops"] = round(issued, 2)
+            out[key + "_bf16_frac"] = round(issued / BF16_MFMA_PEAK_TFS, 4)
+    return out
 
 
 def plain_large_bench(torch, pacosvd, M=4096, N=4096, seed=1):
@@ -311,7 +320,7 @@ def precision_bench(torch, pacosvd, M=8192, N=4096, seed=2):
             m = measures_gpu(torch, A, d, u, vt, s_ref)
             m["wall_ms"] = round(1000 * wall, 2)
             m["phase_ms"] = {k: round(v, 3) for k, v in ph.items()}
-            m.update(large_phase_figures(M, N, ph))
+            m.update(large_phase_figures(M, N, ph, bf16x3=mode == 1))
             res[name] = m
     return res
 
